@@ -1,0 +1,61 @@
+/*
+ * cpu_ref.h -- TEST INFRASTRUCTURE: the oracle.
+ *
+ * A single-threaded FP64 CPU restatement of the reference's CPR-GMRES solve
+ * phase (jxfpamg-0220, one MPI rank, OMP_NUM_THREADS=1).  Every function
+ * names the reference lines it follows and keeps their operation order, so
+ * on the same inputs it reproduces the reference bit for bit.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, as the
+ * checker; the product never links it.
+ */
+#ifndef CPU_REF_H
+#define CPU_REF_H
+
+#define CREF_MAX_LEVELS 25
+
+typedef struct {
+    int n, ncols;
+    const int *ia, *ja;
+    const double *a;
+} cref_csr;
+
+typedef struct {
+    int num_levels;
+    cref_csr A[CREF_MAX_LEVELS]; /* A[0] = pressure matrix */
+    cref_csr P[CREF_MAX_LEVELS]; /* P[l]: n_l x n_{l+1}, l < num_levels-1 */
+} cref_hier;
+
+typedef struct {
+    int nb, b;
+    const int *ia, *ja;
+    const double *a;
+} cref_bsr;
+
+typedef struct {
+    int k_dim, max_iter, is_check_restarted;
+    double tol, a_tol;
+    int pressure_index, stage1_maxit, stage2_maxit;
+    double stage2_weight, stage2_omega;
+} cref_params;
+
+typedef struct {
+    int iterations, converged, num_precond;
+    double final_rel_res, b_norm;
+} cref_stats;
+
+void cref_bsr_matvec(const cref_bsr *A, double alpha, const double *x, double beta, double *y);
+void cref_csr_matvec(const cref_csr *A, double alpha, const double *x, double beta, double *y);
+void cref_csr_matvecT(const cref_csr *A, double alpha, const double *x, double beta, double *y);
+double cref_dot(int n, const double *x, const double *y);
+void cref_relax6(const cref_csr *A, const double *f, double *u);
+int cref_relax9(const cref_csr *A, const double *f, double *u);
+int cref_gselim(double *A, double *x, int n);
+void cref_bsr_block_matvec_stable(double alpha, const double *mat, const double *v, double beta, double *ov, int b);
+void cref_bsr_block_invmatvec_stable(const double *mat, const double *v, double *ov, int b);
+void cref_bsr_hgs(const cref_bsr *A, double *x, const double *rhs, double *vtemp, double w, double omega, int fwd);
+int cref_pamg_solve(const cref_hier *H, const double *f, double *u);
+int cref_cpr_precond(const cref_bsr *A, const cref_hier *H, const cref_params *p, const double *r, double *x);
+int cref_gmres_solve(const cref_bsr *A, const cref_hier *H, const cref_params *p, const double *b, double *x,
+                     double *res_history, cref_stats *st);
+
+#endif
