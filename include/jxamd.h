@@ -59,6 +59,113 @@ int jxamd_gen_sizes(const jxamd_gen_params *p, int *nb, int *nnzb);
 /* fill ia[nb+1], ja[nnzb], a[nnzb*b*b] and (if rhs != NULL) rhs[nb*b] */
 int jxamd_gen_reservoir(const jxamd_gen_params *p, int *ia, int *ja, double *a, double *rhs);
 
+
+/* ------------------------------------------------------------------------
+ * Reference file formats and host setup steps.
+ * ---------------------------------------------------------------------- */
+/* binary BSR: jxf_BSRMatrixPrint_Binary / jxf_BSRMatrixRead_Binary
+ * (jxfpamg-0220/csrc/mat_vec_data/seq_bsr_matrix.c:1935-2019, :2038+) */
+int jxamd_bsr_write_binary(const char *path, int nb, int ncols, int nnzb, int b, const int *ia, const int *ja,
+                           const double *a);
+int jxamd_bsr_read_binary_sizes(const char *path, int *nb, int *ncols, int *nnzb, int *b);
+int jxamd_bsr_read_binary(const char *path, int *ia, int *ja, double *a);
+/* text RHS: read_rhs_file (src/test_bsr_cprgmres.c:26-76) */
+int jxamd_rhs_write_text(const char *path, int n, const double *v);
+int jxamd_rhs_read_text(const char *path, int n, double *v);
+/* ABF decoupling in place, jxf_BSRMatrixDecouple(..., JXF_DECOUP_ABF, 0)
+ * (jxfpamg-0220/csrc/cpr/jxf_bsr_decoup.c:1273-1348); rhs may be NULL */
+int jxamd_bsr_decouple_abf(int nb, int b, const int *ia, const int *ja, double *a, double *rhs);
+/* pressure matrix A_p[i,j] = A_ij[p][p] on A's pattern, jxf_BSRMatrixGetSubmatrix
+ * (jxfpamg-0220/csrc/mat_vec_data/seq_bsr_matrix.c:214-250); p_ia[nb+1], p_ja/p_a[nnzb] */
+int jxamd_bsr_extract_pressure(int nb, int b, const int *ia, const int *ja, const double *a, int pidx, int *p_ia,
+                               int *p_ja, double *p_a);
+
+/* ------------------------------------------------------------------------
+ * Host-side system description (arrays stay owned by the caller).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nb, ncols, nnzb, b;      /* block rows, block columns, stored blocks, block size */
+    const int *ia, *ja;          /* ia[nb+1], ja[nnzb], 0-based, diagonal block first */
+    const double *a;             /* a[nnzb*b*b], row-major blocks */
+} jxamd_bsr;
+
+typedef struct {
+    int n, ncols, nnz;
+    const int *ia, *ja;          /* square level matrices: diagonal stored first */
+    const double *a;
+} jxamd_csr;
+
+/* Pressure-AMG hierarchy imported as built on the host (coarse operators A[l]
+ * and interpolation P[l]: n_l x n_{l+1}; restriction is P^T, as
+ * R_array == P_array in jxfpamg-0220/csrc/amg/par_amg_setup.c:415-425). */
+typedef struct {
+    int num_levels;
+    jxamd_csr A[JXAMD_MAX_LEVELS];
+    jxamd_csr P[JXAMD_MAX_LEVELS];
+} jxamd_hierarchy;
+
+/* Parameters.  CPR keys mirror JXF_CPRSetParameter / JXF_CPRSetRealParameter
+ * (jxfpamg-0220/csrc/cpr/jxf_cpr.c:226-262) plus the stage-2 weights fixed at
+ * jxf_cpr.c:547-549; GMRES keys mirror the JXF_GMRESSet* setters
+ * (jxfpamg-0220/csrc/krylov/pgmres.c:101-306). */
+typedef struct {
+    int k_dim, max_iter, is_check_restarted;
+    double tol, a_tol;
+    int pressure_index, stage1_maxit, stage2_maxit;
+    int stage1_solver_type, stage2_solver_type, print_level;
+    double threshold, stage2_weight, stage2_omega;
+} jxamd_params;
+
+/* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6. */
+void jxamd_params_default(jxamd_params *p);
+
+typedef struct {
+    int iterations;              /* total Arnoldi steps (pgmres.c:1644) */
+    int converged;
+    int num_precond;             /* CPR applications */
+    int num_restarts;
+    double final_rel_res;        /* as JXF_GMRESGetFinalRelativeResidualNorm */
+    double true_rel_res;         /* ||b - A x|| / ||b|| recomputed at exit, FP64 */
+    double b_norm;
+    double t_solve_ms;           /* device time of the whole solve (HIP events) */
+    double t_precond_ms;         /* device time inside CPR applications */
+} jxamd_stats;
+
+typedef struct jxamd_solver_s jxamd_solver;
+
+/* Upload A and the hierarchy into HBM (SELL-64 layouts, level schedules,
+ * coarse inverse).  Replaces the device-side part of JXF_CPRSetup
+ * (jxf_cpr.c:268-422) + JXF_GMRESSetup (pgmres.c:544-622). */
+int jxamd_solver_create(jxamd_solver **s, const jxamd_bsr *A, const jxamd_hierarchy *H, const jxamd_params *p);
+int jxamd_solver_destroy(jxamd_solver *s);
+/* CPR-GMRES solve, host vectors (x in: initial guess, out: solution).
+ * Replaces JXF_GMRESSolve with the CPR callback (pgmres.c:950-1713). */
+int jxamd_solver_solve(jxamd_solver *s, const double *b, double *x, double *res_history, jxamd_stats *st);
+/* Same with b and x already resident in HBM. */
+int jxamd_solver_solve_device(jxamd_solver *s, const double *d_b, double *d_x, double *res_history,
+                              jxamd_stats *st);
+/* x = M^{-1} r, device vectors.  Replaces JXF_CPRPrecond (jxf_cpr.c:424-610). */
+int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x);
+
+/* Single operators on device vectors (parity tests, benchmarks). */
+/* y = alpha*A*x + beta*y, jxf_BSRMatrixMatvec (seq_bsr_matvec.c:21-134) */
+int jxamd_op_bsr_matvec(jxamd_solver *s, double alpha, const double *d_x, double beta, double *d_y);
+/* one relax-6 call (forward + backward GS) on level l, jxf_PAMGRelax6 (par_relax_6.c:224-273) */
+int jxamd_op_hsgs(jxamd_solver *s, int level, const double *d_f, double *d_u);
+/* coarsest-level solve, jxf_PAMGRelax9 (par_relax_9.c:20-142) */
+int jxamd_op_coarse_solve(jxamd_solver *s, const double *d_f, double *d_u);
+/* one V-cycle from u (jxf_PAMGSolve with max_iter 1, par_amg_solve.c:36) */
+int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u);
+/* one block hybrid-SOR sweep, jxf_ParBSRHGSRelax forward (par_relax_bsr.c:193-573) */
+int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, double omega);
+/* y = A*x launched reps times back to back on the solver's stream; the mean
+ * kernel time per launch from HIP events around the whole batch */
+int jxamd_op_bsr_matvec_timed(jxamd_solver *s, const double *d_x, double *d_y, int reps, double *ms_per_launch);
+/* one CPR application r -> x launched reps times; mean device time per apply */
+int jxamd_cpr_precond_timed(jxamd_solver *s, const double *d_r, double *d_x, int reps, double *ms_per_apply);
+/* number of sync-free waits that hit the spin bound since creation (0 when healthy) */
+int jxamd_solver_error_flag(jxamd_solver *s, int *flag);
+
 #ifdef __cplusplus
 }
 #endif

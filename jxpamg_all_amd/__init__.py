@@ -1,0 +1,350 @@
+"""MI355X-native CPR-GMRES solve phase (host bindings).
+
+The product is the C-ABI shared library ``libjxamd.so`` (HIP kernels for
+gfx950 + host C), declared in ``include/jxamd.h``.  This module loads it with
+ctypes and offers thin Python wrappers used by the tests and the benchmark.
+Device vectors are passed as raw pointers (``torch.Tensor.data_ptr()``);
+torch only provides device memory.
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjxamd.so")
+MAX_LEVELS = 25
+
+SUCCESS, ERROR_GENERIC, ERROR_MEMORY, ERROR_ARG, ERROR_CONV, ERROR_DEVICE = 0, 1, 2, 4, 256, 512
+FIELD_IID, FIELD_SMOOTH, FIELD_LAYERED = 0, 1, 2
+
+
+class GenParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("block_size", ctypes.c_int),
+                ("field", ctypes.c_int), ("sigma_lnk", ctypes.c_double), ("corr_len", ctypes.c_double),
+                ("n_smooth_layers", ctypes.c_int), ("channel_ratio", ctypes.c_double),
+                ("kz_ratio", ctypes.c_double), ("n_wells", ctypes.c_int), ("well_boost", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+class BSR(ctypes.Structure):
+    _fields_ = [("nb", ctypes.c_int), ("ncols", ctypes.c_int), ("nnzb", ctypes.c_int), ("b", ctypes.c_int),
+                ("ia", ctypes.c_void_p), ("ja", ctypes.c_void_p), ("a", ctypes.c_void_p)]
+
+
+class CSR(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("ncols", ctypes.c_int), ("nnz", ctypes.c_int),
+                ("ia", ctypes.c_void_p), ("ja", ctypes.c_void_p), ("a", ctypes.c_void_p)]
+
+
+class HierarchyC(ctypes.Structure):
+    _fields_ = [("num_levels", ctypes.c_int), ("A", CSR * MAX_LEVELS), ("P", CSR * MAX_LEVELS)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("k_dim", ctypes.c_int), ("max_iter", ctypes.c_int), ("is_check_restarted", ctypes.c_int),
+                ("tol", ctypes.c_double), ("a_tol", ctypes.c_double),
+                ("pressure_index", ctypes.c_int), ("stage1_maxit", ctypes.c_int), ("stage2_maxit", ctypes.c_int),
+                ("stage1_solver_type", ctypes.c_int), ("stage2_solver_type", ctypes.c_int),
+                ("print_level", ctypes.c_int),
+                ("threshold", ctypes.c_double), ("stage2_weight", ctypes.c_double),
+                ("stage2_omega", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("num_precond", ctypes.c_int),
+                ("num_restarts", ctypes.c_int), ("final_rel_res", ctypes.c_double),
+                ("true_rel_res", ctypes.c_double), ("b_norm", ctypes.c_double),
+                ("t_solve_ms", ctypes.c_double), ("t_precond_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libjxamd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (no CPU fallback)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp, i32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    sigs = {
+        "jxamd_gen_sizes": [vp, vp, vp],
+        "jxamd_gen_reservoir": [vp, vp, vp, vp, vp],
+        "jxamd_bsr_decouple_abf": [i32, i32, vp, vp, vp, vp],
+        "jxamd_bsr_extract_pressure": [i32, i32, vp, vp, vp, i32, vp, vp, vp],
+        "jxamd_bsr_write_binary": [ctypes.c_char_p, i32, i32, i32, i32, vp, vp, vp],
+        "jxamd_bsr_read_binary_sizes": [ctypes.c_char_p, vp, vp, vp, vp],
+        "jxamd_bsr_read_binary": [ctypes.c_char_p, vp, vp, vp],
+        "jxamd_rhs_write_text": [ctypes.c_char_p, i32, vp],
+        "jxamd_rhs_read_text": [ctypes.c_char_p, i32, vp],
+        "jxamd_params_default": [vp],
+        "jxamd_solver_create": [vp, vp, vp, vp],
+        "jxamd_solver_destroy": [vp],
+        "jxamd_solver_solve": [vp, vp, vp, vp, vp],
+        "jxamd_solver_solve_device": [vp, vp, vp, vp, vp],
+        "jxamd_cpr_precond_device": [vp, vp, vp],
+        "jxamd_op_bsr_matvec": [vp, f64, vp, f64, vp],
+        "jxamd_op_hsgs": [vp, i32, vp, vp],
+        "jxamd_op_coarse_solve": [vp, vp, vp],
+        "jxamd_op_vcycle": [vp, vp, vp],
+        "jxamd_op_bhgs": [vp, vp, vp, f64, f64],
+        "jxamd_solver_error_flag": [vp, vp],
+        "jxamd_op_bsr_matvec_timed": [vp, vp, vp, i32, vp],
+        "jxamd_cpr_precond_timed": [vp, vp, vp, i32, vp],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = None if name == "jxamd_params_default" else ctypes.c_int
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with error flag {rc}")
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+CONFIGS = {
+    # 16x16x8, 3x3 FP64, ln k ~ N(0, 1.5^2) iid
+    "C1": dict(nx=16, ny=16, nz=8, block_size=3, field=FIELD_IID, sigma_lnk=1.5, corr_len=0.0,
+               n_smooth_layers=0, channel_ratio=1.0, kz_ratio=0.1, n_wells=0, well_boost=0.0, seed=1),
+    # 60x220x85: top 35 layers smooth lognormal (sigma 1, corr 8), bottom 50 sinuous channels (k ratio 1e3),
+    # 5 well columns boosting the pressure diagonal by 1e4 x mean T
+    "C2": dict(nx=60, ny=220, nz=85, block_size=3, field=FIELD_LAYERED, sigma_lnk=1.0, corr_len=8.0,
+               n_smooth_layers=35, channel_ratio=1e3, kz_ratio=0.1, n_wells=5, well_boost=1e4, seed=2),
+    # 100^3, 4x4 blocks, smooth lognormal sigma 2, corr 5
+    "C4": dict(nx=100, ny=100, nz=100, block_size=4, field=FIELD_SMOOTH, sigma_lnk=2.0, corr_len=5.0,
+               n_smooth_layers=0, channel_ratio=1.0, kz_ratio=0.1, n_wells=0, well_boost=0.0, seed=4),
+    # 120x440x85 layered (C2 style, 4x larger)
+    "C5": dict(nx=120, ny=440, nz=85, block_size=3, field=FIELD_LAYERED, sigma_lnk=1.0, corr_len=8.0,
+               n_smooth_layers=35, channel_ratio=1e3, kz_ratio=0.1, n_wells=5, well_boost=1e4, seed=2001),
+}
+
+
+def gen_params(**kw):
+    d = dict(CONFIGS["C1"])
+    d.update(kw)
+    return GenParams(**d)
+
+
+class BSRMatrix:
+    """Host BSR system (0-based int32 ia/ja, row-major b x b float64 blocks)."""
+
+    def __init__(self, nb, b, ia, ja, a):
+        self.nb, self.b = int(nb), int(b)
+        self.ia = np.ascontiguousarray(ia, dtype=np.int32)
+        self.ja = np.ascontiguousarray(ja, dtype=np.int32)
+        self.a = np.ascontiguousarray(a, dtype=np.float64)
+        self.nnzb = int(self.ia[-1])
+
+    @property
+    def n(self):
+        return self.nb * self.b
+
+    def c_struct(self):
+        return BSR(self.nb, self.nb, self.nnzb, self.b, _p(self.ia).value, _p(self.ja).value, _p(self.a).value)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        b = self.b
+        return sp.bsr_matrix((self.a.reshape(-1, b, b), self.ja, self.ia), shape=(self.n, self.n)).tocsr()
+
+
+def generate(**kw):
+    """Seeded synthetic reservoir system -> (BSRMatrix, rhs)."""
+    L = lib()
+    p = gen_params(**kw)
+    nb, nnzb = ctypes.c_int(), ctypes.c_int()
+    check(L.jxamd_gen_sizes(ctypes.byref(p), ctypes.byref(nb), ctypes.byref(nnzb)), "jxamd_gen_sizes")
+    b = p.block_size
+    ia = np.zeros(nb.value + 1, np.int32)
+    ja = np.zeros(nnzb.value, np.int32)
+    a = np.zeros(nnzb.value * b * b)
+    rhs = np.zeros(nb.value * b)
+    check(L.jxamd_gen_reservoir(ctypes.byref(p), _p(ia), _p(ja), _p(a), _p(rhs)), "jxamd_gen_reservoir")
+    return BSRMatrix(nb.value, b, ia, ja, a), rhs
+
+
+def decouple_abf(A, rhs=None):
+    """ABF decoupling (copy); returns (A_dec, rhs_dec)."""
+    a = A.a.copy()
+    r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).copy()
+    check(lib().jxamd_bsr_decouple_abf(A.nb, A.b, _p(A.ia), _p(A.ja), _p(a), None if r is None else _p(r)),
+          "jxamd_bsr_decouple_abf")
+    return BSRMatrix(A.nb, A.b, A.ia, A.ja, a), r
+
+
+def extract_pressure(A, pidx=0):
+    """(ia, ja, a) of the pressure matrix A_p."""
+    ia = np.zeros(A.nb + 1, np.int32)
+    ja = np.zeros(A.nnzb, np.int32)
+    a = np.zeros(A.nnzb)
+    check(lib().jxamd_bsr_extract_pressure(A.nb, A.b, _p(A.ia), _p(A.ja), _p(A.a), pidx, _p(ia), _p(ja), _p(a)),
+          "jxamd_bsr_extract_pressure")
+    return ia, ja, a
+
+
+def write_bsr_binary(path, A):
+    check(lib().jxamd_bsr_write_binary(path.encode(), A.nb, A.nb, A.nnzb, A.b, _p(A.ia), _p(A.ja), _p(A.a)),
+          "jxamd_bsr_write_binary")
+
+
+def read_bsr_binary(path):
+    L = lib()
+    nb, nc, nnzb, b = (ctypes.c_int() for _ in range(4))
+    check(L.jxamd_bsr_read_binary_sizes(path.encode(), ctypes.byref(nb), ctypes.byref(nc), ctypes.byref(nnzb),
+                                        ctypes.byref(b)), "jxamd_bsr_read_binary_sizes")
+    ia = np.zeros(nb.value + 1, np.int32)
+    ja = np.zeros(nnzb.value, np.int32)
+    a = np.zeros(nnzb.value * b.value * b.value)
+    check(L.jxamd_bsr_read_binary(path.encode(), _p(ia), _p(ja), _p(a)), "jxamd_bsr_read_binary")
+    return BSRMatrix(nb.value, b.value, ia, ja, a)
+
+
+def write_rhs_text(path, v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    check(lib().jxamd_rhs_write_text(path.encode(), len(v), _p(v)), "jxamd_rhs_write_text")
+
+
+def read_rhs_text(path, n):
+    v = np.zeros(n)
+    check(lib().jxamd_rhs_read_text(path.encode(), n, _p(v)), "jxamd_rhs_read_text")
+    return v
+
+
+class Hierarchy:
+    """Pressure-AMG hierarchy: lists of (ia, ja, a, ncols) for A_l and P_l."""
+
+    def __init__(self, A_levels, P_levels):
+        self.A = [tuple(np.ascontiguousarray(x, dt) for x, dt in zip(m[:3], (np.int32, np.int32, np.float64)))
+                  + (int(m[3]),) for m in A_levels]
+        self.P = [tuple(np.ascontiguousarray(x, dt) for x, dt in zip(m[:3], (np.int32, np.int32, np.float64)))
+                  + (int(m[3]),) for m in P_levels]
+
+    @property
+    def num_levels(self):
+        return len(self.A)
+
+    def c_struct(self):
+        h = HierarchyC()
+        h.num_levels = len(self.A)
+        for l, (ia, ja, a, nc) in enumerate(self.A):
+            h.A[l] = CSR(len(ia) - 1, nc, int(ia[-1]), _p(ia).value, _p(ja).value, _p(a).value)
+        for l, (ia, ja, a, nc) in enumerate(self.P):
+            h.P[l] = CSR(len(ia) - 1, nc, int(ia[-1]), _p(ia).value, _p(ja).value, _p(a).value)
+        return h
+
+    @classmethod
+    def from_fixture(cls, d):
+        L = int(d["num_levels"][0])
+        As, Ps = [], []
+        for l in range(L):
+            n, m, nnz = d[f"A{l}_shape"]
+            As.append((d[f"A{l}_ia"], d[f"A{l}_ja"], d[f"A{l}_a"], m))
+            if l < L - 1:
+                n, m, nnz = d[f"P{l}_shape"]
+                Ps.append((d[f"P{l}_ia"], d[f"P{l}_ja"], d[f"P{l}_a"], m))
+        return cls(As, Ps)
+
+
+def default_params(**kw):
+    p = Params()
+    lib().jxamd_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Solver:
+    """Device-resident CPR-GMRES solver (jxamd_solver_*)."""
+
+    def __init__(self, A, H, params=None):
+        self._L = lib()
+        self.A, self.H = A, H
+        self.params = params if params is not None else default_params()
+        self._a = A.c_struct()
+        self._h = H.c_struct()
+        self._s = ctypes.c_void_p()
+        check(self._L.jxamd_solver_create(ctypes.byref(self._s), ctypes.byref(self._a), ctypes.byref(self._h),
+                                          ctypes.byref(self.params)), "jxamd_solver_create")
+
+    def close(self):
+        if self._s:
+            self._L.jxamd_solver_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, b, x0=None, history=True):
+        x = np.zeros(self.A.n) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        hist = np.zeros(self.params.max_iter + 1)
+        st = Stats()
+        rc = self._L.jxamd_solver_solve(self._s, _p(b), _p(x), _p(hist) if history else None, ctypes.byref(st))
+        check(rc, "jxamd_solver_solve")
+        return x, st, hist[: st.iterations + 1]
+
+    def solve_device(self, d_b, d_x, hist=None):
+        st = Stats()
+        rc = self._L.jxamd_solver_solve_device(self._s, ctypes.c_void_p(d_b), ctypes.c_void_p(d_x),
+                                               None if hist is None else _p(hist), ctypes.byref(st))
+        check(rc, "jxamd_solver_solve_device")
+        return st
+
+    def precond(self, d_r, d_x):
+        check(self._L.jxamd_cpr_precond_device(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_x)),
+              "jxamd_cpr_precond_device")
+
+    def bsr_matvec(self, alpha, d_x, beta, d_y):
+        check(self._L.jxamd_op_bsr_matvec(self._s, alpha, ctypes.c_void_p(d_x), beta, ctypes.c_void_p(d_y)),
+              "jxamd_op_bsr_matvec")
+
+    def hsgs(self, level, d_f, d_u):
+        check(self._L.jxamd_op_hsgs(self._s, level, ctypes.c_void_p(d_f), ctypes.c_void_p(d_u)), "jxamd_op_hsgs")
+
+    def coarse_solve(self, d_f, d_u):
+        check(self._L.jxamd_op_coarse_solve(self._s, ctypes.c_void_p(d_f), ctypes.c_void_p(d_u)),
+              "jxamd_op_coarse_solve")
+
+    def vcycle(self, d_f, d_u):
+        check(self._L.jxamd_op_vcycle(self._s, ctypes.c_void_p(d_f), ctypes.c_void_p(d_u)), "jxamd_op_vcycle")
+
+    def bhgs(self, d_x, d_b, w, omega):
+        check(self._L.jxamd_op_bhgs(self._s, ctypes.c_void_p(d_x), ctypes.c_void_p(d_b), w, omega),
+              "jxamd_op_bhgs")
+
+    def bsr_matvec_timed(self, d_x, d_y, reps):
+        ms = ctypes.c_double()
+        check(self._L.jxamd_op_bsr_matvec_timed(self._s, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), reps,
+                                                ctypes.byref(ms)), "jxamd_op_bsr_matvec_timed")
+        return ms.value
+
+    def precond_timed(self, d_r, d_x, reps):
+        ms = ctypes.c_double()
+        check(self._L.jxamd_cpr_precond_timed(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_x), reps,
+                                              ctypes.byref(ms)), "jxamd_cpr_precond_timed")
+        return ms.value
+
+    def error_flag(self):
+        f = ctypes.c_int()
+        check(self._L.jxamd_solver_error_flag(self._s, ctypes.byref(f)), "jxamd_solver_error_flag")
+        return f.value

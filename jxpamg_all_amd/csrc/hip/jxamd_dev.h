@@ -1,0 +1,51 @@
+/*
+ * jxamd_dev.h -- internal C interface between the host solver (C) and the
+ * HIP kernels (jxamd_kernels.hip).  All pointers are device pointers unless
+ * named otherwise; `stream` is a hipStream_t passed as void*.
+ */
+#ifndef JXAMD_DEV_H
+#define JXAMD_DEV_H
+
+#include <stddef.h>
+
+#include "jxamd.h"
+
+int jxamd_dev_count(int *n);
+int jxamd_dev_malloc(void **p, size_t bytes);
+int jxamd_dev_free(void *p);
+int jxamd_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int jxamd_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int jxamd_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int jxamd_stream_create(void **s);
+int jxamd_stream_destroy(void *s);
+int jxamd_stream_sync(void *s);
+int jxamd_event_create(void **e);
+int jxamd_event_destroy(void *e);
+int jxamd_event_record(void *e, void *s);
+int jxamd_event_elapsed(float *ms, void *e0, void *e1);
+int jxamd_last_error(void);
+
+int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
+                     const double *rhs, double *y, int mode, double alpha, double beta, void *stream);
+int jxamd_k_csr_spmv(int n, const int *sptr, const int *col, const double *val, const double *x, int xs,
+                     const double *rhs, int rs, double *y, int mode, void *stream);
+int jxamd_k_gs_sweep(int n, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
+                     const double *f, int fs, const double *uo, double *un, int *flag, int epoch, int dir,
+                     unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
+int jxamd_k_bhgs_sweep(int b, int nb, const int *perm, const int *sptr, const int *col, const double *val,
+                       const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
+                       double *xn, int *flag, int epoch, double w, double omega, unsigned long long *ticket,
+                       unsigned long long *base, int *err, void *stream);
+int jxamd_k_dense_gemv(int n, const double *M, const double *x, double *y, void *stream);
+int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream);
+int jxamd_k_prolong(int nb, int b, int pidx, const double *xp, double *x, void *stream);
+int jxamd_k_fill(size_t n, double *y, double v, void *stream);
+int jxamd_k_copy(size_t n, const double *x, double *y, void *stream);
+int jxamd_k_axpy(size_t n, const double *pa, double a, int neg, const double *x, double *y, void *stream);
+int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *stream);
+int jxamd_dot_grid(size_t n);
+int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
+                    unsigned int *count, double *out, void *stream);
+int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream);
+
+#endif

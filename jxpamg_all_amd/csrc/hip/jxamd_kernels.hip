@@ -1,0 +1,616 @@
+// jxamd_kernels.hip -- CDNA4 (gfx950) kernels of the CPR-GMRES solve phase and
+// their extern "C" launchers.  Host C code drives them through jxamd_dev.h.
+//
+// Device layouts (built on the host by jxamd_solver.c):
+//   SELL-64: rows are grouped in slices of 64 (one wavefront); slice s owns
+//   slot rows [sptr[s], sptr[s+1]); slot row k holds the k-th stored entry of
+//   each of the 64 rows, lane-contiguous, so a wavefront's loads of one slot
+//   are one 512-byte (fp64) transaction.  Short rows are padded with the
+//   row's own column and a zero value.  Block (BSR) values keep b*b planes
+//   per slot row: val[(slot*b*b + e)*64 + lane].
+//   Smoother matrices are stored in sweep order: position p holds row perm[p];
+//   positions are sorted by dependency level so a wavefront's rows become
+//   ready together.  The diagonal is kept apart (diag[p]).
+//
+// Sync-free sweeps: a row's new value is published with a release store of
+// the sweep's epoch into flag[row]; consumers poll with relaxed agent-scope
+// loads and take an acquire fence once all their dependencies are present.
+// Workgroups claim position tiles from a monotonically increasing ticket
+// counter, so every tile's dependencies belong to tiles already claimed by
+// running workgroups; each wavefront retries its unfinished lanes (no lane
+// blocks the wave), and every wait is bounded: on overflow the kernel records
+// JXAMD_ERROR_GENERIC in err[0] and finishes.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WAVE 64
+#define SPIN_LIMIT (1 << 24)
+
+extern "C" {
+#include "jxamd_dev.h"
+}
+
+namespace {
+
+__device__ __forceinline__ int ld_flag(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_flag_release(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_coherent(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool flag_ready(int f, int epoch) { return (int)(f - epoch) >= 0; }
+
+// XCD-aware tile order for streaming kernels: consecutive tiles land on the
+// same XCD (dispatch is round-robin over the 8 XCDs), so the x entries a
+// stencil row shares with its neighbours stay in one L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks)
+{
+    const int nx = 8;
+    int per = (nblocks + nx - 1) / nx;
+    int xcd = bid % nx, k = bid / nx;
+    int t = xcd * per + k;
+    // the last XCD may have fewer tiles; fall back to identity for the tail
+    return (nblocks % nx == 0) ? t : bid;
+}
+
+// ---------------------------------------------------------------------------
+// BSR SpMV, SELL-64, one lane per block row.
+//   mode 0: y = A x;  mode 1: y = rhs - A x;  mode 2: y = alpha*A x + beta*y
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                  const double *__restrict__ val, const double *__restrict__ x,
+                                                  const double *__restrict__ rhs, double *__restrict__ y, int mode,
+                                                  double alpha, double beta)
+{
+    const int nblk = gridDim.x;
+    const int tile = xcd_tile(blockIdx.x, nblk);
+    const int s = tile * (blockDim.x / WAVE) + threadIdx.x / WAVE; // slice
+    const int lane = threadIdx.x % WAVE;
+    const int row = s * WAVE + lane;
+    const int nsl = (nb + WAVE - 1) / WAVE;
+    if (s >= nsl) return;
+    const int k0 = sptr[s], k1 = sptr[s + 1];
+    double acc[B];
+#pragma unroll
+    for (int r = 0; r < B; r++) acc[r] = 0.0;
+    for (int k = k0; k < k1; k++) {
+        const int c = col[(size_t)k * WAVE + lane];
+        double xv[B];
+#pragma unroll
+        for (int t = 0; t < B; t++) xv[t] = x[(size_t)c * B + t];
+        const double *v = val + (size_t)k * B * B * WAVE + lane;
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+#pragma unroll
+            for (int t = 0; t < B; t++) acc[r] += v[(r * B + t) * WAVE] * xv[t];
+        }
+    }
+    if (row >= nb) return;
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+        const size_t o = (size_t)row * B + r;
+        double out;
+        if (mode == 0) out = acc[r];
+        else if (mode == 1) out = rhs[o] - acc[r];
+        else out = alpha * acc[r] + (beta == 0.0 ? 0.0 : beta * y[o]);
+        y[o] = out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scalar CSR SpMV, SELL-64 (levels: A_l residual, R_l restriction, P_l
+// prolongation).  mode 0: y = A x; 1: y = rhs - A x; 2: y += A x.
+// x and rhs may be strided views (xs, rs) so the pressure block of a BSR
+// vector is read in place.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                  const double *__restrict__ val, const double *__restrict__ x, int xs,
+                                                  const double *__restrict__ rhs, int rs, double *__restrict__ y, int mode)
+{
+    const int s = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nsl = (n + WAVE - 1) / WAVE;
+    if (s >= nsl) return;
+    const int row = s * WAVE + lane;
+    double acc = 0.0;
+    for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+        const size_t e = (size_t)k * WAVE + lane;
+        acc += val[e] * x[(size_t)col[e] * xs];
+    }
+    if (row >= n) return;
+    if (mode == 0) y[row] = acc;
+    else if (mode == 1) y[row] = rhs[(size_t)row * rs] - acc;
+    else y[row] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// One Gauss-Seidel sweep of hybrid SSOR (relax 6, weights 1), sync-free.
+//   dir > 0: forward (dependencies j < i), dir < 0: backward (j > i).
+//   Reads old values from uo (or zeros if uo == nullptr), writes un.
+//   f may be strided (fs).  Rows with a zero diagonal copy their old value.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gs_sweep(int n, const int *__restrict__ perm, const int *__restrict__ sptr,
+                                                  const int *__restrict__ col, const double *__restrict__ val,
+                                                  const double *__restrict__ diag, const double *__restrict__ f, int fs,
+                                                  const double *uo, double *un, int *flag, int epoch, int dir,
+                                                  unsigned long long *ticket, unsigned long long base, int *err)
+{
+    __shared__ int s_tile;
+    if (threadIdx.x == 0)
+        s_tile = (int)(__hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
+    __syncthreads();
+    const int s = s_tile * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nsl = (n + WAVE - 1) / WAVE;
+    if (s_tile < 0 || s >= nsl) return;
+    const int p = s * WAVE + lane;
+    const bool active = p < n;
+    const int i = active ? perm[p] : 0;
+    const int k0 = sptr[s], k1 = sptr[s + 1];
+    bool done = !active;
+    int spins = 0;
+    while (true) {
+        if (!done) {
+            bool ready = true;
+            for (int k = k0; k < k1 && ready; k++) {
+                const int j = col[(size_t)k * WAVE + lane];
+                const bool dep = dir > 0 ? (j < i) : (j > i);
+                if (dep && !flag_ready(ld_flag(flag + j), epoch)) ready = false;
+            }
+            if (ready) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const double d = diag[p];
+                double res = f[(size_t)i * fs];
+                double nv;
+                if (d != 0.0) {
+                    for (int k = k0; k < k1; k++) {
+                        const size_t e = (size_t)k * WAVE + lane;
+                        const int j = col[e];
+                        if (j == i) continue; // padding slot
+                        const bool dep = dir > 0 ? (j < i) : (j > i);
+                        const double uj = dep ? ld_coherent(un + j) : (uo ? uo[j] : 0.0);
+                        res -= val[e] * uj;
+                    }
+                    nv = res / d;
+                } else {
+                    nv = uo ? uo[i] : 0.0;
+                }
+                st_coherent(un + i, nv);
+                st_flag_release(flag + i, epoch);
+                done = true;
+            }
+        }
+        if (__all(done)) break;
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One forward block hybrid SOR sweep on the BSR matrix (par_relax_bsr.c
+// semantics, one rank), sync-free over block rows:
+//   tmp = omega*b_i - sum_{j!=i} A_ij x_j(cur) + (1-omega) sum_{j!=i} A_ij x_j(old)
+//   x_i <- (1 - w*omega) x_i(old) + w * Dinv_i tmp
+// Old values come from xo, or, when xo == nullptr, from the prolongated
+// pressure vector xp (pressure slot pidx, zeros elsewhere).  Writes xn.
+// Dependencies are the lower block columns (j < i).
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ void __launch_bounds__(256) k_bhgs_sweep(int nb, const int *__restrict__ perm, const int *__restrict__ sptr,
+                                                    const int *__restrict__ col, const double *__restrict__ val,
+                                                    const double *__restrict__ dinv, const double *__restrict__ rhs,
+                                                    const double *xo, const double *__restrict__ xp, int pidx,
+                                                    double *xn, int *flag, int epoch, double w, double omega,
+                                                    unsigned long long *ticket, unsigned long long base, int *err)
+{
+    __shared__ int s_tile;
+    if (threadIdx.x == 0)
+        s_tile = (int)(__hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
+    __syncthreads();
+    const int s = s_tile * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nsl = (nb + WAVE - 1) / WAVE;
+    if (s_tile < 0 || s >= nsl) return;
+    const int p = s * WAVE + lane;
+    const bool active = p < nb;
+    const int i = active ? perm[p] : 0;
+    const int k0 = sptr[s], k1 = sptr[s + 1];
+    bool done = !active;
+    int spins = 0;
+    while (true) {
+        if (!done) {
+            bool ready = true;
+            for (int k = k0; k < k1 && ready; k++) {
+                const int j = col[(size_t)k * WAVE + lane];
+                if (j < i && !flag_ready(ld_flag(flag + j), epoch)) ready = false;
+            }
+            if (ready) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                double res0[B], res2[B];
+#pragma unroll
+                for (int r = 0; r < B; r++) { res0[r] = 0.0; res2[r] = 0.0; }
+                for (int k = k0; k < k1; k++) {
+                    const int j = col[(size_t)k * WAVE + lane];
+                    if (j == i) continue; // padding slot
+                    double xold[B], xcur[B];
+#pragma unroll
+                    for (int t = 0; t < B; t++) {
+                        xold[t] = xo ? xo[(size_t)j * B + t] : (t == pidx ? xp[j] : 0.0);
+                        xcur[t] = (j < i) ? ld_coherent(xn + (size_t)j * B + t) : xold[t];
+                    }
+                    const double *v = val + (size_t)k * B * B * WAVE + lane;
+#pragma unroll
+                    for (int r = 0; r < B; r++) {
+#pragma unroll
+                        for (int t = 0; t < B; t++) {
+                            const double a = v[(r * B + t) * WAVE];
+                            res0[r] -= a * xcur[t];
+                            res2[r] += a * xold[t];
+                        }
+                    }
+                }
+                double tmp[B];
+#pragma unroll
+                for (int r = 0; r < B; r++)
+                    tmp[r] = omega * rhs[(size_t)i * B + r] + res0[r] + (1.0 - omega) * res2[r];
+                const double prod = 1.0 - w * omega;
+                const double *D = dinv + (size_t)i * B * B;
+#pragma unroll
+                for (int r = 0; r < B; r++) {
+                    double o = 0.0;
+#pragma unroll
+                    for (int t = 0; t < B; t++) o += D[r * B + t] * tmp[t];
+                    const double xi = xo ? xo[(size_t)i * B + r] : (r == pidx ? xp[i] : 0.0);
+                    st_coherent(xn + (size_t)i * B + r, xi * prod + w * o);
+                }
+                st_flag_release(flag + i, epoch);
+                done = true;
+            }
+        }
+        if (__all(done)) break;
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// dense y = M x (row-major n x n), one wavefront per row
+__global__ void __launch_bounds__(256) k_dense_gemv(int n, const double *__restrict__ M, const double *__restrict__ x,
+                                                    double *__restrict__ y)
+{
+    const int row = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int c = lane; c < n; c += WAVE) acc += M[(size_t)row * n + c] * x[c];
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) y[row] = acc;
+}
+
+// y[i] = x[i*stride + off]
+__global__ void k_gather_stride(int n, const double *__restrict__ x, int stride, int off, double *__restrict__ y)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[(size_t)i * stride + off];
+}
+
+// x[i*b + j] = (j == pidx) ? xp[i] : 0   (CPR prolongation type 0)
+__global__ void k_prolong(int nb, int b, int pidx, const double *__restrict__ xp, double *__restrict__ x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    for (int j = 0; j < b; j++) x[(size_t)i * b + j] = (j == pidx) ? xp[i] : 0.0;
+}
+
+__global__ void k_fill(size_t n, double *__restrict__ y, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = v;
+}
+
+__global__ void k_copy(size_t n, const double *__restrict__ x, double *__restrict__ y)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+// y = a*x + y with a read from device memory (*pa, negated if neg) or given
+__global__ void k_axpy(size_t n, const double *__restrict__ pa, double a, int neg, const double *__restrict__ x,
+                       double *__restrict__ y)
+{
+    double s = pa ? *pa : a;
+    if (neg) s = -s;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += s * x[i];
+}
+
+// y = s*y, s = 1/sqrt(*pnorm2) if pnorm2 (zero norm leaves y), else s = a
+__global__ void k_scale(size_t n, const double *__restrict__ pnorm2, double a, double *__restrict__ y)
+{
+    double s = a;
+    if (pnorm2) {
+        double t = sqrt(*pnorm2);
+        if (t == 0.0) return;
+        s = 1.0 / t;
+    }
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] *= s;
+}
+
+// Deterministic dot product with an optional fused MGS update:
+//   if (v_prev): y -= (*h_prev) * v_prev   (h_prev from device memory)
+//   out = <u, y>  (u == y gives the squared norm)
+// Block partials land in part[]; the last block to finish sums them in a
+// fixed order and writes *out, then resets the counter.
+__global__ void __launch_bounds__(256) k_dot_mgs(size_t n, const double *u, double *y,
+                                                 const double *__restrict__ v_prev, const double *__restrict__ h_prev,
+                                                 double *part, unsigned int *count, double *out)
+{
+    __shared__ double sm[256 / WAVE];
+    __shared__ bool last;
+    const double h = v_prev ? *h_prev : 0.0;
+    double acc = 0.0;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    if (v_prev) {
+        for (; i < n; i += stride) {
+            double yi = y[i] - h * v_prev[i];
+            y[i] = yi;
+            acc += u[i] * yi;
+        }
+    } else {
+        for (; i < n; i += stride) acc += u[i] * y[i];
+    }
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (threadIdx.x % WAVE == 0) sm[threadIdx.x / WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / WAVE); w++) b += sm[w];
+        __hip_atomic_store(part + blockIdx.x, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned int prev = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    if (threadIdx.x < WAVE) {
+        double a2 = 0.0;
+        for (unsigned int b = threadIdx.x; b < gridDim.x; b += WAVE)
+            a2 += __hip_atomic_load(part + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) a2 += __shfl_xor(a2, off);
+        if (threadIdx.x == 0) {
+            *out = a2;
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// w = sum_j y[j] * V[j]   (j = 0..m-1, V rows of length n, ld apart), y on host
+__global__ void k_lincomb(size_t n, int m, const double *__restrict__ V, size_t ld, const double *__restrict__ y,
+                          double *__restrict__ w)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double acc = y[m - 1] * V[(size_t)(m - 1) * ld + i];
+        for (int j = m - 2; j >= 0; j--) acc += y[j] * V[(size_t)j * ld + i];
+        w[i] = acc;
+    }
+}
+
+inline int nblocks_stream(size_t n, int bs) {
+    size_t b = (n + bs - 1) / bs;
+    if (b > 4096) b = 4096;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+} // namespace
+
+// ===========================================================================
+// extern "C" launchers (device pointers; stream is a hipStream_t)
+// ===========================================================================
+extern "C" {
+
+static int chk(hipError_t e) { return e == hipSuccess ? 0 : JXAMD_ERROR_DEVICE; }
+
+int jxamd_dev_count(int *n)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return 0;
+}
+int jxamd_dev_malloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    return chk(hipMalloc(p, bytes));
+}
+int jxamd_dev_free(void *p) { return p ? chk(hipFree(p)) : 0; }
+int jxamd_h2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    return chk(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+}
+int jxamd_d2h(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    return chk(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+}
+int jxamd_d2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    return chk(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+}
+int jxamd_stream_create(void **s)
+{
+    hipStream_t st;
+    int e = chk(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = (void *)st;
+    return e;
+}
+int jxamd_stream_destroy(void *s) { return s ? chk(hipStreamDestroy((hipStream_t)s)) : 0; }
+int jxamd_stream_sync(void *s) { return chk(hipStreamSynchronize((hipStream_t)s)); }
+int jxamd_event_create(void **e)
+{
+    hipEvent_t ev;
+    int r = chk(hipEventCreate(&ev));
+    *e = (void *)ev;
+    return r;
+}
+int jxamd_event_destroy(void *e) { return e ? chk(hipEventDestroy((hipEvent_t)e)) : 0; }
+int jxamd_event_record(void *e, void *s) { return chk(hipEventRecord((hipEvent_t)e, (hipStream_t)s)); }
+int jxamd_event_elapsed(float *ms, void *e0, void *e1)
+{
+    return chk(hipEventElapsedTime(ms, (hipEvent_t)e0, (hipEvent_t)e1));
+}
+int jxamd_last_error(void) { return chk(hipGetLastError()); }
+
+int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
+                     const double *rhs, double *y, int mode, double alpha, double beta, void *stream)
+{
+    const int nsl = (nb + WAVE - 1) / WAVE;
+    const int grid = (nsl + 3) / 4;
+    if (grid == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    switch (b) {
+    case 2: k_bsr_spmv<2><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
+    case 3: k_bsr_spmv<3><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
+    case 4: k_bsr_spmv<4><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
+    default: return JXAMD_ERROR_ARG;
+    }
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_csr_spmv(int n, const int *sptr, const int *col, const double *val, const double *x, int xs,
+                     const double *rhs, int rs, double *y, int mode, void *stream)
+{
+    const int nsl = (n + WAVE - 1) / WAVE;
+    const int grid = (nsl + 3) / 4;
+    if (grid == 0) return 0;
+    k_csr_spmv<<<grid, 256, 0, (hipStream_t)stream>>>(n, sptr, col, val, x, xs, rhs, rs, y, mode);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_gs_sweep(int n, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
+                     const double *f, int fs, const double *uo, double *un, int *flag, int epoch, int dir,
+                     unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+{
+    const int nsl = (n + WAVE - 1) / WAVE;
+    const int grid = (nsl + 3) / 4;
+    if (grid == 0) return 0;
+    k_gs_sweep<<<grid, 256, 0, (hipStream_t)stream>>>(n, perm, sptr, col, val, diag, f, fs, uo, un, flag, epoch, dir,
+                                                      ticket, *base, err);
+    *base += (unsigned long long)grid;
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_bhgs_sweep(int b, int nb, const int *perm, const int *sptr, const int *col, const double *val,
+                       const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
+                       double *xn, int *flag, int epoch, double w, double omega, unsigned long long *ticket,
+                       unsigned long long *base, int *err, void *stream)
+{
+    const int nsl = (nb + WAVE - 1) / WAVE;
+    const int grid = (nsl + 3) / 4;
+    if (grid == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    switch (b) {
+    case 2: k_bhgs_sweep<2><<<grid, 256, 0, st>>>(nb, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, flag, epoch, w, omega, ticket, *base, err); break;
+    case 3: k_bhgs_sweep<3><<<grid, 256, 0, st>>>(nb, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, flag, epoch, w, omega, ticket, *base, err); break;
+    case 4: k_bhgs_sweep<4><<<grid, 256, 0, st>>>(nb, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, flag, epoch, w, omega, ticket, *base, err); break;
+    default: return JXAMD_ERROR_ARG;
+    }
+    *base += (unsigned long long)grid;
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_dense_gemv(int n, const double *M, const double *x, double *y, void *stream)
+{
+    if (n == 0) return 0;
+    k_dense_gemv<<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, M, x, y);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream)
+{
+    if (n == 0) return 0;
+    k_gather_stride<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, x, stride, off, y);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_prolong(int nb, int b, int pidx, const double *xp, double *x, void *stream)
+{
+    if (nb == 0) return 0;
+    k_prolong<<<(nb + 255) / 256, 256, 0, (hipStream_t)stream>>>(nb, b, pidx, xp, x);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_fill(size_t n, double *y, double v, void *stream)
+{
+    if (n == 0) return 0;
+    k_fill<<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, y, v);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_copy(size_t n, const double *x, double *y, void *stream)
+{
+    if (n == 0) return 0;
+    k_copy<<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, x, y);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_axpy(size_t n, const double *pa, double a, int neg, const double *x, double *y, void *stream)
+{
+    if (n == 0) return 0;
+    k_axpy<<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, pa, a, neg, x, y);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *stream)
+{
+    if (n == 0) return 0;
+    k_scale<<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, pnorm2, a, y);
+    return chk(hipGetLastError());
+}
+
+int jxamd_dot_grid(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 1024) b = 1024;
+    return b < 1 ? 1 : (int)b;
+}
+
+int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
+                    unsigned int *count, double *out, void *stream)
+{
+    k_dot_mgs<<<jxamd_dot_grid(n), 256, 0, (hipStream_t)stream>>>(n, u, y, v_prev, h_prev, part, count, out);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream)
+{
+    if (n == 0 || m == 0) return 0;
+    k_lincomb<<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, m, V, ld, y, w);
+    return chk(hipGetLastError());
+}
+
+} // extern "C"

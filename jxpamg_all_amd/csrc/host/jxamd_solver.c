@@ -1,0 +1,999 @@
+/*
+ * jxamd_solver.c -- host side of the MI355X CPR-GMRES solve phase.
+ *
+ * Builds the device layouts once (SELL-64 matrices, dependency-level sweep
+ * orders, explicit inverses of the diagonal blocks and of the coarsest
+ * matrix), then runs the reference's algorithms with every vector resident
+ * in HBM:
+ *   GMRES(k) with modified Gram-Schmidt     jxfpamg-0220/csrc/krylov/pgmres.c:950-1713
+ *   CPR two-stage preconditioner            jxfpamg-0220/csrc/cpr/jxf_cpr.c:424-610
+ *   pressure AMG V-cycle                    jxfpamg-0220/csrc/amg/par_cycle.c:20-969
+ *   hybrid SSOR (relax 6), 2 per visit      jxfpamg-0220/csrc/operation/relaxations/par_relax_6.c:224-273
+ *   coarsest direct solve (relax 9)         jxfpamg-0220/csrc/operation/relaxations/par_relax_9.c:20-142
+ *   block hybrid SOR (stage 2)              jxfpamg-0220/csrc/amg/par_relax_bsr.c:193-573
+ * The Hessenberg/Givens recurrences run on the host in FP64, one column
+ * read back per Arnoldi step.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "jxamd.h"
+#include "../hip/jxamd_dev.h"
+
+#define SELLW 64
+
+/* ---------------------------------------------------------------------- */
+/* device matrix containers                                                */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    int n, ncols, nsl;
+    int *sptr, *col, *perm, *flag;
+    double *val, *diag;
+} dsell;
+
+typedef struct {
+    int nb, b, nsl;
+    int *sptr, *col;
+    double *val;
+    /* forward block-HGS order */
+    int gnsl;
+    int *gperm, *gsptr, *gcol, *flag;
+    double *gval, *dinv;
+} dbsr;
+
+struct jxamd_solver_s {
+    jxamd_params prm;
+    void *stream;
+    void *ev0, *ev1, *ev2, *ev3;
+    int nb, b;
+    size_t N;
+    dbsr A;
+    int L;
+    int nlev[JXAMD_MAX_LEVELS];
+    dsell Ares[JXAMD_MAX_LEVELS], Gf[JXAMD_MAX_LEVELS], Gb[JXAMD_MAX_LEVELS];
+    dsell R[JXAMD_MAX_LEVELS], P[JXAMD_MAX_LEVELS];
+    double *cinv;
+    int nc;
+    double *F[JXAMD_MAX_LEVELS], *U[JXAMD_MAX_LEVELS], *T[JXAMD_MAX_LEVELS], *Vt[JXAMD_MAX_LEVELS];
+    double *xtmp;          /* stage-2 ping-pong partner, N */
+    double *V;             /* (k_dim+1) Krylov vectors, N each */
+    double *Rv, *Wv, *Bd, *Xd; /* GMRES work, host-solve staging */
+    double *dsc;           /* device scalars: h[0..k+1], y[0..k], tmp */
+    double *part;
+    unsigned int *count;
+    unsigned long long *ticket;
+    unsigned long long tbase;
+    int *err;
+    int epoch;
+    double t_prec_ms;
+};
+
+#define CK(x)                                                                                                          \
+    do {                                                                                                               \
+        int e__ = (x);                                                                                                 \
+        if (e__) return e__;                                                                                           \
+    } while (0)
+
+void jxamd_params_default(jxamd_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->k_dim = 30;
+    p->max_iter = 100;
+    p->is_check_restarted = 1;
+    p->tol = 1e-6;
+    p->a_tol = 0.0;
+    p->pressure_index = 0;
+    p->stage1_maxit = 1;
+    p->stage2_maxit = 3;
+    p->stage1_solver_type = 1;
+    p->stage2_solver_type = 2;
+    p->print_level = 0;
+    p->threshold = 1e-15;
+    p->stage2_weight = 0.7;
+    p->stage2_omega = 0.9;
+}
+
+/* ---------------------------------------------------------------------- */
+/* host-side layout construction                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Dependency levels of one GS sweep: forward (dir>0) uses the strictly lower
+ * part, backward the strictly upper part.  perm lists rows by (level, row). */
+static int level_perm(int n, const int *ia, const int *ja, int dir, int *perm)
+{
+    int *lev = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    if (!lev) return JXAMD_ERROR_MEMORY;
+    int maxlev = 0;
+    if (dir > 0) {
+        for (int i = 0; i < n; i++) {
+            int l = 0;
+            for (int k = ia[i]; k < ia[i + 1]; k++) {
+                int j = ja[k];
+                if (j < i && lev[j] + 1 > l) l = lev[j] + 1;
+            }
+            lev[i] = l;
+            if (l > maxlev) maxlev = l;
+        }
+    } else {
+        for (int i = n - 1; i >= 0; i--) {
+            int l = 0;
+            for (int k = ia[i]; k < ia[i + 1]; k++) {
+                int j = ja[k];
+                if (j > i && lev[j] + 1 > l) l = lev[j] + 1;
+            }
+            lev[i] = l;
+            if (l > maxlev) maxlev = l;
+        }
+    }
+    int *cnt = (int *)calloc((size_t)maxlev + 2, sizeof(int));
+    if (!cnt) { free(lev); return JXAMD_ERROR_MEMORY; }
+    for (int i = 0; i < n; i++) cnt[lev[i] + 1]++;
+    for (int l = 0; l <= maxlev; l++) cnt[l + 1] += cnt[l];
+    for (int i = 0; i < n; i++) perm[cnt[lev[i]]++] = i;
+    free(cnt);
+    free(lev);
+    return 0;
+}
+
+/* Scalar SELL-64 on the host.  Rows in the order given by perm (identity if
+ * NULL).  offdiag: drop the first (diagonal) entry of every row into diag[].
+ * pad_self: pad with the row's own index (square smoother matrices), else 0. */
+typedef struct {
+    int n, nsl;
+    int *sptr, *col, *perm;
+    double *val, *diag;
+} hsell;
+
+static void hsell_free(hsell *h)
+{
+    free(h->sptr); free(h->col); free(h->perm); free(h->val); free(h->diag);
+    memset(h, 0, sizeof(*h));
+}
+
+static int hsell_build(hsell *h, int n, const int *ia, const int *ja, const double *a, int *perm_owned, int offdiag,
+                       int pad_self)
+{
+    memset(h, 0, sizeof(*h));
+    h->n = n;
+    h->nsl = (n + SELLW - 1) / SELLW;
+    h->perm = perm_owned;
+    h->sptr = (int *)malloc(sizeof(int) * (h->nsl + 1));
+    if (offdiag) h->diag = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    if (!h->sptr || (offdiag && !h->diag)) return JXAMD_ERROR_MEMORY;
+    const int skip = offdiag ? 1 : 0;
+    long total = 0;
+    h->sptr[0] = 0;
+    for (int s = 0; s < h->nsl; s++) {
+        int w = 0;
+        for (int l = 0; l < SELLW; l++) {
+            int p = s * SELLW + l;
+            if (p >= n) break;
+            int i = perm_owned ? perm_owned[p] : p;
+            int len = ia[i + 1] - ia[i] - skip;
+            if (len > w) w = len;
+        }
+        total += w;
+        if (total > 0x7fffffffL / SELLW) return JXAMD_ERROR_ARG;
+        h->sptr[s + 1] = (int)total;
+    }
+    h->col = (int *)malloc(sizeof(int) * (size_t)(total > 0 ? total : 1) * SELLW);
+    h->val = (double *)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1) * SELLW);
+    if (!h->col || !h->val) return JXAMD_ERROR_MEMORY;
+    for (int s = 0; s < h->nsl; s++) {
+        int k0 = h->sptr[s], w = h->sptr[s + 1] - k0;
+        for (int l = 0; l < SELLW; l++) {
+            int p = s * SELLW + l;
+            int i = -1, len = 0, beg = 0;
+            if (p < n) {
+                i = perm_owned ? perm_owned[p] : p;
+                beg = ia[i] + skip;
+                len = ia[i + 1] - beg;
+                if (offdiag) h->diag[p] = a[ia[i]];
+            }
+            for (int k = 0; k < w; k++) {
+                size_t e = (size_t)(k0 + k) * SELLW + l;
+                if (k < len) {
+                    h->col[e] = ja[beg + k];
+                    h->val[e] = a[beg + k];
+                } else {
+                    h->col[e] = (pad_self && i >= 0) ? i : 0;
+                    h->val[e] = 0.0;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, int with_flag)
+{
+    memset(d, 0, sizeof(*d));
+    d->n = h->n;
+    d->ncols = ncols;
+    d->nsl = h->nsl;
+    size_t slots = (size_t)(h->nsl ? h->sptr[h->nsl] : 0) * SELLW;
+    CK(jxamd_dev_malloc((void **)&d->sptr, sizeof(int) * (h->nsl + 1)));
+    CK(jxamd_dev_malloc((void **)&d->col, sizeof(int) * (slots ? slots : 1)));
+    CK(jxamd_dev_malloc((void **)&d->val, sizeof(double) * (slots ? slots : 1)));
+    CK(jxamd_h2d(d->sptr, h->sptr, sizeof(int) * (h->nsl + 1), st));
+    CK(jxamd_h2d(d->col, h->col, sizeof(int) * slots, st));
+    CK(jxamd_h2d(d->val, h->val, sizeof(double) * slots, st));
+    if (h->perm) {
+        CK(jxamd_dev_malloc((void **)&d->perm, sizeof(int) * (h->n ? h->n : 1)));
+        CK(jxamd_h2d(d->perm, h->perm, sizeof(int) * h->n, st));
+    }
+    if (h->diag) {
+        CK(jxamd_dev_malloc((void **)&d->diag, sizeof(double) * (h->n ? h->n : 1)));
+        CK(jxamd_h2d(d->diag, h->diag, sizeof(double) * h->n, st));
+    }
+    if (with_flag) {
+        int *z = (int *)calloc((size_t)(h->n ? h->n : 1), sizeof(int));
+        if (!z) return JXAMD_ERROR_MEMORY;
+        CK(jxamd_dev_malloc((void **)&d->flag, sizeof(int) * (h->n ? h->n : 1)));
+        int e = jxamd_h2d(d->flag, z, sizeof(int) * (h->n ? h->n : 1), st);
+        if (!e) e = jxamd_stream_sync(st);
+        free(z);
+        CK(e);
+    }
+    return 0;
+}
+
+static void dsell_free(dsell *d)
+{
+    jxamd_dev_free(d->sptr); jxamd_dev_free(d->col); jxamd_dev_free(d->val);
+    jxamd_dev_free(d->perm); jxamd_dev_free(d->diag); jxamd_dev_free(d->flag);
+    memset(d, 0, sizeof(*d));
+}
+
+/* In-place inverse of a dense n x n row-major matrix by Gauss-Jordan with
+ * partial pivoting.  Returns nonzero if a pivot is exactly zero. */
+static int dense_inverse(double *M, int n)
+{
+    int *piv = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    if (!piv) return JXAMD_ERROR_MEMORY;
+    for (int i = 0; i < n; i++) piv[i] = i;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = fabs(M[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; i++) {
+            double v = fabs(M[(size_t)i * n + k]);
+            if (v > best) { best = v; p = i; }
+        }
+        if (best == 0.0) { free(piv); return JXAMD_ERROR_GENERIC; }
+        if (p != k) {
+            for (int j = 0; j < n; j++) {
+                double t = M[(size_t)k * n + j];
+                M[(size_t)k * n + j] = M[(size_t)p * n + j];
+                M[(size_t)p * n + j] = t;
+            }
+            int t = piv[k]; piv[k] = piv[p]; piv[p] = t;
+        }
+        double d = 1.0 / M[(size_t)k * n + k];
+        M[(size_t)k * n + k] = 1.0;
+        for (int j = 0; j < n; j++) M[(size_t)k * n + j] *= d;
+        for (int i = 0; i < n; i++) {
+            if (i == k) continue;
+            double f = M[(size_t)i * n + k];
+            if (f == 0.0) continue;
+            M[(size_t)i * n + k] = 0.0;
+            for (int j = 0; j < n; j++) M[(size_t)i * n + j] -= f * M[(size_t)k * n + j];
+        }
+    }
+    /* undo the row permutation as a column permutation */
+    double *tmp = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    if (!tmp) { free(piv); return JXAMD_ERROR_MEMORY; }
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) tmp[piv[j]] = M[(size_t)i * n + j];
+        memcpy(M + (size_t)i * n, tmp, sizeof(double) * n);
+    }
+    free(tmp);
+    free(piv);
+    return 0;
+}
+
+/* transpose of a CSR matrix (rows of R sorted by the source row, which keeps
+ * the accumulation order of jxf_CSRMatrixMatvecT, seq_csr_matvec.c:211-330) */
+static int csr_transpose(int n, int m, const int *ia, const int *ja, const double *a, int **tia, int **tja,
+                         double **ta)
+{
+    int nnz = ia[n];
+    int *ri = (int *)calloc((size_t)m + 1, sizeof(int));
+    int *rj = (int *)malloc(sizeof(int) * (nnz > 0 ? nnz : 1));
+    double *ra = (double *)malloc(sizeof(double) * (nnz > 0 ? nnz : 1));
+    if (!ri || !rj || !ra) { free(ri); free(rj); free(ra); return JXAMD_ERROR_MEMORY; }
+    for (int k = 0; k < nnz; k++) ri[ja[k] + 1]++;
+    for (int c = 0; c < m; c++) ri[c + 1] += ri[c];
+    int *pos = (int *)malloc(sizeof(int) * ((size_t)m + 1));
+    if (!pos) { free(ri); free(rj); free(ra); return JXAMD_ERROR_MEMORY; }
+    memcpy(pos, ri, sizeof(int) * ((size_t)m + 1));
+    for (int i = 0; i < n; i++)
+        for (int k = ia[i]; k < ia[i + 1]; k++) {
+            int q = pos[ja[k]]++;
+            rj[q] = i;
+            ra[q] = a[k];
+        }
+    free(pos);
+    *tia = ri; *tja = rj; *ta = ra;
+    return 0;
+}
+
+static int check_csr(const jxamd_csr *c, int square)
+{
+    if (!c->ia || !c->ja || !c->a || c->n < 0 || c->ncols < 0) return JXAMD_ERROR_ARG;
+    if (c->ia[0] != 0) return JXAMD_ERROR_ARG;
+    for (int i = 0; i < c->n; i++) {
+        if (c->ia[i + 1] < c->ia[i]) return JXAMD_ERROR_ARG;
+        for (int k = c->ia[i]; k < c->ia[i + 1]; k++)
+            if (c->ja[k] < 0 || c->ja[k] >= c->ncols) return JXAMD_ERROR_ARG;
+        if (square && (c->ia[i + 1] == c->ia[i] || c->ja[c->ia[i]] != i)) return JXAMD_ERROR_ARG;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* create / destroy                                                        */
+/* ---------------------------------------------------------------------- */
+static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
+{
+    const int nb = A->nb, b = A->b, bb = b * b;
+    dbsr *d = &s->A;
+    d->nb = nb;
+    d->b = b;
+    /* natural-order SELL over all blocks (SpMV) */
+    {
+        hsell h;
+        jxamd_csr pat = {nb, nb, A->nnzb, A->ia, A->ja, NULL};
+        /* reuse the scalar builder for the pattern, then place block values */
+        double *dummy = (double *)calloc((size_t)(A->nnzb > 0 ? A->nnzb : 1), sizeof(double));
+        if (!dummy) return JXAMD_ERROR_MEMORY;
+        int e = hsell_build(&h, nb, pat.ia, pat.ja, dummy, NULL, 0, 0);
+        free(dummy);
+        if (e) { hsell_free(&h); return e; }
+        size_t slots = (size_t)(h.nsl ? h.sptr[h.nsl] : 0);
+        double *v = (double *)calloc(slots * bb * SELLW + 1, sizeof(double));
+        if (!v) { hsell_free(&h); return JXAMD_ERROR_MEMORY; }
+        for (int sl = 0; sl < h.nsl; sl++) {
+            int k0 = h.sptr[sl], w = h.sptr[sl + 1] - k0;
+            for (int l = 0; l < SELLW; l++) {
+                int i = sl * SELLW + l;
+                if (i >= nb) break;
+                int len = A->ia[i + 1] - A->ia[i];
+                for (int k = 0; k < w && k < len; k++) {
+                    const double *blk = A->a + (size_t)(A->ia[i] + k) * bb;
+                    for (int e2 = 0; e2 < bb; e2++) v[((size_t)(k0 + k) * bb + e2) * SELLW + l] = blk[e2];
+                }
+            }
+        }
+        d->nsl = h.nsl;
+        int rc = jxamd_dev_malloc((void **)&d->sptr, sizeof(int) * (h.nsl + 1));
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->col, sizeof(int) * (slots * SELLW + 1));
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->val, sizeof(double) * (slots * bb * SELLW + 1));
+        if (!rc) rc = jxamd_h2d(d->sptr, h.sptr, sizeof(int) * (h.nsl + 1), s->stream);
+        if (!rc) rc = jxamd_h2d(d->col, h.col, sizeof(int) * slots * SELLW, s->stream);
+        if (!rc) rc = jxamd_h2d(d->val, v, sizeof(double) * slots * bb * SELLW, s->stream);
+        if (!rc) rc = jxamd_stream_sync(s->stream);
+        free(v);
+        hsell_free(&h);
+        CK(rc);
+    }
+    /* forward block-HGS order: off-diagonal blocks in level order + Dinv */
+    {
+        int *perm = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+        if (!perm) return JXAMD_ERROR_MEMORY;
+        CK(level_perm(nb, A->ia, A->ja, 1, perm));
+        hsell h;
+        double *dummy = (double *)calloc((size_t)(A->nnzb > 0 ? A->nnzb : 1), sizeof(double));
+        if (!dummy) { free(perm); return JXAMD_ERROR_MEMORY; }
+        int e = hsell_build(&h, nb, A->ia, A->ja, dummy, perm, 1, 1);
+        free(dummy);
+        if (e) { hsell_free(&h); return e; }
+        size_t slots = (size_t)(h.nsl ? h.sptr[h.nsl] : 0);
+        double *v = (double *)calloc(slots * bb * SELLW + 1, sizeof(double));
+        double *dinv = (double *)malloc(sizeof(double) * (size_t)nb * bb + 8);
+        if (!v || !dinv) { free(v); free(dinv); hsell_free(&h); return JXAMD_ERROR_MEMORY; }
+        for (int sl = 0; sl < h.nsl; sl++) {
+            int k0 = h.sptr[sl], w = h.sptr[sl + 1] - k0;
+            for (int l = 0; l < SELLW; l++) {
+                int p = sl * SELLW + l;
+                if (p >= nb) break;
+                int i = perm[p];
+                int beg = A->ia[i] + 1, len = A->ia[i + 1] - beg;
+                for (int k = 0; k < w && k < len; k++) {
+                    const double *blk = A->a + (size_t)(beg + k) * bb;
+                    for (int e2 = 0; e2 < bb; e2++) v[((size_t)(k0 + k) * bb + e2) * SELLW + l] = blk[e2];
+                }
+            }
+        }
+        for (int i = 0; i < nb; i++) {
+            double M[JXAMD_MAX_BLOCK * JXAMD_MAX_BLOCK];
+            memcpy(M, A->a + (size_t)A->ia[i] * bb, sizeof(double) * bb);
+            if (dense_inverse(M, b)) {
+                /* singular diagonal block: diagonal approximation (cf. seq_bsr_matrix.c:985-991) */
+                for (int r = 0; r < bb; r++) M[r] = 0.0;
+                for (int r = 0; r < b; r++) {
+                    double dg = A->a[(size_t)A->ia[i] * bb + r * b + r];
+                    M[r * b + r] = fabs(dg) > 1e-12 ? 1.0 / dg : 0.0;
+                }
+            }
+            memcpy(dinv + (size_t)i * bb, M, sizeof(double) * bb);
+        }
+        d->gnsl = h.nsl;
+        int rc = jxamd_dev_malloc((void **)&d->gsptr, sizeof(int) * (h.nsl + 1));
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->gcol, sizeof(int) * (slots * SELLW + 1));
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->gval, sizeof(double) * (slots * bb * SELLW + 1));
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->gperm, sizeof(int) * (nb + 1));
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->dinv, sizeof(double) * ((size_t)nb * bb + 1));
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->flag, sizeof(int) * (nb + 1));
+        if (!rc) rc = jxamd_h2d(d->gsptr, h.sptr, sizeof(int) * (h.nsl + 1), s->stream);
+        if (!rc) rc = jxamd_h2d(d->gcol, h.col, sizeof(int) * slots * SELLW, s->stream);
+        if (!rc) rc = jxamd_h2d(d->gval, v, sizeof(double) * slots * bb * SELLW, s->stream);
+        if (!rc) rc = jxamd_h2d(d->gperm, perm, sizeof(int) * nb, s->stream);
+        if (!rc) rc = jxamd_h2d(d->dinv, dinv, sizeof(double) * (size_t)nb * bb, s->stream);
+        int *z = (int *)calloc((size_t)nb + 1, sizeof(int));
+        if (!rc && z) rc = jxamd_h2d(d->flag, z, sizeof(int) * (nb + 1), s->stream);
+        if (!rc) rc = jxamd_stream_sync(s->stream);
+        free(z);
+        free(v);
+        free(dinv);
+        hsell_free(&h); /* frees perm */
+        CK(rc);
+    }
+    return 0;
+}
+
+static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
+{
+    const jxamd_csr *A = &H->A[l];
+    const int n = A->n;
+    hsell h;
+    int e;
+    /* natural order, full rows: residual F - A U */
+    e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 0, 0);
+    if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, 0);
+    if (!e) e = jxamd_stream_sync(s->stream);
+    hsell_free(&h);
+    CK(e);
+    if (l < s->L - 1 || s->L == 1) {
+        /* forward and backward sweep orders, off-diagonal entries */
+        int *pf = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+        if (!pf) return JXAMD_ERROR_MEMORY;
+        CK(level_perm(n, A->ia, A->ja, 1, pf));
+        e = hsell_build(&h, n, A->ia, A->ja, A->a, pf, 1, 1);
+        if (!e) e = dsell_upload(&s->Gf[l], &h, n, s->stream, 1);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        CK(e);
+        int *pb = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+        if (!pb) return JXAMD_ERROR_MEMORY;
+        CK(level_perm(n, A->ia, A->ja, -1, pb));
+        e = hsell_build(&h, n, A->ia, A->ja, A->a, pb, 1, 1);
+        if (!e) e = dsell_upload(&s->Gb[l], &h, n, s->stream, 0);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        CK(e);
+        s->Gb[l].flag = NULL; /* both directions share the forward flag array */
+    }
+    if (l < s->L - 1) {
+        /* interpolation P (n x nc) and restriction R = P^T (nc x n) */
+        const jxamd_csr *Pm = &H->P[l];
+        e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
+        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, 0);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        CK(e);
+        int *ri, *rj;
+        double *ra;
+        CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
+        e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
+        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, 0);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        free(ri); free(rj); free(ra);
+        CK(e);
+    } else if (s->L > 1) {
+        /* coarsest: explicit inverse of the densified matrix */
+        double *M = (double *)calloc((size_t)n * n + 1, sizeof(double));
+        if (!M) return JXAMD_ERROR_MEMORY;
+        for (int i = 0; i < n; i++)
+            for (int k = A->ia[i]; k < A->ia[i + 1]; k++) M[(size_t)i * n + A->ja[k]] = A->a[k];
+        e = dense_inverse(M, n);
+        if (!e) e = jxamd_dev_malloc((void **)&s->cinv, sizeof(double) * ((size_t)n * n + 1));
+        if (!e) e = jxamd_h2d(s->cinv, M, sizeof(double) * (size_t)n * n, s->stream);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        free(M);
+        s->nc = n;
+        CK(e);
+    }
+    return 0;
+}
+
+int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hierarchy *H, const jxamd_params *p)
+{
+    if (!out || !A || !H || !p) return JXAMD_ERROR_ARG;
+    *out = NULL;
+    if (A->b < 2 || A->b > JXAMD_MAX_BLOCK || A->nb < 1 || A->ncols != A->nb) return JXAMD_ERROR_ARG;
+    if (H->num_levels < 1 || H->num_levels > JXAMD_MAX_LEVELS) return JXAMD_ERROR_ARG;
+    if (p->k_dim < 1 || p->pressure_index < 0 || p->pressure_index >= A->b) return JXAMD_ERROR_ARG;
+    if (p->stage1_maxit < 0 || p->stage2_maxit < 0) return JXAMD_ERROR_ARG;
+    {
+        jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
+        if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
+        CK(check_csr(&pat, 1));
+    }
+    if (H->A[0].n != A->nb) return JXAMD_ERROR_ARG;
+    for (int l = 0; l < H->num_levels; l++) {
+        if (H->A[l].ncols != H->A[l].n) return JXAMD_ERROR_ARG;
+        CK(check_csr(&H->A[l], 1));
+        if (l < H->num_levels - 1) {
+            if (H->P[l].n != H->A[l].n || H->P[l].ncols != H->A[l + 1].n) return JXAMD_ERROR_ARG;
+            CK(check_csr(&H->P[l], 0));
+        }
+    }
+    int ndev = 0;
+    jxamd_dev_count(&ndev);
+    if (ndev < 1) return JXAMD_ERROR_DEVICE;
+
+    jxamd_solver *s = (jxamd_solver *)calloc(1, sizeof(jxamd_solver));
+    if (!s) return JXAMD_ERROR_MEMORY;
+    s->prm = *p;
+    s->nb = A->nb;
+    s->b = A->b;
+    s->N = (size_t)A->nb * A->b;
+    s->L = H->num_levels;
+    int e = jxamd_stream_create(&s->stream);
+    if (!e) e = jxamd_event_create(&s->ev0);
+    if (!e) e = jxamd_event_create(&s->ev1);
+    if (!e) e = jxamd_event_create(&s->ev2);
+    if (!e) e = jxamd_event_create(&s->ev3);
+    if (!e) e = build_bsr(s, A);
+    for (int l = 0; l < s->L && !e; l++) {
+        s->nlev[l] = H->A[l].n;
+        e = build_level(s, l, H);
+    }
+    for (int l = 0; l < s->L && !e; l++) {
+        size_t n = (size_t)s->nlev[l] + 1;
+        e = jxamd_dev_malloc((void **)&s->F[l], sizeof(double) * n);
+        if (!e) e = jxamd_dev_malloc((void **)&s->U[l], sizeof(double) * n);
+        if (!e) e = jxamd_dev_malloc((void **)&s->T[l], sizeof(double) * n);
+        if (!e) e = jxamd_dev_malloc((void **)&s->Vt[l], sizeof(double) * n);
+    }
+    const size_t N = s->N;
+    if (!e) e = jxamd_dev_malloc((void **)&s->xtmp, sizeof(double) * N);
+    if (!e) e = jxamd_dev_malloc((void **)&s->V, sizeof(double) * N * (size_t)(p->k_dim + 1));
+    if (!e) e = jxamd_dev_malloc((void **)&s->Rv, sizeof(double) * N);
+    if (!e) e = jxamd_dev_malloc((void **)&s->Wv, sizeof(double) * N);
+    if (!e) e = jxamd_dev_malloc((void **)&s->Bd, sizeof(double) * N);
+    if (!e) e = jxamd_dev_malloc((void **)&s->Xd, sizeof(double) * N);
+    if (!e) e = jxamd_dev_malloc((void **)&s->dsc, sizeof(double) * (2 * (size_t)p->k_dim + 16));
+    if (!e) e = jxamd_dev_malloc((void **)&s->part, sizeof(double) * 1024);
+    if (!e) e = jxamd_dev_malloc((void **)&s->count, sizeof(unsigned int) * 4);
+    if (!e) e = jxamd_dev_malloc((void **)&s->ticket, sizeof(unsigned long long));
+    if (!e) e = jxamd_dev_malloc((void **)&s->err, sizeof(int) * 4);
+    if (!e) {
+        unsigned int zc[4] = {0, 0, 0, 0};
+        unsigned long long zt = 0;
+        int ze[4] = {0, 0, 0, 0};
+        e = jxamd_h2d(s->count, zc, sizeof(zc), s->stream);
+        if (!e) e = jxamd_h2d(s->ticket, &zt, sizeof(zt), s->stream);
+        if (!e) e = jxamd_h2d(s->err, ze, sizeof(ze), s->stream);
+        if (!e) e = jxamd_stream_sync(s->stream);
+    }
+    s->tbase = 0;
+    s->epoch = 0;
+    if (e) {
+        jxamd_solver_destroy(s);
+        return e;
+    }
+    *out = s;
+    return 0;
+}
+
+int jxamd_solver_destroy(jxamd_solver *s)
+{
+    if (!s) return 0;
+    if (s->stream) jxamd_stream_sync(s->stream);
+    dbsr *d = &s->A;
+    jxamd_dev_free(d->sptr); jxamd_dev_free(d->col); jxamd_dev_free(d->val);
+    jxamd_dev_free(d->gperm); jxamd_dev_free(d->gsptr); jxamd_dev_free(d->gcol);
+    jxamd_dev_free(d->gval); jxamd_dev_free(d->dinv); jxamd_dev_free(d->flag);
+    for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
+        dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]);
+        dsell_free(&s->R[l]); dsell_free(&s->P[l]);
+        jxamd_dev_free(s->F[l]); jxamd_dev_free(s->U[l]); jxamd_dev_free(s->T[l]); jxamd_dev_free(s->Vt[l]);
+    }
+    jxamd_dev_free(s->cinv);
+    jxamd_dev_free(s->xtmp); jxamd_dev_free(s->V); jxamd_dev_free(s->Rv); jxamd_dev_free(s->Wv);
+    jxamd_dev_free(s->Bd); jxamd_dev_free(s->Xd);
+    jxamd_dev_free(s->dsc); jxamd_dev_free(s->part); jxamd_dev_free(s->count);
+    jxamd_dev_free(s->ticket); jxamd_dev_free(s->err);
+    jxamd_event_destroy(s->ev0); jxamd_event_destroy(s->ev1);
+    jxamd_event_destroy(s->ev2); jxamd_event_destroy(s->ev3);
+    jxamd_stream_destroy(s->stream);
+    free(s);
+    return 0;
+}
+
+int jxamd_solver_error_flag(jxamd_solver *s, int *flag)
+{
+    if (!s || !flag) return JXAMD_ERROR_ARG;
+    CK(jxamd_d2h(flag, s->err, sizeof(int), s->stream));
+    return jxamd_stream_sync(s->stream);
+}
+
+/* ---------------------------------------------------------------------- */
+/* preconditioner pieces                                                   */
+/* ---------------------------------------------------------------------- */
+
+/* one relax-6 call: forward sweep u -> T, backward sweep T -> u.
+ * uo_zero: the incoming u is known to be zero (first visit of a level). */
+static int hsgs(jxamd_solver *s, int l, const double *f, int fs, double *u, int uo_zero)
+{
+    dsell *g = &s->Gf[l], *h = &s->Gb[l];
+    int *flag = s->Gf[l].flag;
+    s->epoch++;
+    CK(jxamd_k_gs_sweep(g->n, g->perm, g->sptr, g->col, g->val, g->diag, f, fs, uo_zero ? NULL : u, s->T[l], flag,
+                        s->epoch, 1, s->ticket, &s->tbase, s->err, s->stream));
+    s->epoch++;
+    CK(jxamd_k_gs_sweep(h->n, h->perm, h->sptr, h->col, h->val, h->diag, f, fs, s->T[l], u, flag, s->epoch, -1,
+                        s->ticket, &s->tbase, s->err, s->stream));
+    return 0;
+}
+
+/* V-cycle on F[0] (possibly a strided view) into U[0].  u0_zero: U[0] is zero
+ * on entry (the CPR stage-1 start x_p := 0, jxf_cpr.c:448). */
+static int vcycle(jxamd_solver *s, const double *f0, int fs0, int u0_zero)
+{
+    const int L = s->L;
+    if (L == 1) {
+        /* no coarsening: one relax-6 call (par_cycle.c:236-255) */
+        return hsgs(s, 0, f0, fs0, s->U[0], u0_zero);
+    }
+    for (int l = 0; l < L - 1; l++) {
+        const double *f = l == 0 ? f0 : s->F[l];
+        const int fs = l == 0 ? fs0 : 1;
+        const int zero = l == 0 ? u0_zero : 1;
+        CK(hsgs(s, l, f, fs, s->U[l], zero));
+        CK(hsgs(s, l, f, fs, s->U[l], 0));
+        /* Vtemp = F - A U; F_{l+1} = P^T Vtemp */
+        dsell *A = &s->Ares[l];
+        CK(jxamd_k_csr_spmv(A->n, A->sptr, A->col, A->val, s->U[l], 1, f, fs, s->Vt[l], 1, s->stream));
+        dsell *R = &s->R[l];
+        CK(jxamd_k_csr_spmv(R->n, R->sptr, R->col, R->val, s->Vt[l], 1, NULL, 1, s->F[l + 1], 0, s->stream));
+    }
+    CK(jxamd_k_dense_gemv(s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
+    for (int l = L - 2; l >= 0; l--) {
+        dsell *P = &s->P[l];
+        CK(jxamd_k_csr_spmv(P->n, P->sptr, P->col, P->val, s->U[l + 1], 1, NULL, 1, s->U[l], 2, s->stream));
+        const double *f = l == 0 ? f0 : s->F[l];
+        const int fs = l == 0 ? fs0 : 1;
+        CK(hsgs(s, l, f, fs, s->U[l], 0));
+        CK(hsgs(s, l, f, fs, s->U[l], 0));
+    }
+    return 0;
+}
+
+static int bhgs_sweep(jxamd_solver *s, const double *rhs, const double *xo, const double *xp, double *xn, double w,
+                      double omega)
+{
+    dbsr *d = &s->A;
+    s->epoch++;
+    return jxamd_k_bhgs_sweep(d->b, d->nb, d->gperm, d->gsptr, d->gcol, d->gval, d->dinv, rhs, xo, xp,
+                              s->prm.pressure_index, xn, d->flag, s->epoch, w, omega, s->ticket, &s->tbase, s->err,
+                              s->stream);
+}
+
+/* x = M^{-1} r (jxf_cpr.c:424-610) */
+static int cpr_apply(jxamd_solver *s, const double *r, double *x)
+{
+    const int b = s->b, pi = s->prm.pressure_index;
+    int zero = 1;
+    /* stage 1: the restricted residual is read in place (stride b) */
+    for (int it = 0; it < s->prm.stage1_maxit; it++) {
+        CK(vcycle(s, r + pi, b, zero));
+        zero = 0;
+    }
+    if (s->prm.stage1_maxit == 0) CK(jxamd_k_fill(s->nlev[0], s->U[0], 0.0, s->stream));
+    /* prolongation (type 0) + stage-2 sweeps; the first sweep reads the
+     * prolongated vector on the fly and the last one lands in x */
+    const int m = s->prm.stage2_maxit;
+    if (m == 0) return jxamd_k_prolong(s->nb, b, pi, s->U[0], x, s->stream);
+    double *dst = (m % 2 == 1) ? x : s->xtmp;
+    CK(bhgs_sweep(s, r, NULL, s->U[0], dst, s->prm.stage2_weight, s->prm.stage2_omega));
+    for (int it = 1; it < m; it++) {
+        double *src = dst;
+        dst = (src == x) ? s->xtmp : x;
+        CK(bhgs_sweep(s, r, src, NULL, dst, s->prm.stage2_weight, s->prm.stage2_omega));
+    }
+    return 0;
+}
+
+int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x)
+{
+    if (!s || !d_r || !d_x) return JXAMD_ERROR_ARG;
+    CK(cpr_apply(s, d_r, d_x));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_op_bsr_matvec(jxamd_solver *s, double alpha, const double *d_x, double beta, double *d_y)
+{
+    if (!s) return JXAMD_ERROR_ARG;
+    dbsr *d = &s->A;
+    CK(jxamd_k_bsr_spmv(d->b, d->nb, d->sptr, d->col, d->val, d_x, NULL, d_y, 2, alpha, beta, s->stream));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_op_bsr_matvec_timed(jxamd_solver *s, const double *d_x, double *d_y, int reps, double *ms_per_launch)
+{
+    if (!s || reps < 1 || !ms_per_launch) return JXAMD_ERROR_ARG;
+    dbsr *d = &s->A;
+    CK(jxamd_event_record(s->ev0, s->stream));
+    for (int r = 0; r < reps; r++)
+        CK(jxamd_k_bsr_spmv(d->b, d->nb, d->sptr, d->col, d->val, d_x, NULL, d_y, 0, 1.0, 0.0, s->stream));
+    CK(jxamd_event_record(s->ev1, s->stream));
+    CK(jxamd_stream_sync(s->stream));
+    float ms = 0.f;
+    CK(jxamd_event_elapsed(&ms, s->ev0, s->ev1));
+    *ms_per_launch = ms / reps;
+    return 0;
+}
+
+int jxamd_cpr_precond_timed(jxamd_solver *s, const double *d_r, double *d_x, int reps, double *ms_per_apply)
+{
+    if (!s || reps < 1 || !ms_per_apply) return JXAMD_ERROR_ARG;
+    CK(jxamd_event_record(s->ev0, s->stream));
+    for (int r = 0; r < reps; r++) CK(cpr_apply(s, d_r, d_x));
+    CK(jxamd_event_record(s->ev1, s->stream));
+    CK(jxamd_stream_sync(s->stream));
+    float ms = 0.f;
+    CK(jxamd_event_elapsed(&ms, s->ev0, s->ev1));
+    *ms_per_apply = ms / reps;
+    return 0;
+}
+
+int jxamd_op_hsgs(jxamd_solver *s, int level, const double *d_f, double *d_u)
+{
+    if (!s || level < 0 || (s->L > 1 && level >= s->L - 1) || level >= s->L) return JXAMD_ERROR_ARG;
+    CK(hsgs(s, level, d_f, 1, d_u, 0));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_op_coarse_solve(jxamd_solver *s, const double *d_f, double *d_u)
+{
+    if (!s || s->L < 2) return JXAMD_ERROR_ARG;
+    CK(jxamd_k_dense_gemv(s->nc, s->cinv, d_f, d_u, s->stream));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u)
+{
+    if (!s) return JXAMD_ERROR_ARG;
+    CK(jxamd_k_copy(s->nlev[0], d_u, s->U[0], s->stream));
+    CK(vcycle(s, d_f, 1, 0));
+    CK(jxamd_k_copy(s->nlev[0], s->U[0], d_u, s->stream));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, double omega)
+{
+    if (!s) return JXAMD_ERROR_ARG;
+    CK(bhgs_sweep(s, d_b, d_x, NULL, s->xtmp, w, omega));
+    CK(jxamd_k_copy(s->N, s->xtmp, d_x, s->stream));
+    return jxamd_stream_sync(s->stream);
+}
+
+/* ---------------------------------------------------------------------- */
+/* GMRES (pgmres.c:950-1713)                                               */
+/* ---------------------------------------------------------------------- */
+static int dev_dot(jxamd_solver *s, const double *u, double *y, const double *vprev, const double *hprev, double *out)
+{
+    return jxamd_k_dot_mgs(s->N, u, y, vprev, hprev, s->part, s->count, out, s->stream);
+}
+
+static int host_norm(jxamd_solver *s, const double *v, double *nrm)
+{
+    double *slot = s->dsc + 2 * s->prm.k_dim + 8;
+    CK(dev_dot(s, v, (double *)v, NULL, NULL, slot));
+    double h;
+    CK(jxamd_d2h(&h, slot, sizeof(double), s->stream));
+    CK(jxamd_stream_sync(s->stream));
+    *nrm = sqrt(h);
+    return 0;
+}
+
+static int precond_timed(jxamd_solver *s, const double *r, double *x)
+{
+    CK(jxamd_event_record(s->ev2, s->stream));
+    CK(cpr_apply(s, r, x));
+    CK(jxamd_event_record(s->ev3, s->stream));
+    return 0;
+}
+
+static int accumulate_precond_time(jxamd_solver *s)
+{
+    float ms = 0.f;
+    CK(jxamd_stream_sync(s->stream));
+    CK(jxamd_event_elapsed(&ms, s->ev2, s->ev3));
+    s->t_prec_ms += ms;
+    return 0;
+}
+
+int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, double *res_history, jxamd_stats *st)
+{
+    if (!s || !bvec || !x) return JXAMD_ERROR_ARG;
+    const int k_dim = s->prm.k_dim, max_iter = s->prm.max_iter;
+    const size_t N = s->N;
+    const double r_tol = s->prm.tol, a_tol = s->prm.a_tol, epsmac = 1.e-16, convfac_threshold = 1.0e4;
+    double *V = s->V;
+#define VEC(j) (V + (size_t)(j) * N)
+    double *h = s->dsc;              /* MGS column, k_dim+2 */
+    double *ydev = s->dsc + k_dim + 4; /* back-solved coefficients, k_dim+1 */
+    double *rs = (double *)calloc(k_dim + 1, sizeof(double));
+    double *c = (double *)calloc(k_dim, sizeof(double));
+    double *sn = (double *)calloc(k_dim, sizeof(double));
+    double *hcol = (double *)calloc(k_dim + 2, sizeof(double));
+    double **hh = (double **)malloc(sizeof(double *) * (k_dim + 1));
+    if (!rs || !c || !sn || !hcol || !hh) return JXAMD_ERROR_MEMORY;
+    for (int q = 0; q <= k_dim; q++) hh[q] = (double *)calloc(k_dim, sizeof(double));
+    int err = 0, converged = 0, nprec = 0, nrestart = 0;
+    int i = 0, j, k, iter = 0;
+    double t, gamma, r_norm = 0.0, b_norm = 0.0, den_norm, epsilon, r_norm_0 = 0.0;
+    double normup, normdown, conv_factor = 0.0;
+    s->t_prec_ms = 0.0;
+    dbsr *A = &s->A;
+
+#define TRY(x)                                                                                                         \
+    do {                                                                                                               \
+        err = (x);                                                                                                     \
+        if (err) goto done;                                                                                            \
+    } while (0)
+
+    TRY(jxamd_event_record(s->ev0, s->stream));
+    /* P0 = b - A x */
+    TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, VEC(0), 1, 0.0, 0.0, s->stream));
+    TRY(host_norm(s, bvec, &b_norm));
+    if (b_norm != 0.0 && (b_norm / b_norm) != (b_norm / b_norm)) { err = JXAMD_ERROR_GENERIC; goto done; }
+    TRY(host_norm(s, VEC(0), &r_norm));
+    r_norm_0 = r_norm;
+    if (r_norm != 0.0 && (r_norm / r_norm) != (r_norm / r_norm)) { err = JXAMD_ERROR_GENERIC; goto done; }
+    normdown = r_norm;
+    if (res_history) res_history[0] = r_norm;
+    den_norm = b_norm > 0.0 ? b_norm : r_norm;
+    epsilon = a_tol > r_tol * den_norm ? a_tol : r_tol * den_norm;
+
+    while (iter < max_iter) {
+        rs[0] = r_norm;
+        if (r_norm == 0.0) goto done;
+        if (r_norm <= epsilon) {
+            /* false convergence 1 check (pgmres.c:1200-1241) */
+            TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, s->Rv, 1, 0.0, 0.0, s->stream));
+            TRY(host_norm(s, s->Rv, &r_norm));
+            if (r_norm <= epsilon) { converged = 1; break; }
+            if (s->prm.is_check_restarted == 0) break;
+        }
+        TRY(jxamd_k_scale(N, NULL, 1.0 / r_norm, VEC(0), s->stream));
+        i = 0;
+        while (i < k_dim && iter < max_iter) {
+            i++;
+            iter++;
+            TRY(precond_timed(s, VEC(i - 1), s->Rv));
+            nprec++;
+            TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, s->Rv, NULL, VEC(i), 0, 0.0, 0.0, s->stream));
+            /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
+            TRY(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
+            for (j = 1; j < i; j++) TRY(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
+            TRY(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i));
+            TRY(jxamd_k_scale(N, h + i, 0.0, VEC(i), s->stream));
+            TRY(jxamd_d2h(hcol, h, sizeof(double) * (i + 1), s->stream));
+            TRY(accumulate_precond_time(s)); /* also synchronises the stream */
+            for (j = 0; j < i; j++) hh[j][i - 1] = hcol[j];
+            hh[i][i - 1] = sqrt(hcol[i]);
+            for (j = 1; j < i; j++) {
+                t = hh[j - 1][i - 1];
+                hh[j - 1][i - 1] = sn[j - 1] * hh[j][i - 1] + c[j - 1] * t;
+                hh[j][i - 1] = -sn[j - 1] * t + c[j - 1] * hh[j][i - 1];
+            }
+            t = hh[i][i - 1] * hh[i][i - 1];
+            t += hh[i - 1][i - 1] * hh[i - 1][i - 1];
+            gamma = sqrt(t);
+            if (gamma == 0.0) gamma = epsmac;
+            c[i - 1] = hh[i - 1][i - 1] / gamma;
+            sn[i - 1] = hh[i][i - 1] / gamma;
+            rs[i] = -hh[i][i - 1] * rs[i - 1];
+            rs[i] /= gamma;
+            rs[i - 1] = c[i - 1] * rs[i - 1];
+            hh[i - 1][i - 1] = sn[i - 1] * hh[i][i - 1] + c[i - 1] * hh[i - 1][i - 1];
+            r_norm = fabs(rs[i]);
+            normup = r_norm;
+            conv_factor = normup / normdown;
+            normdown = normup;
+            if (res_history) res_history[iter] = r_norm;
+            if (s->prm.print_level > 0) printf("% 5d    %e    %e\n", iter, r_norm, b_norm > 0 ? r_norm / b_norm : r_norm);
+            if (iter > 1 && conv_factor > convfac_threshold) break;
+            if (r_norm <= epsilon) break;
+        }
+        /* back-solve and update x += M^{-1} (sum y_j v_j) (pgmres.c:1477-1503) */
+        rs[i - 1] = rs[i - 1] / hh[i - 1][i - 1];
+        for (k = i - 2; k >= 0; k--) {
+            t = 0.0;
+            for (j = k + 1; j < i; j++) t -= hh[k][j] * rs[j];
+            t += rs[k];
+            rs[k] = t / hh[k][k];
+        }
+        TRY(jxamd_h2d(ydev, rs, sizeof(double) * i, s->stream));
+        TRY(jxamd_k_lincomb(N, i, V, N, ydev, s->Wv, s->stream));
+        TRY(precond_timed(s, s->Wv, s->Rv));
+        nprec++;
+        TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Rv, x, s->stream));
+        TRY(accumulate_precond_time(s));
+        if (r_norm <= epsilon) {
+            TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, s->Rv, 1, 0.0, 0.0, s->stream));
+            TRY(host_norm(s, s->Rv, &r_norm));
+            if (r_norm <= epsilon) { converged = 1; break; }
+            if (s->prm.is_check_restarted == 1) {
+                /* false convergence 2: restart from the true residual */
+                TRY(jxamd_k_copy(N, s->Rv, VEC(0), s->stream));
+                i = 0;
+                nrestart++;
+            } else {
+                break;
+            }
+        }
+        if (iter > 1 && conv_factor > convfac_threshold) break;
+        /* restart residual from the Givens recurrence (pgmres.c:1617-1637) */
+        for (j = i; j > 0; j--) {
+            rs[j - 1] = -sn[j - 1] * rs[j];
+            rs[j] = c[j - 1] * rs[j];
+        }
+        if (i) TRY(jxamd_k_axpy(N, NULL, rs[i] - 1.0, 0, VEC(i), VEC(i), s->stream));
+        for (j = i - 1; j > 0; j--) TRY(jxamd_k_axpy(N, NULL, rs[j], 0, VEC(j), VEC(i), s->stream));
+        if (i) {
+            TRY(jxamd_k_axpy(N, NULL, rs[0] - 1.0, 0, VEC(0), VEC(0), s->stream));
+            TRY(jxamd_k_axpy(N, NULL, 1.0, 0, VEC(i), VEC(0), s->stream));
+            nrestart++;
+        }
+    }
+done:
+    if (!err) err = jxamd_event_record(s->ev1, s->stream);
+    if (!err) err = jxamd_stream_sync(s->stream);
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->iterations = iter;
+        st->converged = converged;
+        st->num_precond = nprec;
+        st->num_restarts = nrestart;
+        st->b_norm = b_norm;
+        st->final_rel_res = b_norm > 0.0 ? r_norm / b_norm : (r_norm_0 > 0.0 ? r_norm / r_norm_0 : 0.0);
+        float ms = 0.f;
+        if (!err) jxamd_event_elapsed(&ms, s->ev0, s->ev1);
+        st->t_solve_ms = ms;
+        st->t_precond_ms = s->t_prec_ms;
+        if (!err) {
+            double tr = 0.0;
+            int e2 = jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, s->Rv, 1, 0.0, 0.0, s->stream);
+            if (!e2) e2 = host_norm(s, s->Rv, &tr);
+            st->true_rel_res = b_norm > 0.0 ? tr / b_norm : tr;
+        }
+    }
+    if (!err) {
+        int ef = 0;
+        if (!jxamd_solver_error_flag(s, &ef) && ef) err = JXAMD_ERROR_GENERIC;
+    }
+    for (int q = 0; q <= k_dim; q++) free(hh[q]);
+    free(hh); free(rs); free(c); free(sn); free(hcol);
+#undef TRY
+#undef VEC
+    return err;
+}
+
+int jxamd_solver_solve(jxamd_solver *s, const double *b, double *x, double *res_history, jxamd_stats *st)
+{
+    if (!s || !b || !x) return JXAMD_ERROR_ARG;
+    CK(jxamd_h2d(s->Bd, b, sizeof(double) * s->N, s->stream));
+    CK(jxamd_h2d(s->Xd, x, sizeof(double) * s->N, s->stream));
+    int e = jxamd_solver_solve_device(s, s->Bd, s->Xd, res_history, st);
+    int e2 = jxamd_d2h(x, s->Xd, sizeof(double) * s->N, s->stream);
+    if (!e2) e2 = jxamd_stream_sync(s->stream);
+    return e ? e : e2;
+}

@@ -1,0 +1,143 @@
+"""TEST INFRASTRUCTURE: ctypes binding of the oracle (oracle/_ref/libcpuref.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker.  Built by `make -C oracle cpuref`.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_ref", "libcpuref.so")
+MAX_LEVELS = 25
+
+
+class CSR(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("ncols", ctypes.c_int), ("ia", ctypes.c_void_p), ("ja", ctypes.c_void_p),
+                ("a", ctypes.c_void_p)]
+
+
+class Hier(ctypes.Structure):
+    _fields_ = [("num_levels", ctypes.c_int), ("A", CSR * MAX_LEVELS), ("P", CSR * MAX_LEVELS)]
+
+
+class BSR(ctypes.Structure):
+    _fields_ = [("nb", ctypes.c_int), ("b", ctypes.c_int), ("ia", ctypes.c_void_p), ("ja", ctypes.c_void_p),
+                ("a", ctypes.c_void_p)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("k_dim", ctypes.c_int), ("max_iter", ctypes.c_int), ("is_check_restarted", ctypes.c_int),
+                ("tol", ctypes.c_double), ("a_tol", ctypes.c_double), ("pressure_index", ctypes.c_int),
+                ("stage1_maxit", ctypes.c_int), ("stage2_maxit", ctypes.c_int), ("stage2_weight", ctypes.c_double),
+                ("stage2_omega", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("num_precond", ctypes.c_int),
+                ("final_rel_res", ctypes.c_double), ("b_norm", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle cpuref`")
+        _lib = ctypes.CDLL(LIB_PATH)
+        vp, f64, i32 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        for name, args, res in [
+            ("cref_bsr_matvec", [vp, f64, vp, f64, vp], None),
+            ("cref_csr_matvec", [vp, f64, vp, f64, vp], None),
+            ("cref_csr_matvecT", [vp, f64, vp, f64, vp], None),
+            ("cref_dot", [i32, vp, vp], f64),
+            ("cref_relax6", [vp, vp, vp], None),
+            ("cref_relax9", [vp, vp, vp], i32),
+            ("cref_gselim", [vp, vp, i32], i32),
+            ("cref_bsr_block_matvec_stable", [f64, vp, vp, f64, vp, i32], None),
+            ("cref_bsr_block_invmatvec_stable", [vp, vp, vp, i32], None),
+            ("cref_bsr_hgs", [vp, vp, vp, vp, f64, f64, i32], None),
+            ("cref_pamg_solve", [vp, vp, vp], i32),
+            ("cref_cpr_precond", [vp, vp, vp, vp, vp], i32),
+            ("cref_gmres_solve", [vp, vp, vp, vp, vp, vp, vp], i32),
+        ]:
+            f = getattr(_lib, name)
+            f.argtypes = args
+            f.restype = res
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Oracle:
+    """Holds ctypes views of one system (BSR matrix + hierarchy) for the oracle."""
+
+    def __init__(self, A, H=None):
+        self.L = lib()
+        self.A = A
+        self.keep = [A.ia, A.ja, A.a]
+        self.bsr = BSR(A.nb, A.b, _p(A.ia).value, _p(A.ja).value, _p(A.a).value)
+        self.hier = Hier()
+        if H is not None:
+            self.hier.num_levels = H.num_levels
+            for l, (ia, ja, a, nc) in enumerate(H.A):
+                self.hier.A[l] = CSR(len(ia) - 1, nc, _p(ia).value, _p(ja).value, _p(a).value)
+                self.keep += [ia, ja, a]
+            for l, (ia, ja, a, nc) in enumerate(H.P):
+                self.hier.P[l] = CSR(len(ia) - 1, nc, _p(ia).value, _p(ja).value, _p(a).value)
+                self.keep += [ia, ja, a]
+
+    @staticmethod
+    def params(k_dim=30, max_iter=100, is_check_restarted=1, tol=1e-6, a_tol=0.0, pressure_index=0,
+               stage1_maxit=1, stage2_maxit=3, stage2_weight=0.7, stage2_omega=0.9):
+        return Params(k_dim, max_iter, is_check_restarted, tol, a_tol, pressure_index, stage1_maxit, stage2_maxit,
+                      stage2_weight, stage2_omega)
+
+    def matvec(self, alpha, x, beta, y):
+        y = np.array(y, dtype=np.float64, copy=True)
+        self.L.cref_bsr_matvec(ctypes.byref(self.bsr), alpha, _p(np.ascontiguousarray(x)), beta, _p(y))
+        return y
+
+    def relax6(self, level, f, u):
+        u = np.array(u, dtype=np.float64, copy=True)
+        self.L.cref_relax6(ctypes.byref(self.hier.A[level]), _p(np.ascontiguousarray(f)), _p(u))
+        return u
+
+    def relax9(self, f):
+        L = self.hier.num_levels
+        u = np.zeros(self.hier.A[L - 1].n)
+        self.L.cref_relax9(ctypes.byref(self.hier.A[L - 1]), _p(np.ascontiguousarray(f)), _p(u))
+        return u
+
+    def vcycle(self, f, u=None):
+        u = np.zeros(self.hier.A[0].n) if u is None else np.array(u, dtype=np.float64, copy=True)
+        self.L.cref_pamg_solve(ctypes.byref(self.hier), _p(np.ascontiguousarray(f)), _p(u))
+        return u
+
+    def bhgs(self, x, rhs, w=0.7, omega=0.9, fwd=1):
+        x = np.array(x, dtype=np.float64, copy=True)
+        work = np.zeros_like(x)
+        self.L.cref_bsr_hgs(ctypes.byref(self.bsr), _p(x), _p(np.ascontiguousarray(rhs)), _p(work), w, omega, fwd)
+        return x
+
+    def cpr(self, r, params=None):
+        p = params if params is not None else self.params()
+        x = np.zeros(self.A.n)
+        rc = self.L.cref_cpr_precond(ctypes.byref(self.bsr), ctypes.byref(self.hier), ctypes.byref(p),
+                                     _p(np.ascontiguousarray(r)), _p(x))
+        assert rc == 0
+        return x
+
+    def gmres(self, b, params=None, x0=None):
+        p = params if params is not None else self.params()
+        x = np.zeros(self.A.n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        hist = np.zeros(p.max_iter + 1)
+        st = Stats()
+        rc = self.L.cref_gmres_solve(ctypes.byref(self.bsr), ctypes.byref(self.hier), ctypes.byref(p),
+                                     _p(np.ascontiguousarray(b)), _p(x), _p(hist), ctypes.byref(st))
+        return x, st, hist[: st.iterations + 1], rc

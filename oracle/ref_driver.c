@@ -12,7 +12,7 @@
  * and intermediate objects are dumped as golden vectors.  Nothing in the
  * product links or runs this program.
  *
- * usage: ref_driver A.bin b.txt out.dump rtol max_iter stage2_maxit decouple(0/1) dump_level
+ * usage: ref_driver A.bin b.txt out.dump rtol max_iter stage2_maxit decouple(0/1) dump_level [k_dim]
  *   dump_level 0: scalars + residual history only; 1: + hierarchy and
  *   single-operator vectors; 2: + decoupled system.
  */
@@ -94,6 +94,7 @@ int main(int argc, char **argv)
     int stage2_maxit = atoi(argv[6]);
     int decouple = atoi(argv[7]);
     int dump_level = atoi(argv[8]);
+    int k_dim = argc > 9 ? atoi(argv[9]) : 30;
     g_out = fopen(argv[3], "wb");
     if (!g_out) { fprintf(stderr, "cannot open %s\n", argv[3]); return 1; }
 
@@ -236,7 +237,7 @@ int main(int argc, char **argv)
     /* 5. GMRES (src/test_bsr_cprgmres.c:346-392) */
     JXF_Solver gmres;
     JXF_ParBSRGMRESCreate(MPI_COMM_WORLD, &gmres);
-    JXF_GMRESSetKDim(gmres, 30);
+    JXF_GMRESSetKDim(gmres, k_dim);
     JXF_GMRESSetIsCheckRestarted(gmres, 1);
     JXF_GMRESSetMaxIter(gmres, max_iter);
     JXF_GMRESSetTol(gmres, rtol);

@@ -1,0 +1,188 @@
+"""Pins the oracle (oracle/cpu_ref.c) two ways:
+
+1. against the golden vectors written by the reference itself (FP64 build of
+   jxfpamg-0220 driven like src/test_bsr_cprgmres.c): bit-exact;
+2. against what the mathematics fixes, independently of the reference:
+   dense brute force, triangular solves, LAPACK solves, GMRES invariants.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle.cpu_ref import Oracle
+from tests._common import relerr, system
+
+OP_CASES = ["c1", "tiny", "b4", "b2", "lvl3", "nodec"]
+
+
+@pytest.mark.parametrize("name", OP_CASES)
+def test_operators_bit_exact_vs_reference(name):
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    if H is None:
+        pytest.skip("fixture has no operator dumps")
+    o = Oracle(Ad, H)
+    assert np.array_equal(o.relax6(0, d["hsgs_f"], d["hsgs_u_in"]), d["hsgs_u_out"])
+    assert np.array_equal(o.relax9(d["coarse_f"]), d["coarse_u"])
+    assert np.array_equal(o.vcycle(d["vcycle_f"]), d["vcycle_u"])
+    assert np.array_equal(o.bhgs(d["bhgs_x_in"], d["bhgs_b"], 0.7, 0.9), d["bhgs_x_out"])
+    p = o.params(stage2_maxit=int(meta["stage2_maxit"]))
+    assert np.array_equal(o.cpr(d["cpr_r"], p), d["cpr_x"])
+
+
+@pytest.mark.parametrize("name", ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "restart", "layered"])
+def test_gmres_matches_reference(name):
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    if H is None:
+        pytest.skip("needs the reference hierarchy")
+    o = Oracle(Ad, H)
+    p = o.params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]), tol=meta["rtol"],
+                 stage2_maxit=int(meta["stage2_maxit"]))
+    x, st, hist, rc = o.gmres(bd, p)
+    assert rc == 0
+    assert st.iterations == int(d["iterations"][0])
+    assert np.array_equal(hist, d["res_history"])
+    assert st.final_rel_res == d["final_rel_res"][0]
+    if "x_final" in d:
+        assert np.array_equal(x, d["x_final"])
+
+
+def _small_pressure_system(n=40, seed=0):
+    rng = np.random.default_rng(seed)
+    M = sp.random(n, n, density=0.15, random_state=seed, format="csr")
+    M = -abs(M) - abs(M.T)
+    M = M.tolil()
+    M.setdiag(0)
+    M = M.tocsr()
+    d = -np.asarray(M.sum(axis=1)).ravel() + 0.5 + rng.random(n)
+    A = (M + sp.diags(d)).tocsr()
+    # diagonal first in every row
+    ia, ja, a = [0], [], []
+    for i in range(n):
+        row = A.getrow(i)
+        cols = list(row.indices)
+        vals = list(row.data)
+        k = cols.index(i)
+        cols = [cols[k]] + cols[:k] + cols[k + 1:]
+        vals = [vals[k]] + vals[:k] + vals[k + 1:]
+        ja += cols
+        a += vals
+        ia.append(len(ja))
+    return A, (np.array(ia, np.int32), np.array(ja, np.int32), np.array(a), n)
+
+
+def test_relax6_is_forward_then_backward_triangular_solve():
+    import jxpamg_all_amd as jx
+    A, csr = _small_pressure_system()
+    n = A.shape[0]
+    H = jx.Hierarchy([csr], [])
+    # dummy BSR to build the oracle view
+    Ab = jx.BSRMatrix(1, 2, np.array([0, 1], np.int32), np.array([0], np.int32), np.eye(2).ravel())
+    o = Oracle(Ab, H)
+    rng = np.random.default_rng(1)
+    f, u0 = rng.standard_normal(n), rng.standard_normal(n)
+    got = o.relax6(0, f, u0)
+    L = sp.tril(A, 0).tocsr()
+    U = sp.triu(A, 1).tocsr()
+    u1 = spla.spsolve_triangular(L, f - U @ u0, lower=True)
+    Lo = sp.tril(A, -1).tocsr()
+    Ut = sp.triu(A, 0).tocsr()
+    u2 = spla.spsolve_triangular(Ut, f - Lo @ u1, lower=False)
+    assert relerr(got, u2) < 1e-13
+
+
+def test_gselim_solves_diagonally_dominant_systems():
+    import ctypes
+    from oracle.cpu_ref import lib
+    rng = np.random.default_rng(2)
+    n = 30
+    M = rng.standard_normal((n, n)) + n * np.eye(n)
+    b = rng.standard_normal(n)
+    Mc, x = M.copy(), b.copy()
+    assert lib().cref_gselim(ctypes.c_void_p(Mc.ctypes.data), ctypes.c_void_p(x.ctypes.data), n) == 0
+    assert relerr(x, np.linalg.solve(M, b)) < 1e-13
+
+
+@pytest.mark.parametrize("b", [2, 3, 4])
+def test_block_kernels(b):
+    import ctypes
+    from oracle.cpu_ref import lib
+    rng = np.random.default_rng(b)
+    for trial in range(20):
+        M = rng.standard_normal((b, b))
+        if trial % 3 == 0:
+            M[0, 0] = 1e-3  # forces a row exchange in the pivoted LU
+        v = rng.standard_normal(b)
+        out = np.zeros(b)
+        lib().cref_bsr_block_invmatvec_stable(ctypes.c_void_p(M.ctypes.data), ctypes.c_void_p(v.ctypes.data),
+                                              ctypes.c_void_p(out.ctypes.data), b)
+        assert relerr(out, np.linalg.solve(M, v)) < 1e-10
+        ov = rng.standard_normal(b)
+        ref = ov - M @ v
+        lib().cref_bsr_block_matvec_stable(-1.0, ctypes.c_void_p(M.ctypes.data), ctypes.c_void_p(v.ctypes.data),
+                                           1.0, ctypes.c_void_p(ov.ctypes.data), b)
+        assert relerr(ov, ref) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["tiny", "b4", "b2"])
+def test_bsr_matvec_vs_scipy(name):
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    o = Oracle(Ad)
+    S = Ad.to_scipy()
+    rng = np.random.default_rng(3)
+    x, y = rng.standard_normal(Ad.n), rng.standard_normal(Ad.n)
+    assert relerr(o.matvec(1.0, x, 0.0, y), S @ x) < 1e-13
+    assert relerr(o.matvec(-1.0, x, 1.0, y), y - S @ x) < 1e-13
+    # A * 1 = block row sums
+    one = np.ones(Ad.n)
+    sums = Ad.a.reshape(-1, Ad.b, Ad.b).sum(axis=2)
+    rows = np.repeat(np.arange(Ad.nb), np.diff(Ad.ia))
+    expect = np.zeros((Ad.nb, Ad.b))
+    np.add.at(expect, rows, sums)
+    assert relerr(o.matvec(1.0, one, 0.0, one), expect.ravel()) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["tiny", "b4", "b2"])
+def test_bhgs_unit_weights_is_block_lower_solve(name):
+    """w = omega = 1: one forward sweep equals x1 = (D + L)^{-1} (b - U x0)."""
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    if not int(meta["decouple"]):
+        pytest.skip()
+    o = Oracle(Ad)
+    S = Ad.to_scipy().tocsr()
+    b = Ad.b
+    rng = np.random.default_rng(4)
+    x0, r = rng.standard_normal(Ad.n), rng.standard_normal(Ad.n)
+    got = o.bhgs(x0, r, 1.0, 1.0)
+    blk_row = np.arange(Ad.n) // b
+    S = S.tocoo()
+    low = blk_row[S.col] <= blk_row[S.row]
+    DL = sp.csr_matrix((S.data[low], (S.row[low], S.col[low])), shape=S.shape)
+    Up = sp.csr_matrix((S.data[~low], (S.row[~low], S.col[~low])), shape=S.shape)
+    expect = spla.spsolve_triangular(DL, r - Up @ x0, lower=True)
+    assert relerr(got, expect) < 1e-12
+
+
+def test_gmres_invariants_and_true_residual():
+    d, A, rhs, Ad, bd, H, meta = system("c1")
+    o = Oracle(Ad, H)
+    x, st, hist, rc = o.gmres(bd, o.params(tol=1e-8))
+    S = Ad.to_scipy()
+    true = np.linalg.norm(bd - S @ x) / np.linalg.norm(bd)
+    assert st.converged == 1 and true <= 1e-8
+    # within one restart cycle the GMRES residual estimate never increases
+    assert np.all(np.diff(hist) <= 1e-15 * hist[0])
+
+
+def test_vcycle_is_linear_and_reduces_error():
+    d, A, rhs, Ad, bd, H, meta = system("lvl3")
+    o = Oracle(Ad, H)
+    rng = np.random.default_rng(5)
+    f1, f2 = rng.standard_normal(H.A[0][0].size - 1), rng.standard_normal(H.A[0][0].size - 1)
+    u1, u2, u12 = o.vcycle(f1), o.vcycle(f2), o.vcycle(2.0 * f1 - 3.0 * f2)
+    assert relerr(u12, 2.0 * u1 - 3.0 * u2) < 1e-10
+    ia, ja, a, _ = H.A[0]
+    Ap = sp.csr_matrix((a, ja, ia))
+    e = np.linalg.norm(f1 - Ap @ u1) / np.linalg.norm(f1)
+    assert e < 0.5
