@@ -29,13 +29,15 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
                      const double *rhs, double *y, int mode, double alpha, double beta, void *stream);
 int jxamd_k_csr_spmv(int n, const int *sptr, const int *col, const double *val, const double *x, int xs,
                      const double *rhs, int rs, double *y, int mode, void *stream);
-int jxamd_k_gs_sweep(int n, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
-                     const double *f, int fs, const double *uo, double *un, int *flag, int epoch, int dir,
-                     unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
-int jxamd_k_bhgs_sweep(int b, int nb, const int *perm, const int *sptr, const int *col, const double *val,
+int jxamd_sweep_waves(int nsl);
+int jxamd_k_gs_sweep(int nsl, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
+                     const double *f, int fs, const double *uo, double *un, const int *dep_ptr, const int *dep_list,
+                     int *sflag, int epoch, int dir, unsigned long long *ticket, unsigned long long *base, int *err,
+                     void *stream);
+int jxamd_k_bhgs_sweep(int b, int nsl, const int *perm, const int *sptr, const int *col, const double *val,
                        const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
-                       double *xn, int *flag, int epoch, double w, double omega, unsigned long long *ticket,
-                       unsigned long long *base, int *err, void *stream);
+                       double *xn, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, double w,
+                       double omega, unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
 int jxamd_k_dense_gemv(int n, const double *M, const double *x, double *y, void *stream);
 int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream);
 int jxamd_k_prolong(int nb, int b, int pidx, const double *xp, double *x, void *stream);

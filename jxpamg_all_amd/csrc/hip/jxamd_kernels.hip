@@ -12,14 +12,9 @@
 //   positions are sorted by dependency level so a wavefront's rows become
 //   ready together.  The diagonal is kept apart (diag[p]).
 //
-// Sync-free sweeps: a row's new value is published with a release store of
-// the sweep's epoch into flag[row]; consumers poll with relaxed agent-scope
-// loads and take an acquire fence once all their dependencies are present.
-// Workgroups claim position tiles from a monotonically increasing ticket
-// counter, so every tile's dependencies belong to tiles already claimed by
-// running workgroups; each wavefront retries its unfinished lanes (no lane
-// blocks the wave), and every wait is bounded: on overflow the kernel records
-// JXAMD_ERROR_GENERIC in err[0] and finishes.
+// Sync-free sweeps (k_gs_sweep, k_bhgs_sweep) run persistent wavefronts over
+// level-pure slices with per-slice completion flags; every wait is bounded
+// and an overflow is recorded in err[0] (see the comment at the kernels).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -135,158 +130,168 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// One Gauss-Seidel sweep of hybrid SSOR (relax 6, weights 1), sync-free.
-//   dir > 0: forward (dependencies j < i), dir < 0: backward (j > i).
-//   Reads old values from uo (or zeros if uo == nullptr), writes un.
-//   f may be strided (fs).  Rows with a zero diagonal copy their old value.
+// Sync-free sweeps over level-pure wavefront slices.
+//
+// The host sorts rows by dependency level and pads every level to a multiple
+// of 64 positions, so a slice (one wavefront's 64 positions, perm[p] < 0 for
+// padding) never holds a row and one of its dependencies.  dep_ptr/dep_list
+// give, per slice, the earlier slices holding its rows' dependencies.  Waves
+// are persistent: each claims slices in order from the ticket counter (so a
+// claimed slice's dependencies belong to slices claimed before it, all held
+// by running waves), loads its row data, waits on its dependency slices'
+// flags (bounded), takes an acquire fence, computes, stores with device-
+// coherent stores and publishes sflag[slice] = epoch with release semantics.
+// Every wave ends with exactly one failed claim, so a launch consumes
+// nsl + (number of waves) tickets.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_gs_sweep(int n, const int *__restrict__ perm, const int *__restrict__ sptr,
+__device__ __forceinline__ int claim_slice(unsigned long long *ticket, unsigned long long base, int lane)
+{
+    int s = 0;
+    if (lane == 0) s = (int)(__hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
+    return __shfl(s, 0);
+}
+
+__device__ __forceinline__ void wait_slices(const int *__restrict__ dep_ptr, const int *__restrict__ dep_list,
+                                            const int *sflag, int s, int epoch, int lane, int *err)
+{
+    const int d0 = dep_ptr[s], d1 = dep_ptr[s + 1];
+    for (int q = d0 + lane; q < d1; q += WAVE) {
+        const int w = dep_list[q];
+        int spins = 0, nap = 0;
+        while (!flag_ready(ld_flag(sflag + w), epoch)) {
+            if (++spins > SPIN_LIMIT) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            if (nap < 4) { __builtin_amdgcn_s_sleep(1); nap++; }
+            else __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// One Gauss-Seidel sweep of hybrid SSOR (relax 6, weights 1).
+//   dir > 0: forward (dependencies j < i), dir < 0: backward (j > i).
+//   Old values from uo (zeros if uo == nullptr), new values into un.
+//   f may be strided (fs).  A row with a zero diagonal keeps its old value.
+__global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict__ perm, const int *__restrict__ sptr,
                                                   const int *__restrict__ col, const double *__restrict__ val,
                                                   const double *__restrict__ diag, const double *__restrict__ f, int fs,
-                                                  const double *uo, double *un, int *flag, int epoch, int dir,
+                                                  const double *uo, double *un, const int *__restrict__ dep_ptr,
+                                                  const int *__restrict__ dep_list, int *sflag, int epoch, int dir,
                                                   unsigned long long *ticket, unsigned long long base, int *err)
 {
-    __shared__ int s_tile;
-    if (threadIdx.x == 0)
-        s_tile = (int)(__hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
-    __syncthreads();
-    const int s = s_tile * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
-    const int nsl = (n + WAVE - 1) / WAVE;
-    if (s_tile < 0 || s >= nsl) return;
-    const int p = s * WAVE + lane;
-    const bool active = p < n;
-    const int i = active ? perm[p] : 0;
-    const int k0 = sptr[s], k1 = sptr[s + 1];
-    bool done = !active;
-    int spins = 0;
-    while (true) {
-        if (!done) {
-            bool ready = true;
-            for (int k = k0; k < k1 && ready; k++) {
-                const int j = col[(size_t)k * WAVE + lane];
+    for (;;) {
+        const int s = claim_slice(ticket, base, lane);
+        if (s < 0 || s >= nsl) break;
+        const int p = s * WAVE + lane;
+        const int i = perm[p];
+        const int k0 = sptr[s], k1 = sptr[s + 1];
+        double res = 0.0, d = 1.0, uold_i = 0.0;
+        if (i >= 0) {
+            d = diag[p];
+            res = f[(size_t)i * fs];
+            if (uo) uold_i = uo[i];
+            // terms on old values need no wait
+            for (int k = k0; k < k1; k++) {
+                const size_t e = (size_t)k * WAVE + lane;
+                const int j = col[e];
                 const bool dep = dir > 0 ? (j < i) : (j > i);
-                if (dep && !flag_ready(ld_flag(flag + j), epoch)) ready = false;
-            }
-            if (ready) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                const double d = diag[p];
-                double res = f[(size_t)i * fs];
-                double nv;
-                if (d != 0.0) {
-                    for (int k = k0; k < k1; k++) {
-                        const size_t e = (size_t)k * WAVE + lane;
-                        const int j = col[e];
-                        if (j == i) continue; // padding slot
-                        const bool dep = dir > 0 ? (j < i) : (j > i);
-                        const double uj = dep ? ld_coherent(un + j) : (uo ? uo[j] : 0.0);
-                        res -= val[e] * uj;
-                    }
-                    nv = res / d;
-                } else {
-                    nv = uo ? uo[i] : 0.0;
-                }
-                st_coherent(un + i, nv);
-                st_flag_release(flag + i, epoch);
-                done = true;
+                if (!dep && j != i && uo) res -= val[e] * uo[j];
             }
         }
-        if (__all(done)) break;
-        if (++spins > SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
+        wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
+        if (i >= 0) {
+            for (int k = k0; k < k1; k++) {
+                const size_t e = (size_t)k * WAVE + lane;
+                const int j = col[e];
+                const bool dep = dir > 0 ? (j < i) : (j > i);
+                if (dep) res -= val[e] * ld_coherent(un + j);
+            }
+            st_coherent(un + i, d != 0.0 ? res / d : uold_i);
         }
-        __builtin_amdgcn_s_sleep(1);
+        st_flag_release(sflag + s, epoch);
     }
 }
 
-// ---------------------------------------------------------------------------
 // One forward block hybrid SOR sweep on the BSR matrix (par_relax_bsr.c
-// semantics, one rank), sync-free over block rows:
+// semantics, one rank):
 //   tmp = omega*b_i - sum_{j!=i} A_ij x_j(cur) + (1-omega) sum_{j!=i} A_ij x_j(old)
 //   x_i <- (1 - w*omega) x_i(old) + w * Dinv_i tmp
 // Old values come from xo, or, when xo == nullptr, from the prolongated
 // pressure vector xp (pressure slot pidx, zeros elsewhere).  Writes xn.
 // Dependencies are the lower block columns (j < i).
-// ---------------------------------------------------------------------------
 template <int B>
-__global__ void __launch_bounds__(256) k_bhgs_sweep(int nb, const int *__restrict__ perm, const int *__restrict__ sptr,
+__global__ void __launch_bounds__(256) k_bhgs_sweep(int nsl, const int *__restrict__ perm, const int *__restrict__ sptr,
                                                     const int *__restrict__ col, const double *__restrict__ val,
                                                     const double *__restrict__ dinv, const double *__restrict__ rhs,
                                                     const double *xo, const double *__restrict__ xp, int pidx,
-                                                    double *xn, int *flag, int epoch, double w, double omega,
-                                                    unsigned long long *ticket, unsigned long long base, int *err)
+                                                    double *xn, const int *__restrict__ dep_ptr,
+                                                    const int *__restrict__ dep_list, int *sflag, int epoch, double w,
+                                                    double omega, unsigned long long *ticket, unsigned long long base,
+                                                    int *err)
 {
-    __shared__ int s_tile;
-    if (threadIdx.x == 0)
-        s_tile = (int)(__hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
-    __syncthreads();
-    const int s = s_tile * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
-    const int nsl = (nb + WAVE - 1) / WAVE;
-    if (s_tile < 0 || s >= nsl) return;
-    const int p = s * WAVE + lane;
-    const bool active = p < nb;
-    const int i = active ? perm[p] : 0;
-    const int k0 = sptr[s], k1 = sptr[s + 1];
-    bool done = !active;
-    int spins = 0;
-    while (true) {
-        if (!done) {
-            bool ready = true;
-            for (int k = k0; k < k1 && ready; k++) {
+    for (;;) {
+        const int s = claim_slice(ticket, base, lane);
+        if (s < 0 || s >= nsl) break;
+        const int p = s * WAVE + lane;
+        const int i = perm[p];
+        const int k0 = sptr[s], k1 = sptr[s + 1];
+        double res0[B], res2[B];
+#pragma unroll
+        for (int r = 0; r < B; r++) { res0[r] = 0.0; res2[r] = 0.0; }
+        if (i >= 0) {
+            // all old-value terms, and the current-value terms of upper neighbours
+            for (int k = k0; k < k1; k++) {
                 const int j = col[(size_t)k * WAVE + lane];
-                if (j < i && !flag_ready(ld_flag(flag + j), epoch)) ready = false;
-            }
-            if (ready) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                double res0[B], res2[B];
+                if (j == i) continue; // padding slot
+                double xold[B];
 #pragma unroll
-                for (int r = 0; r < B; r++) { res0[r] = 0.0; res2[r] = 0.0; }
-                for (int k = k0; k < k1; k++) {
-                    const int j = col[(size_t)k * WAVE + lane];
-                    if (j == i) continue; // padding slot
-                    double xold[B], xcur[B];
-#pragma unroll
-                    for (int t = 0; t < B; t++) {
-                        xold[t] = xo ? xo[(size_t)j * B + t] : (t == pidx ? xp[j] : 0.0);
-                        xcur[t] = (j < i) ? ld_coherent(xn + (size_t)j * B + t) : xold[t];
-                    }
-                    const double *v = val + (size_t)k * B * B * WAVE + lane;
-#pragma unroll
-                    for (int r = 0; r < B; r++) {
-#pragma unroll
-                        for (int t = 0; t < B; t++) {
-                            const double a = v[(r * B + t) * WAVE];
-                            res0[r] -= a * xcur[t];
-                            res2[r] += a * xold[t];
-                        }
-                    }
-                }
-                double tmp[B];
-#pragma unroll
-                for (int r = 0; r < B; r++)
-                    tmp[r] = omega * rhs[(size_t)i * B + r] + res0[r] + (1.0 - omega) * res2[r];
-                const double prod = 1.0 - w * omega;
-                const double *D = dinv + (size_t)i * B * B;
+                for (int t = 0; t < B; t++) xold[t] = xo ? xo[(size_t)j * B + t] : (t == pidx ? xp[j] : 0.0);
+                const double *v = val + (size_t)k * B * B * WAVE + lane;
 #pragma unroll
                 for (int r = 0; r < B; r++) {
-                    double o = 0.0;
 #pragma unroll
-                    for (int t = 0; t < B; t++) o += D[r * B + t] * tmp[t];
-                    const double xi = xo ? xo[(size_t)i * B + r] : (r == pidx ? xp[i] : 0.0);
-                    st_coherent(xn + (size_t)i * B + r, xi * prod + w * o);
+                    for (int t = 0; t < B; t++) {
+                        const double a = v[(r * B + t) * WAVE];
+                        res2[r] += a * xold[t];
+                        if (j > i) res0[r] -= a * xold[t];
+                    }
                 }
-                st_flag_release(flag + i, epoch);
-                done = true;
             }
         }
-        if (__all(done)) break;
-        if (++spins > SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
+        wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
+        if (i >= 0) {
+            for (int k = k0; k < k1; k++) {
+                const int j = col[(size_t)k * WAVE + lane];
+                if (j >= i) continue;
+                double xc[B];
+#pragma unroll
+                for (int t = 0; t < B; t++) xc[t] = ld_coherent(xn + (size_t)j * B + t);
+                const double *v = val + (size_t)k * B * B * WAVE + lane;
+#pragma unroll
+                for (int r = 0; r < B; r++) {
+#pragma unroll
+                    for (int t = 0; t < B; t++) res0[r] -= v[(r * B + t) * WAVE] * xc[t];
+                }
+            }
+            double tmp[B];
+#pragma unroll
+            for (int r = 0; r < B; r++) tmp[r] = omega * rhs[(size_t)i * B + r] + res0[r] + (1.0 - omega) * res2[r];
+            const double prod = 1.0 - w * omega;
+            const double *D = dinv + (size_t)i * B * B;
+#pragma unroll
+            for (int r = 0; r < B; r++) {
+                double o = 0.0;
+#pragma unroll
+                for (int t = 0; t < B; t++) o += D[r * B + t] * tmp[t];
+                const double xi = xo ? xo[(size_t)i * B + r] : (r == pidx ? xp[i] : 0.0);
+                st_coherent(xn + (size_t)i * B + r, xi * prod + w * o);
+            }
         }
-        __builtin_amdgcn_s_sleep(1);
+        st_flag_release(sflag + s, epoch);
     }
 }
 
@@ -511,35 +516,43 @@ int jxamd_k_csr_spmv(int n, const int *sptr, const int *col, const double *val, 
     return chk(hipGetLastError());
 }
 
-int jxamd_k_gs_sweep(int n, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
-                     const double *f, int fs, const double *uo, double *un, int *flag, int epoch, int dir,
-                     unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+int jxamd_sweep_waves(int nsl)
 {
-    const int nsl = (n + WAVE - 1) / WAVE;
-    const int grid = (nsl + 3) / 4;
-    if (grid == 0) return 0;
-    k_gs_sweep<<<grid, 256, 0, (hipStream_t)stream>>>(n, perm, sptr, col, val, diag, f, fs, uo, un, flag, epoch, dir,
-                                                      ticket, *base, err);
-    *base += (unsigned long long)grid;
+    const int cap = 2048; /* persistent wavefronts: 8 per CU */
+    return nsl < cap ? (nsl > 0 ? nsl : 1) : cap;
+}
+
+int jxamd_k_gs_sweep(int nsl, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
+                     const double *f, int fs, const double *uo, double *un, const int *dep_ptr, const int *dep_list,
+                     int *sflag, int epoch, int dir, unsigned long long *ticket, unsigned long long *base, int *err,
+                     void *stream)
+{
+    if (nsl == 0) return 0;
+    const int waves = jxamd_sweep_waves(nsl);
+    const int grid = (waves + 3) / 4;
+    k_gs_sweep<<<grid, 256, 0, (hipStream_t)stream>>>(nsl, perm, sptr, col, val, diag, f, fs, uo, un, dep_ptr,
+                                                      dep_list, sflag, epoch, dir, ticket, *base, err);
+    *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
     return chk(hipGetLastError());
 }
 
-int jxamd_k_bhgs_sweep(int b, int nb, const int *perm, const int *sptr, const int *col, const double *val,
+int jxamd_k_bhgs_sweep(int b, int nsl, const int *perm, const int *sptr, const int *col, const double *val,
                        const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
-                       double *xn, int *flag, int epoch, double w, double omega, unsigned long long *ticket,
-                       unsigned long long *base, int *err, void *stream)
+                       double *xn, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, double w,
+                       double omega, unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
 {
-    const int nsl = (nb + WAVE - 1) / WAVE;
-    const int grid = (nsl + 3) / 4;
-    if (grid == 0) return 0;
+    if (nsl == 0) return 0;
+    const int waves = jxamd_sweep_waves(nsl);
+    const int grid = (waves + 3) / 4;
     hipStream_t st = (hipStream_t)stream;
+    const unsigned long long b0 = *base;
     switch (b) {
-    case 2: k_bhgs_sweep<2><<<grid, 256, 0, st>>>(nb, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, flag, epoch, w, omega, ticket, *base, err); break;
-    case 3: k_bhgs_sweep<3><<<grid, 256, 0, st>>>(nb, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, flag, epoch, w, omega, ticket, *base, err); break;
-    case 4: k_bhgs_sweep<4><<<grid, 256, 0, st>>>(nb, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, flag, epoch, w, omega, ticket, *base, err); break;
+    case 2: k_bhgs_sweep<2><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, dep_ptr, dep_list, sflag, epoch, w, omega, ticket, b0, err); break;
+    case 3: k_bhgs_sweep<3><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, dep_ptr, dep_list, sflag, epoch, w, omega, ticket, b0, err); break;
+    case 4: k_bhgs_sweep<4><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, dep_ptr, dep_list, sflag, epoch, w, omega, ticket, b0, err); break;
     default: return JXAMD_ERROR_ARG;
     }
-    *base += (unsigned long long)grid;
+    *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
     return chk(hipGetLastError());
 }
 

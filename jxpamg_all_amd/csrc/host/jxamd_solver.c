@@ -30,6 +30,7 @@
 typedef struct {
     int n, ncols, nsl;
     int *sptr, *col, *perm, *flag;
+    int *dep_ptr, *dep_list; /* sweep layouts: dependency slices per slice */
     double *val, *diag;
 } dsell;
 
@@ -39,7 +40,7 @@ typedef struct {
     double *val;
     /* forward block-HGS order */
     int gnsl;
-    int *gperm, *gsptr, *gcol, *flag;
+    int *gperm, *gsptr, *gcol, *flag, *gdep_ptr, *gdep_list;
     double *gval, *dinv;
 } dbsr;
 
@@ -99,41 +100,95 @@ void jxamd_params_default(jxamd_params *p)
 /* host-side layout construction                                           */
 /* ---------------------------------------------------------------------- */
 
-/* Dependency levels of one GS sweep: forward (dir>0) uses the strictly lower
- * part, backward the strictly upper part.  perm lists rows by (level, row). */
-static int level_perm(int n, const int *ia, const int *ja, int dir, int *perm)
+/* Sweep schedule of one GS direction: forward (dir>0) depends on the strictly
+ * lower part, backward on the strictly upper part.  Rows are sorted by
+ * dependency level and every level is padded to a multiple of 64 positions
+ * (perm = -1), so no 64-wide slice holds a row together with one of its
+ * dependencies.  dep_ptr/dep_list: per slice, the distinct earlier slices
+ * that hold its rows' dependencies. */
+typedef struct {
+    int npos, nsl, nlevels;
+    int *perm, *dep_ptr, *dep_list;
+} hsched;
+
+static void hsched_free(hsched *h)
 {
+    free(h->perm); free(h->dep_ptr); free(h->dep_list);
+    memset(h, 0, sizeof(*h));
+}
+
+static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *o)
+{
+    memset(o, 0, sizeof(*o));
     int *lev = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
     if (!lev) return JXAMD_ERROR_MEMORY;
-    int maxlev = 0;
-    if (dir > 0) {
-        for (int i = 0; i < n; i++) {
-            int l = 0;
-            for (int k = ia[i]; k < ia[i + 1]; k++) {
-                int j = ja[k];
-                if (j < i && lev[j] + 1 > l) l = lev[j] + 1;
-            }
-            lev[i] = l;
-            if (l > maxlev) maxlev = l;
+    int maxlev = -1;
+    for (int t = 0; t < n; t++) {
+        int i = dir > 0 ? t : n - 1 - t;
+        int l = 0;
+        for (int k = ia[i]; k < ia[i + 1]; k++) {
+            int j = ja[k];
+            if ((dir > 0 ? j < i : j > i) && lev[j] + 1 > l) l = lev[j] + 1;
         }
-    } else {
-        for (int i = n - 1; i >= 0; i--) {
-            int l = 0;
+        lev[i] = l;
+        if (l > maxlev) maxlev = l;
+    }
+    const int nl = maxlev + 1;
+    long *start = (long *)calloc((size_t)nl + 1, sizeof(long));
+    int *cnt = (int *)calloc((size_t)nl + 1, sizeof(int));
+    if (!start || !cnt) { free(lev); free(start); free(cnt); return JXAMD_ERROR_MEMORY; }
+    for (int i = 0; i < n; i++) cnt[lev[i]]++;
+    for (int l = 0; l < nl; l++) start[l + 1] = start[l] + ((cnt[l] + SELLW - 1) / SELLW) * SELLW;
+    if (start[nl] > 0x7fffffffL) { free(lev); free(start); free(cnt); return JXAMD_ERROR_ARG; }
+    o->npos = (int)start[nl];
+    o->nsl = o->npos / SELLW;
+    o->nlevels = nl;
+    o->perm = (int *)malloc(sizeof(int) * (o->npos > 0 ? o->npos : 1));
+    int *sl = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    if (!o->perm || !sl) { free(lev); free(start); free(cnt); free(sl); return JXAMD_ERROR_MEMORY; }
+    for (int p = 0; p < o->npos; p++) o->perm[p] = -1;
+    for (int l = 0; l < nl; l++) cnt[l] = 0;
+    for (int i = 0; i < n; i++) { /* rows in ascending order inside a level */
+        long p = start[lev[i]] + cnt[lev[i]]++;
+        o->perm[p] = i;
+        sl[i] = (int)(p / SELLW);
+    }
+    /* dependency slices */
+    o->dep_ptr = (int *)malloc(sizeof(int) * ((size_t)o->nsl + 1));
+    int *mark = (int *)malloc(sizeof(int) * ((size_t)o->nsl + 1));
+    size_t cap = (size_t)o->nsl * 4 + 16, used = 0;
+    o->dep_list = (int *)malloc(sizeof(int) * cap);
+    if (!o->dep_ptr || !mark || !o->dep_list) {
+        free(lev); free(start); free(cnt); free(sl); free(mark);
+        return JXAMD_ERROR_MEMORY;
+    }
+    for (int q = 0; q < o->nsl; q++) mark[q] = -1;
+    int bad = 0;
+    for (int q = 0; q < o->nsl; q++) {
+        o->dep_ptr[q] = (int)used;
+        for (int l = 0; l < SELLW; l++) {
+            int i = o->perm[q * SELLW + l];
+            if (i < 0) continue;
             for (int k = ia[i]; k < ia[i + 1]; k++) {
                 int j = ja[k];
-                if (j > i && lev[j] + 1 > l) l = lev[j] + 1;
+                if (!(dir > 0 ? j < i : j > i)) continue;
+                int d = sl[j];
+                if (d >= q) bad = 1; /* cannot happen: dependencies sit on lower levels */
+                if (mark[d] == q) continue;
+                mark[d] = q;
+                if (used == cap) {
+                    cap *= 2;
+                    int *t = (int *)realloc(o->dep_list, sizeof(int) * cap);
+                    if (!t) { bad = 2; break; }
+                    o->dep_list = t;
+                }
+                o->dep_list[used++] = d;
             }
-            lev[i] = l;
-            if (l > maxlev) maxlev = l;
         }
     }
-    int *cnt = (int *)calloc((size_t)maxlev + 2, sizeof(int));
-    if (!cnt) { free(lev); return JXAMD_ERROR_MEMORY; }
-    for (int i = 0; i < n; i++) cnt[lev[i] + 1]++;
-    for (int l = 0; l <= maxlev; l++) cnt[l + 1] += cnt[l];
-    for (int i = 0; i < n; i++) perm[cnt[lev[i]]++] = i;
-    free(cnt);
-    free(lev);
+    o->dep_ptr[o->nsl] = (int)used;
+    free(lev); free(start); free(cnt); free(sl); free(mark);
+    if (bad) { hsched_free(o); return bad == 2 ? JXAMD_ERROR_MEMORY : JXAMD_ERROR_GENERIC; }
     return 0;
 }
 
@@ -152,6 +207,8 @@ static void hsell_free(hsell *h)
     memset(h, 0, sizeof(*h));
 }
 
+/* n is the number of positions; with perm_owned, perm_owned[p] < 0 marks a
+ * padding position (empty row). */
 static int hsell_build(hsell *h, int n, const int *ia, const int *ja, const double *a, int *perm_owned, int offdiag,
                        int pad_self)
 {
@@ -171,6 +228,7 @@ static int hsell_build(hsell *h, int n, const int *ia, const int *ja, const doub
             int p = s * SELLW + l;
             if (p >= n) break;
             int i = perm_owned ? perm_owned[p] : p;
+            if (i < 0) continue;
             int len = ia[i + 1] - ia[i] - skip;
             if (len > w) w = len;
         }
@@ -188,9 +246,13 @@ static int hsell_build(hsell *h, int n, const int *ia, const int *ja, const doub
             int i = -1, len = 0, beg = 0;
             if (p < n) {
                 i = perm_owned ? perm_owned[p] : p;
-                beg = ia[i] + skip;
-                len = ia[i + 1] - beg;
-                if (offdiag) h->diag[p] = a[ia[i]];
+                if (i >= 0) {
+                    beg = ia[i] + skip;
+                    len = ia[i + 1] - beg;
+                    if (offdiag) h->diag[p] = a[ia[i]];
+                } else if (offdiag) {
+                    h->diag[p] = 1.0;
+                }
             }
             for (int k = 0; k < w; k++) {
                 size_t e = (size_t)(k0 + k) * SELLW + l;
@@ -207,7 +269,26 @@ static int hsell_build(hsell *h, int n, const int *ia, const int *ja, const doub
     return 0;
 }
 
-static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, int with_flag)
+static int upload_ints(int **dst, const int *src, size_t n, void *st)
+{
+    CK(jxamd_dev_malloc((void **)dst, sizeof(int) * (n ? n : 1)));
+    return jxamd_h2d(*dst, src, sizeof(int) * n, st);
+}
+
+static int zero_ints(int **dst, size_t n, void *st)
+{
+    int *z = (int *)calloc(n ? n : 1, sizeof(int));
+    if (!z) return JXAMD_ERROR_MEMORY;
+    int e = jxamd_dev_malloc((void **)dst, sizeof(int) * (n ? n : 1));
+    if (!e) e = jxamd_h2d(*dst, z, sizeof(int) * (n ? n : 1), st);
+    if (!e) e = jxamd_stream_sync(st);
+    free(z);
+    return e;
+}
+
+/* sched != NULL: a sweep layout; uploads its dependency lists and a zeroed
+ * per-slice flag array */
+static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsched *sched)
 {
     memset(d, 0, sizeof(*d));
     d->n = h->n;
@@ -228,22 +309,19 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, int with_
         CK(jxamd_dev_malloc((void **)&d->diag, sizeof(double) * (h->n ? h->n : 1)));
         CK(jxamd_h2d(d->diag, h->diag, sizeof(double) * h->n, st));
     }
-    if (with_flag) {
-        int *z = (int *)calloc((size_t)(h->n ? h->n : 1), sizeof(int));
-        if (!z) return JXAMD_ERROR_MEMORY;
-        CK(jxamd_dev_malloc((void **)&d->flag, sizeof(int) * (h->n ? h->n : 1)));
-        int e = jxamd_h2d(d->flag, z, sizeof(int) * (h->n ? h->n : 1), st);
-        if (!e) e = jxamd_stream_sync(st);
-        free(z);
-        CK(e);
+    if (sched) {
+        CK(upload_ints(&d->dep_ptr, sched->dep_ptr, (size_t)sched->nsl + 1, st));
+        CK(upload_ints(&d->dep_list, sched->dep_list, (size_t)sched->dep_ptr[sched->nsl], st));
+        CK(zero_ints(&d->flag, (size_t)sched->nsl, st));
     }
-    return 0;
+    return jxamd_stream_sync(st);
 }
 
 static void dsell_free(dsell *d)
 {
     jxamd_dev_free(d->sptr); jxamd_dev_free(d->col); jxamd_dev_free(d->val);
     jxamd_dev_free(d->perm); jxamd_dev_free(d->diag); jxamd_dev_free(d->flag);
+    jxamd_dev_free(d->dep_ptr); jxamd_dev_free(d->dep_list);
     memset(d, 0, sizeof(*d));
 }
 
@@ -380,15 +458,17 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
     }
     /* forward block-HGS order: off-diagonal blocks in level order + Dinv */
     {
-        int *perm = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
-        if (!perm) return JXAMD_ERROR_MEMORY;
-        CK(level_perm(nb, A->ia, A->ja, 1, perm));
+        hsched sc;
+        CK(build_schedule(nb, A->ia, A->ja, 1, &sc));
+        int *perm = sc.perm;
+        const int npos = sc.npos;
+        sc.perm = NULL; /* owned by h below */
         hsell h;
         double *dummy = (double *)calloc((size_t)(A->nnzb > 0 ? A->nnzb : 1), sizeof(double));
-        if (!dummy) { free(perm); return JXAMD_ERROR_MEMORY; }
-        int e = hsell_build(&h, nb, A->ia, A->ja, dummy, perm, 1, 1);
+        if (!dummy) { free(perm); hsched_free(&sc); return JXAMD_ERROR_MEMORY; }
+        int e = hsell_build(&h, npos, A->ia, A->ja, dummy, perm, 1, 1);
         free(dummy);
-        if (e) { hsell_free(&h); return e; }
+        if (e) { hsell_free(&h); hsched_free(&sc); return e; }
         size_t slots = (size_t)(h.nsl ? h.sptr[h.nsl] : 0);
         double *v = (double *)calloc(slots * bb * SELLW + 1, sizeof(double));
         double *dinv = (double *)malloc(sizeof(double) * (size_t)nb * bb + 8);
@@ -397,8 +477,8 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             int k0 = h.sptr[sl], w = h.sptr[sl + 1] - k0;
             for (int l = 0; l < SELLW; l++) {
                 int p = sl * SELLW + l;
-                if (p >= nb) break;
                 int i = perm[p];
+                if (i < 0) continue;
                 int beg = A->ia[i] + 1, len = A->ia[i + 1] - beg;
                 for (int k = 0; k < w && k < len; k++) {
                     const double *blk = A->a + (size_t)(beg + k) * bb;
@@ -423,21 +503,20 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         int rc = jxamd_dev_malloc((void **)&d->gsptr, sizeof(int) * (h.nsl + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->gcol, sizeof(int) * (slots * SELLW + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->gval, sizeof(double) * (slots * bb * SELLW + 1));
-        if (!rc) rc = jxamd_dev_malloc((void **)&d->gperm, sizeof(int) * (nb + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->dinv, sizeof(double) * ((size_t)nb * bb + 1));
-        if (!rc) rc = jxamd_dev_malloc((void **)&d->flag, sizeof(int) * (nb + 1));
         if (!rc) rc = jxamd_h2d(d->gsptr, h.sptr, sizeof(int) * (h.nsl + 1), s->stream);
         if (!rc) rc = jxamd_h2d(d->gcol, h.col, sizeof(int) * slots * SELLW, s->stream);
         if (!rc) rc = jxamd_h2d(d->gval, v, sizeof(double) * slots * bb * SELLW, s->stream);
-        if (!rc) rc = jxamd_h2d(d->gperm, perm, sizeof(int) * nb, s->stream);
+        if (!rc) rc = upload_ints(&d->gperm, perm, (size_t)npos, s->stream);
         if (!rc) rc = jxamd_h2d(d->dinv, dinv, sizeof(double) * (size_t)nb * bb, s->stream);
-        int *z = (int *)calloc((size_t)nb + 1, sizeof(int));
-        if (!rc && z) rc = jxamd_h2d(d->flag, z, sizeof(int) * (nb + 1), s->stream);
+        if (!rc) rc = upload_ints(&d->gdep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
+        if (!rc) rc = upload_ints(&d->gdep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
+        if (!rc) rc = zero_ints(&d->flag, (size_t)sc.nsl, s->stream);
         if (!rc) rc = jxamd_stream_sync(s->stream);
-        free(z);
         free(v);
         free(dinv);
         hsell_free(&h); /* frees perm */
+        hsched_free(&sc);
         CK(rc);
     }
     return 0;
@@ -451,35 +530,29 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     int e;
     /* natural order, full rows: residual F - A U */
     e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 0, 0);
-    if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, 0);
+    if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL);
     if (!e) e = jxamd_stream_sync(s->stream);
     hsell_free(&h);
     CK(e);
     if (l < s->L - 1 || s->L == 1) {
         /* forward and backward sweep orders, off-diagonal entries */
-        int *pf = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
-        if (!pf) return JXAMD_ERROR_MEMORY;
-        CK(level_perm(n, A->ia, A->ja, 1, pf));
-        e = hsell_build(&h, n, A->ia, A->ja, A->a, pf, 1, 1);
-        if (!e) e = dsell_upload(&s->Gf[l], &h, n, s->stream, 1);
-        if (!e) e = jxamd_stream_sync(s->stream);
-        hsell_free(&h);
-        CK(e);
-        int *pb = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
-        if (!pb) return JXAMD_ERROR_MEMORY;
-        CK(level_perm(n, A->ia, A->ja, -1, pb));
-        e = hsell_build(&h, n, A->ia, A->ja, A->a, pb, 1, 1);
-        if (!e) e = dsell_upload(&s->Gb[l], &h, n, s->stream, 0);
-        if (!e) e = jxamd_stream_sync(s->stream);
-        hsell_free(&h);
-        CK(e);
-        s->Gb[l].flag = NULL; /* both directions share the forward flag array */
+        for (int dir = 1; dir >= -1; dir -= 2) {
+            hsched sc;
+            CK(build_schedule(n, A->ia, A->ja, dir, &sc));
+            int *perm = sc.perm;
+            sc.perm = NULL; /* owned by h below */
+            e = hsell_build(&h, sc.npos, A->ia, A->ja, A->a, perm, 1, 1);
+            if (!e) e = dsell_upload(dir > 0 ? &s->Gf[l] : &s->Gb[l], &h, n, s->stream, &sc);
+            hsell_free(&h);
+            hsched_free(&sc);
+            CK(e);
+        }
     }
     if (l < s->L - 1) {
         /* interpolation P (n x nc) and restriction R = P^T (nc x n) */
         const jxamd_csr *Pm = &H->P[l];
         e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, 0);
+        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL);
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         CK(e);
@@ -487,7 +560,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         double *ra;
         CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
         e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, 0);
+        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL);
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         free(ri); free(rj); free(ra);
@@ -598,6 +671,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(d->sptr); jxamd_dev_free(d->col); jxamd_dev_free(d->val);
     jxamd_dev_free(d->gperm); jxamd_dev_free(d->gsptr); jxamd_dev_free(d->gcol);
     jxamd_dev_free(d->gval); jxamd_dev_free(d->dinv); jxamd_dev_free(d->flag);
+    jxamd_dev_free(d->gdep_ptr); jxamd_dev_free(d->gdep_list);
     for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
         dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]);
         dsell_free(&s->R[l]); dsell_free(&s->P[l]);
@@ -631,13 +705,12 @@ int jxamd_solver_error_flag(jxamd_solver *s, int *flag)
 static int hsgs(jxamd_solver *s, int l, const double *f, int fs, double *u, int uo_zero)
 {
     dsell *g = &s->Gf[l], *h = &s->Gb[l];
-    int *flag = s->Gf[l].flag;
     s->epoch++;
-    CK(jxamd_k_gs_sweep(g->n, g->perm, g->sptr, g->col, g->val, g->diag, f, fs, uo_zero ? NULL : u, s->T[l], flag,
-                        s->epoch, 1, s->ticket, &s->tbase, s->err, s->stream));
+    CK(jxamd_k_gs_sweep(g->nsl, g->perm, g->sptr, g->col, g->val, g->diag, f, fs, uo_zero ? NULL : u, s->T[l],
+                        g->dep_ptr, g->dep_list, g->flag, s->epoch, 1, s->ticket, &s->tbase, s->err, s->stream));
     s->epoch++;
-    CK(jxamd_k_gs_sweep(h->n, h->perm, h->sptr, h->col, h->val, h->diag, f, fs, s->T[l], u, flag, s->epoch, -1,
-                        s->ticket, &s->tbase, s->err, s->stream));
+    CK(jxamd_k_gs_sweep(h->nsl, h->perm, h->sptr, h->col, h->val, h->diag, f, fs, s->T[l], u, h->dep_ptr,
+                        h->dep_list, h->flag, s->epoch, -1, s->ticket, &s->tbase, s->err, s->stream));
     return 0;
 }
 
@@ -679,9 +752,9 @@ static int bhgs_sweep(jxamd_solver *s, const double *rhs, const double *xo, cons
 {
     dbsr *d = &s->A;
     s->epoch++;
-    return jxamd_k_bhgs_sweep(d->b, d->nb, d->gperm, d->gsptr, d->gcol, d->gval, d->dinv, rhs, xo, xp,
-                              s->prm.pressure_index, xn, d->flag, s->epoch, w, omega, s->ticket, &s->tbase, s->err,
-                              s->stream);
+    return jxamd_k_bhgs_sweep(d->b, d->gnsl, d->gperm, d->gsptr, d->gcol, d->gval, d->dinv, rhs, xo, xp,
+                              s->prm.pressure_index, xn, d->gdep_ptr, d->gdep_list, d->flag, s->epoch, w, omega,
+                              s->ticket, &s->tbase, s->err, s->stream);
 }
 
 /* x = M^{-1} r (jxf_cpr.c:424-610) */
