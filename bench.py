@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""CPR-GMRES solve phase benchmark on one MI355X.
+
+Workload (one "step" = one full solve): the 60x220x85 reservoir system
+(1,122,000 cells, 3x3 FP64 blocks, 7,780,000 stored blocks; top 35 layers
+smooth lognormal, bottom 50 layers sinuous channels with k ratio 1e3, 5 well
+columns), ABF-decoupled, CPR-GMRES(30) to a relative residual of 1e-6 from
+x0 = 0, with the pressure-AMG hierarchy built on the host.  b and x live in
+HBM when the timed region starts; host setup (generation, decoupling, AMG
+setup, upload) is reported separately.
+
+Prints one JSON line (rank 0).  Also measured live on the solver's stream:
+per-kernel-class device time and algorithmic bytes (roofline of the
+dominant kernel) and the BSR SpMV bandwidth.  The CPU baseline is the oracle
+(a bit-exact single-thread restatement of the reference) on a bounded
+sample of the same solve.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "CPR-GMRES time-to-rtol 1e-6 (s) & BSR SpMV GB/s; mixed+learned vs plain FP64"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--cpu-iters", type=int, default=2, help="GMRES iterations in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--relax-chunk", type=int, default=None,
+                    help="rows per relax-6 chunk (0: one-thread sweeps); default: the package's large-workload value")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprof counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the solve phase runs on a single MI355X); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import jxpamg_all_amd as jx
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    cfg = jx.CONFIGS[args.config]
+
+    t0 = time.time()
+    A, rhs = jx.generate(**cfg)
+    Ad, bd = jx.decouple_abf(A, rhs)
+    t_gen = time.time() - t0
+    t0 = time.time()
+    ia, ja, a = jx.extract_pressure(Ad)
+    H = jx.amg_setup(ia, ja, a)
+    t_amg = time.time() - t0
+    log(f"{args.config}: nb={Ad.nb} nnzb={Ad.nnzb} levels={H.num_levels} sizes={[len(m[0]) - 1 for m in H.A]} "
+        f"gen+decouple {t_gen:.1f}s amg {t_amg:.1f}s")
+    chunk = jx.BENCH_RELAX_CHUNK if args.relax_chunk is None else args.relax_chunk
+    params = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk)
+    t0 = time.time()
+    s = jx.Solver(Ad, H, params)
+    t_create = time.time() - t0
+    log(f"solver create (layouts + upload) {t_create:.1f}s")
+
+    b_d = torch.tensor(bd, dtype=torch.float64, device="cuda")
+    x_d = torch.zeros_like(b_d)
+    hist = np.zeros(params.max_iter + 1)
+
+    for w in range(args.warmup):
+        x_d.zero_()
+        st = s.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
+        log(f"warmup {w}: iters {st.iterations} true_rel {st.true_rel_res:.3e} device {st.t_solve_ms:.1f} ms")
+    times = []
+    for k in range(args.steps):
+        x_d.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = s.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"step {k}: {times[-1] * 1e3:.1f} ms iters {st.iterations} true_rel {st.true_rel_res:.3e}")
+    sec = float(np.mean(times))
+    iters = st.iterations
+    assert st.converged == 1 and st.true_rel_res <= 1e-6, "solve did not reach rtol 1e-6"
+
+    # per-kernel-class device time of one more solve, with events around every launch
+    s.profile(True)
+    x_d.zero_()
+    s.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
+    prof = s.profile_read()
+    s.profile(False)
+    dom = max(prof, key=lambda k: prof[k][1])
+    n_l, ms_l, by_l = prof[dom]
+    per_launch_ms = ms_l / n_l
+    achieved = (by_l / n_l) / (per_launch_ms * 1e-3) / 1e9
+    kernel_names = {"gs_sweep": "k_gs_sweep", "bhgs_sweep": "k_bhgs_sweep", "bsr_spmv": "k_bsr_spmv",
+                    "csr_spmv": "k_csr_spmv", "coarse_solve": "k_dense_gemv", "mgs": "k_dot_mgs",
+                    "vector_ops": "k_scale/k_axpy/k_lincomb"}
+    traffic = read_traffic(kernel_names[dom])
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "kernel": kernel_names[dom], "launches_per_solve": n_l,
+                "mean_launch_us": round(per_launch_ms * 1e3, 2),
+                "algorithmic_bytes_per_launch": round(by_l / n_l)}
+    breakdown = {k: {"launches": v[0], "ms": round(v[1], 3),
+                     "GBs": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 else None} for k, v in prof.items()}
+    log(f"kernel classes: {json.dumps(breakdown)}")
+
+    # BSR SpMV bandwidth (second half of the metric)
+    y_d = torch.empty_like(b_d)
+    s.bsr_matvec_timed(b_d.data_ptr(), y_d.data_ptr(), 5)
+    ms = s.bsr_matvec_timed(b_d.data_ptr(), y_d.data_ptr(), args.spmv_reps)
+    spmv_bytes = Ad.nnzb * (8 * Ad.b * Ad.b + 4) + 4 * (Ad.nb + 1) + 16 * Ad.n
+    spmv = {"kernel": "k_bsr_spmv<3>", "mean_launch_us": round(ms * 1e3, 2), "algorithmic_bytes": spmv_bytes,
+            "achieved_GBs": round(spmv_bytes / (ms * 1e-3) / 1e9, 1),
+            "frac": round(spmv_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": read_traffic("k_bsr_spmv")}
+    log(f"BSR SpMV {spmv}")
+    assert s.error_flag() == 0
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.cpu_ref import Oracle
+        o = Oracle(Ad, H)
+        p = o.params(tol=1e-6, max_iter=args.cpu_iters)  # one-thread reference semantics
+        t0 = time.time()
+        xo, sto, histo, rc = o.gmres(bd, p)
+        t_cpu = time.time() - t0
+        applies = sto.num_precond
+        est = t_cpu / applies * st.num_precond
+        cpu = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iters} GMRES iterations ({applies} CPR applications, {t_cpu:.1f} s) of the "
+                         f"same solve with the oracle (bit-exact single-thread restatement of the reference), "
+                         f"scaled to the {st.num_precond} CPR applications of the full solve"}
+        log(f"cpu baseline {cpu}")
+
+    out = {"metric": METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{args.config} {cfg['nx']}x{cfg['ny']}x{cfg['nz']} 3x3 BSR FP64, ABF, "
+                                  f"CPR(AMG V-cycle hSGS + 3 block HGS)-GMRES(30) to rtol 1e-6",
+                      "relax6_rows_per_chunk": chunk,
+                      "cells": Ad.nb, "nnzb": Ad.nnzb, "iterations": iters,
+                      "true_rel_res": st.true_rel_res, "amg_levels": [len(m[0]) - 1 for m in H.A],
+                      "host_setup_s": {"generate_decouple": round(t_gen, 2), "amg": round(t_amg, 2),
+                                       "upload": round(t_create, 2)}},
+           "roofline": roofline, "spmv": spmv, "kernel_classes": breakdown, "cpu_baseline": cpu}
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

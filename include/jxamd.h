@@ -104,6 +104,22 @@ typedef struct {
     jxamd_csr P[JXAMD_MAX_LEVELS];
 } jxamd_hierarchy;
 
+/* Host construction of the hierarchy with the settings of JXF_CPRSetup
+ * (jxfpamg-0220/csrc/cpr/jxf_cpr.c:319-377): classical strength, HMIS
+ * coarsening, extended+i interpolation, Galerkin coarse operators; the
+ * operators are bit-identical to the reference's on one rank. */
+typedef struct {
+    int max_levels;          /* 25 */
+    int coarse_threshold;    /* 1000: stop once a level has at most this many points */
+    double strong_threshold; /* 0.25 */
+    double max_row_sum;      /* 0.9 */
+} jxamd_amg_params;
+void jxamd_amg_params_default(jxamd_amg_params *p);
+/* A0: pressure matrix (square, diagonal first).  Allocates every level
+ * (A[0] is a copy); release with jxamd_hierarchy_free. */
+int jxamd_amg_setup(const jxamd_csr *A0, const jxamd_amg_params *p, jxamd_hierarchy *H);
+void jxamd_hierarchy_free(jxamd_hierarchy *H);
+
 /* Parameters.  CPR keys mirror JXF_CPRSetParameter / JXF_CPRSetRealParameter
  * (jxfpamg-0220/csrc/cpr/jxf_cpr.c:226-262) plus the stage-2 weights fixed at
  * jxf_cpr.c:547-549; GMRES keys mirror the JXF_GMRESSet* setters
@@ -114,9 +130,17 @@ typedef struct {
     int pressure_index, stage1_maxit, stage2_maxit;
     int stage1_solver_type, stage2_solver_type, print_level;
     double threshold, stage2_weight, stage2_omega;
+    /* AMG smoother (relax 6) threading, jxfpamg-0220/csrc/operation/
+     * relaxations/par_relax_6.c: relax_threads = T > 1 is the reference's
+     * T-thread hybrid SSOR (T contiguous chunks, Gauss-Seidel inside a chunk,
+     * Jacobi across chunks) on every level, as with OMP_NUM_THREADS = T;
+     * relax_chunk = c > 0 uses ceil(n_l / c) chunks on level l instead.
+     * Both 0/1: the one-thread sequential sweeps. */
+    int relax_threads, relax_chunk;
 } jxamd_params;
 
-/* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6. */
+/* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread
+ * smoothing (relax_threads = relax_chunk = 0). */
 void jxamd_params_default(jxamd_params *p);
 
 typedef struct {
@@ -163,6 +187,22 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
 int jxamd_op_bsr_matvec_timed(jxamd_solver *s, const double *d_x, double *d_y, int reps, double *ms_per_launch);
 /* one CPR application r -> x launched reps times; mean device time per apply */
 int jxamd_cpr_precond_timed(jxamd_solver *s, const double *d_r, double *d_x, int reps, double *ms_per_apply);
+/* Per-kernel-class device timing.  While enabled, every launch of the
+ * solver is bracketed by HIP events on its stream and charged to a class
+ * together with its algorithmic HBM bytes (the bytes the algorithm must move
+ * once: matrix entries, indices, each vector read or written once). */
+enum {
+    JXAMD_PROF_GS_SWEEP = 0,   /* scalar hybrid-GS sweeps, all levels */
+    JXAMD_PROF_BHGS_SWEEP = 1, /* stage-2 block sweeps */
+    JXAMD_PROF_BSR_SPMV = 2,   /* y = A x, residuals */
+    JXAMD_PROF_CSR_SPMV = 3,   /* level residual, restriction, prolongation */
+    JXAMD_PROF_COARSE = 4,     /* coarsest-level dense solve */
+    JXAMD_PROF_MGS = 5,        /* fused Gram-Schmidt dots / norms */
+    JXAMD_PROF_VEC = 6,        /* scale, axpy, linear combinations, copies */
+    JXAMD_PROF_KINDS = 7
+};
+int jxamd_solver_profile(jxamd_solver *s, int enable); /* enable also clears the counters */
+int jxamd_solver_profile_read(jxamd_solver *s, int kind, long *launches, double *ms, double *bytes);
 /* number of sync-free waits that hit the spin bound since creation (0 when healthy) */
 int jxamd_solver_error_flag(jxamd_solver *s, int *flag);
 

@@ -43,6 +43,11 @@ class HierarchyC(ctypes.Structure):
     _fields_ = [("num_levels", ctypes.c_int), ("A", CSR * MAX_LEVELS), ("P", CSR * MAX_LEVELS)]
 
 
+class AMGParams(ctypes.Structure):
+    _fields_ = [("max_levels", ctypes.c_int), ("coarse_threshold", ctypes.c_int),
+                ("strong_threshold", ctypes.c_double), ("max_row_sum", ctypes.c_double)]
+
+
 class Params(ctypes.Structure):
     _fields_ = [("k_dim", ctypes.c_int), ("max_iter", ctypes.c_int), ("is_check_restarted", ctypes.c_int),
                 ("tol", ctypes.c_double), ("a_tol", ctypes.c_double),
@@ -50,7 +55,7 @@ class Params(ctypes.Structure):
                 ("stage1_solver_type", ctypes.c_int), ("stage2_solver_type", ctypes.c_int),
                 ("print_level", ctypes.c_int),
                 ("threshold", ctypes.c_double), ("stage2_weight", ctypes.c_double),
-                ("stage2_omega", ctypes.c_double)]
+                ("stage2_omega", ctypes.c_double), ("relax_threads", ctypes.c_int), ("relax_chunk", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -99,12 +104,18 @@ def _declare(L):
         "jxamd_op_bhgs": [vp, vp, vp, f64, f64],
         "jxamd_solver_error_flag": [vp, vp],
         "jxamd_op_bsr_matvec_timed": [vp, vp, vp, i32, vp],
+        "jxamd_amg_params_default": [vp],
+        "jxamd_solver_profile": [vp, i32],
+        "jxamd_solver_profile_read": [vp, i32, vp, vp, vp],
+        "jxamd_amg_setup": [vp, vp, vp],
+        "jxamd_hierarchy_free": [vp],
         "jxamd_cpr_precond_timed": [vp, vp, vp, i32, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
         f.argtypes = args
-        f.restype = None if name == "jxamd_params_default" else ctypes.c_int
+        f.restype = None if name in ("jxamd_params_default", "jxamd_amg_params_default",
+                                     "jxamd_hierarchy_free") else ctypes.c_int
 
 
 def _p(a):
@@ -134,6 +145,11 @@ CONFIGS = {
     "C5": dict(nx=120, ny=440, nz=85, block_size=3, field=FIELD_LAYERED, sigma_lnk=1.0, corr_len=8.0,
                n_smooth_layers=35, channel_ratio=1e3, kz_ratio=0.1, n_wells=5, well_boost=1e4, seed=2001),
 }
+
+
+# rows per relax-6 chunk on the GPU for the large workloads: the reference's
+# threaded hybrid SSOR (par_relax_6.c) with ceil(n_l / 16) chunks on level l
+BENCH_RELAX_CHUNK = 16
 
 
 def gen_params(**kw):
@@ -262,6 +278,34 @@ class Hierarchy:
         return cls(As, Ps)
 
 
+def amg_setup(ia, ja, a, **kw):
+    """Pressure-AMG hierarchy built on the host (jxamd_amg_setup) -> Hierarchy."""
+    L = lib()
+    p = AMGParams()
+    L.jxamd_amg_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    a = np.ascontiguousarray(a, np.float64)
+    n = len(ia) - 1
+    A0 = CSR(n, n, int(ia[-1]), _p(ia).value, _p(ja).value, _p(a).value)
+    h = HierarchyC()
+    check(L.jxamd_amg_setup(ctypes.byref(A0), ctypes.byref(p), ctypes.byref(h)), "jxamd_amg_setup")
+    try:
+        def grab(c):
+            m, nnz, nc = c.n, c.nnz, c.ncols
+            gi = np.ctypeslib.as_array(ctypes.cast(c.ia, ctypes.POINTER(ctypes.c_int)), (m + 1,)).copy()
+            gj = np.ctypeslib.as_array(ctypes.cast(c.ja, ctypes.POINTER(ctypes.c_int)), (max(nnz, 1),))[:nnz].copy()
+            ga = np.ctypeslib.as_array(ctypes.cast(c.a, ctypes.POINTER(ctypes.c_double)), (max(nnz, 1),))[:nnz].copy()
+            return (gi, gj, ga, nc)
+        As = [grab(h.A[l]) for l in range(h.num_levels)]
+        Ps = [grab(h.P[l]) for l in range(h.num_levels - 1)]
+    finally:
+        L.jxamd_hierarchy_free(ctypes.byref(h))
+    return Hierarchy(As, Ps)
+
+
 def default_params(**kw):
     p = Params()
     lib().jxamd_params_default(ctypes.byref(p))
@@ -343,6 +387,21 @@ class Solver:
         check(self._L.jxamd_cpr_precond_timed(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_x), reps,
                                               ctypes.byref(ms)), "jxamd_cpr_precond_timed")
         return ms.value
+
+    PROF_KINDS = ["gs_sweep", "bhgs_sweep", "bsr_spmv", "csr_spmv", "coarse_solve", "mgs", "vector_ops"]
+
+    def profile(self, enable=True):
+        check(self._L.jxamd_solver_profile(self._s, 1 if enable else 0), "jxamd_solver_profile")
+
+    def profile_read(self):
+        """{kind: (launches, total_ms, total_algorithmic_bytes)}"""
+        out = {}
+        for k, name in enumerate(self.PROF_KINDS):
+            n, ms, by = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
+            check(self._L.jxamd_solver_profile_read(self._s, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)),
+                  "jxamd_solver_profile_read")
+            out[name] = (n.value, ms.value, by.value)
+        return out
 
     def error_flag(self):
         f = ctypes.c_int()

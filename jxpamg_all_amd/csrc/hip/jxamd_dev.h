@@ -38,6 +38,8 @@ int jxamd_k_bhgs_sweep(int b, int nsl, const int *perm, const int *sptr, const i
                        const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
                        double *xn, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, double w,
                        double omega, unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
+int jxamd_k_gs_chunk(int n, int nchunks, const int *wstep, const int *sptr, const int *col, const double *val,
+                     const double *diag, const double *f, int fs, const double *uo, double *un, void *stream);
 int jxamd_k_dense_gemv(int n, const double *M, const double *x, double *y, void *stream);
 int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream);
 int jxamd_k_prolong(int nb, int b, int pidx, const double *xp, double *x, void *stream);

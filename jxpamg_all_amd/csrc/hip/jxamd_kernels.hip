@@ -295,6 +295,76 @@ __global__ void __launch_bounds__(256) k_bhgs_sweep(int nsl, const int *__restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Hybrid SSOR with T threads (relax 6, weights 1; par_relax_6.c:132-223):
+// rows split into T contiguous chunks (first n % T chunks one row longer);
+// one lane per chunk runs a forward then a backward Gauss-Seidel sweep over
+// its chunk, reading neighbours outside the chunk from the values at entry.
+// Entry values come from uo (zeros if uo == nullptr); results go to un.
+// Inside the forward sweep a neighbour j of row i in the same chunk is new
+// (un) if j < i and still at its entry value (uo) if j > i; in the backward
+// sweep every in-chunk neighbour has been written to un.
+// Layout: wave w owns slices [wstep[w], wstep[w+1]); slice wstep[w] + t holds
+// the t-th row of each of the wave's 64 chunks (SELL-64, off-diagonals, the
+// diagonal in diag[p]).  No synchronisation between lanes.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int *__restrict__ wstep,
+                                                  const int *__restrict__ sptr, const int *__restrict__ col,
+                                                  const double *__restrict__ val, const double *__restrict__ diag,
+                                                  const double *__restrict__ f, int fs, const double *__restrict__ uo,
+                                                  double *__restrict__ un)
+{
+    const int w = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int c = w * WAVE + lane;
+    if (w * WAVE >= nchunks) return;
+    const int size = n / nchunks, rest = n - size * nchunks;
+    int ns = 0, ne = 0;
+    if (c < nchunks) {
+        ns = c < rest ? c * size + c : c * size + rest;
+        ne = ns + size + (c < rest ? 1 : 0);
+    }
+    const int len = ne - ns;
+    const int s0 = wstep[w], nsteps = wstep[w + 1] - s0;
+    // forward sweep
+    for (int t = 0; t < nsteps; t++) {
+        if (t >= len) continue;
+        const int i = ns + t;
+        const int s = s0 + t;
+        const double d = diag[(size_t)s * WAVE + lane];
+        double res = f[(size_t)i * fs];
+        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+            const size_t e = (size_t)k * WAVE + lane;
+            const int j = col[e];
+            if (j == i) continue; // padding slot
+            double xj;
+            if (j >= ns && j < i) xj = un[j];
+            else xj = uo ? uo[j] : 0.0;
+            res -= val[e] * xj;
+        }
+        un[i] = d != 0.0 ? res / d : (uo ? uo[i] : 0.0);
+    }
+    // backward sweep
+    for (int t = nsteps - 1; t >= 0; t--) {
+        if (t >= len) continue;
+        const int i = ns + t;
+        const int s = s0 + t;
+        const double d = diag[(size_t)s * WAVE + lane];
+        if (d == 0.0) continue;
+        double res = f[(size_t)i * fs];
+        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+            const size_t e = (size_t)k * WAVE + lane;
+            const int j = col[e];
+            if (j == i) continue;
+            double xj;
+            if (j >= ns && j < ne) xj = un[j];
+            else xj = uo ? uo[j] : 0.0;
+            res -= val[e] * xj;
+        }
+        un[i] = res / d;
+    }
+}
+
 // dense y = M x (row-major n x n), one wavefront per row
 __global__ void __launch_bounds__(256) k_dense_gemv(int n, const double *__restrict__ M, const double *__restrict__ x,
                                                     double *__restrict__ y)
@@ -553,6 +623,16 @@ int jxamd_k_bhgs_sweep(int b, int nsl, const int *perm, const int *sptr, const i
     default: return JXAMD_ERROR_ARG;
     }
     *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_gs_chunk(int n, int nchunks, const int *wstep, const int *sptr, const int *col, const double *val,
+                     const double *diag, const double *f, int fs, const double *uo, double *un, void *stream)
+{
+    if (n == 0 || nchunks < 1) return 0;
+    const int waves = (nchunks + WAVE - 1) / WAVE;
+    k_gs_chunk<<<(waves + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, nchunks, wstep, sptr, col, val, diag, f, fs, uo,
+                                                                  un);
     return chk(hipGetLastError());
 }
 

@@ -24,6 +24,12 @@
 
 #define SELLW 64
 
+#define CK(x)                                                                                                          \
+    do {                                                                                                               \
+        int e__ = (x);                                                                                                 \
+        if (e__) return e__;                                                                                           \
+    } while (0)
+
 /* ---------------------------------------------------------------------- */
 /* device matrix containers                                                */
 /* ---------------------------------------------------------------------- */
@@ -32,6 +38,7 @@ typedef struct {
     int *sptr, *col, *perm, *flag;
     int *dep_ptr, *dep_list; /* sweep layouts: dependency slices per slice */
     double *val, *diag;
+    double abytes;           /* algorithmic HBM bytes of one launch over it */
 } dsell;
 
 typedef struct {
@@ -42,7 +49,19 @@ typedef struct {
     int gnsl;
     int *gperm, *gsptr, *gcol, *flag, *gdep_ptr, *gdep_list;
     double *gval, *dinv;
+    double spmv_bytes, sweep_bytes; /* algorithmic bytes: y = A x, one block-HGS sweep */
 } dbsr;
+
+/* per-kernel-class device timing (HIP events around each launch) */
+#define PROF_POOL 4096
+typedef struct {
+    int on, n;
+    void *ev0[PROF_POOL], *ev1[PROF_POOL];
+    int kind[PROF_POOL];
+    double bytes[PROF_POOL];
+    long count[JXAMD_PROF_KINDS];
+    double ms[JXAMD_PROF_KINDS], tbytes[JXAMD_PROF_KINDS];
+} prof_t;
 
 struct jxamd_solver_s {
     jxamd_params prm;
@@ -54,6 +73,9 @@ struct jxamd_solver_s {
     int L;
     int nlev[JXAMD_MAX_LEVELS];
     dsell Ares[JXAMD_MAX_LEVELS], Gf[JXAMD_MAX_LEVELS], Gb[JXAMD_MAX_LEVELS];
+    dsell Gc[JXAMD_MAX_LEVELS]; /* threaded relax-6 layout (chunked mode) */
+    int *wstep[JXAMD_MAX_LEVELS];
+    int nchunks[JXAMD_MAX_LEVELS]; /* 0: sequential sweeps on this level */
     dsell R[JXAMD_MAX_LEVELS], P[JXAMD_MAX_LEVELS];
     double *cinv;
     int nc;
@@ -69,13 +91,45 @@ struct jxamd_solver_s {
     int *err;
     int epoch;
     double t_prec_ms;
+    prof_t prof;
 };
 
-#define CK(x)                                                                                                          \
-    do {                                                                                                               \
-        int e__ = (x);                                                                                                 \
-        if (e__) return e__;                                                                                           \
-    } while (0)
+static int prof_flush(jxamd_solver *s)
+{
+    prof_t *p = &s->prof;
+    if (!p->n) return 0;
+    CK(jxamd_stream_sync(s->stream));
+    for (int q = 0; q < p->n; q++) {
+        float ms = 0.f;
+        CK(jxamd_event_elapsed(&ms, p->ev0[q], p->ev1[q]));
+        p->count[p->kind[q]]++;
+        p->ms[p->kind[q]] += ms;
+        p->tbytes[p->kind[q]] += p->bytes[q];
+    }
+    p->n = 0;
+    return 0;
+}
+
+/* returns the slot to close with prof_end, or -1 when profiling is off */
+static int prof_begin(jxamd_solver *s)
+{
+    prof_t *p = &s->prof;
+    if (!p->on) return -1;
+    if (p->n == PROF_POOL && prof_flush(s)) return -1;
+    if (jxamd_event_record(p->ev0[p->n], s->stream)) return -1;
+    return p->n;
+}
+
+static void prof_end(jxamd_solver *s, int slot, int kind, double bytes)
+{
+    prof_t *p = &s->prof;
+    if (slot < 0) return;
+    if (jxamd_event_record(p->ev1[slot], s->stream)) return;
+    p->kind[slot] = kind;
+    p->bytes[slot] = bytes;
+    p->n++;
+}
+
 
 void jxamd_params_default(jxamd_params *p)
 {
@@ -445,6 +499,8 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             }
         }
         d->nsl = h.nsl;
+        d->spmv_bytes = (double)A->nnzb * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 16.0 * nb * b;
+        d->sweep_bytes = d->spmv_bytes + 16.0 * nb * b + 8.0 * nb * bb; /* + old x, rhs, Dinv */
         int rc = jxamd_dev_malloc((void **)&d->sptr, sizeof(int) * (h.nsl + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->col, sizeof(int) * (slots * SELLW + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->val, sizeof(double) * (slots * bb * SELLW + 1));
@@ -534,7 +590,46 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     if (!e) e = jxamd_stream_sync(s->stream);
     hsell_free(&h);
     CK(e);
-    if (l < s->L - 1 || s->L == 1) {
+    /* CSR SpMV bytes: nnz (value + index) + row pointer + x + rhs + y */
+    s->Ares[l].abytes = 12.0 * A->ia[n] + 4.0 * (n + 1) + 24.0 * n;
+    int T = 0;
+    if (s->prm.relax_threads > 1) T = s->prm.relax_threads;
+    else if (s->prm.relax_chunk > 0) T = (n + s->prm.relax_chunk - 1) / s->prm.relax_chunk;
+    if (T > n) T = n;
+    if (T > 1 && (l < s->L - 1 || s->L == 1)) {
+        /* threaded relax 6: wave w holds chunks 64w..64w+63; slice wstep[w]+t
+         * holds row ns(c)+t of each chunk c of the wave */
+        const int size = n / T, rest = n - size * T;
+        const int waves = (T + SELLW - 1) / SELLW;
+        int *wstep = (int *)malloc(sizeof(int) * ((size_t)waves + 1));
+        if (!wstep) return JXAMD_ERROR_MEMORY;
+        wstep[0] = 0;
+        for (int w = 0; w < waves; w++) wstep[w + 1] = wstep[w] + size + (w * SELLW < rest ? 1 : 0);
+        const int npos = wstep[waves] * SELLW;
+        int *perm = (int *)malloc(sizeof(int) * (npos > 0 ? npos : 1));
+        if (!perm) { free(wstep); return JXAMD_ERROR_MEMORY; }
+        for (int w = 0; w < waves; w++)
+            for (int t = 0; t < wstep[w + 1] - wstep[w]; t++)
+                for (int ln = 0; ln < SELLW; ln++) {
+                    int c = w * SELLW + ln, row = -1;
+                    if (c < T) {
+                        int ns = c < rest ? c * size + c : c * size + rest;
+                        int len = size + (c < rest ? 1 : 0);
+                        if (t < len) row = ns + t;
+                    }
+                    perm[(size_t)(wstep[w] + t) * SELLW + ln] = row;
+                }
+        e = hsell_build(&h, npos, A->ia, A->ja, A->a, perm, 1, 1);
+        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL);
+        if (!e) e = upload_ints(&s->wstep[l], wstep, (size_t)waves + 1, s->stream);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        free(wstep);
+        CK(e);
+        s->nchunks[l] = T;
+        /* forward + backward pass over the matrix, old + new + rhs vectors */
+        s->Gc[l].abytes = 2.0 * (12.0 * A->ia[n] + 4.0 * (n + 1)) + 24.0 * n;
+    } else if (l < s->L - 1 || s->L == 1) {
         /* forward and backward sweep orders, off-diagonal entries */
         for (int dir = 1; dir >= -1; dir -= 2) {
             hsched sc;
@@ -546,6 +641,8 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
             hsell_free(&h);
             hsched_free(&sc);
             CK(e);
+            /* sweep: matrix once + old x + new x + rhs */
+            (dir > 0 ? &s->Gf[l] : &s->Gb[l])->abytes = 12.0 * A->ia[n] + 4.0 * (n + 1) + 24.0 * n;
         }
     }
     if (l < s->L - 1) {
@@ -553,6 +650,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         const jxamd_csr *Pm = &H->P[l];
         e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
         if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL);
+        s->P[l].abytes = 12.0 * Pm->ia[Pm->n] + 4.0 * (Pm->n + 1) + 8.0 * Pm->ncols + 16.0 * Pm->n;
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         CK(e);
@@ -561,6 +659,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
         e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
         if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL);
+        s->R[l].abytes = 12.0 * Pm->ia[Pm->n] + 4.0 * (Pm->ncols + 1) + 8.0 * Pm->n + 8.0 * Pm->ncols;
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         free(ri); free(rj); free(ra);
@@ -590,6 +689,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (H->num_levels < 1 || H->num_levels > JXAMD_MAX_LEVELS) return JXAMD_ERROR_ARG;
     if (p->k_dim < 1 || p->pressure_index < 0 || p->pressure_index >= A->b) return JXAMD_ERROR_ARG;
     if (p->stage1_maxit < 0 || p->stage2_maxit < 0) return JXAMD_ERROR_ARG;
+    if (p->relax_threads < 0 || p->relax_chunk < 0) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -673,7 +773,8 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(d->gval); jxamd_dev_free(d->dinv); jxamd_dev_free(d->flag);
     jxamd_dev_free(d->gdep_ptr); jxamd_dev_free(d->gdep_list);
     for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
-        dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]);
+        dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]); dsell_free(&s->Gc[l]);
+        jxamd_dev_free(s->wstep[l]);
         dsell_free(&s->R[l]); dsell_free(&s->P[l]);
         jxamd_dev_free(s->F[l]); jxamd_dev_free(s->U[l]); jxamd_dev_free(s->T[l]); jxamd_dev_free(s->Vt[l]);
     }
@@ -682,6 +783,10 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(s->Bd); jxamd_dev_free(s->Xd);
     jxamd_dev_free(s->dsc); jxamd_dev_free(s->part); jxamd_dev_free(s->count);
     jxamd_dev_free(s->ticket); jxamd_dev_free(s->err);
+    for (int q = 0; q < PROF_POOL; q++) {
+        jxamd_event_destroy(s->prof.ev0[q]);
+        jxamd_event_destroy(s->prof.ev1[q]);
+    }
     jxamd_event_destroy(s->ev0); jxamd_event_destroy(s->ev1);
     jxamd_event_destroy(s->ev2); jxamd_event_destroy(s->ev3);
     jxamd_stream_destroy(s->stream);
@@ -706,12 +811,49 @@ static int hsgs(jxamd_solver *s, int l, const double *f, int fs, double *u, int 
 {
     dsell *g = &s->Gf[l], *h = &s->Gb[l];
     s->epoch++;
+    int pq = prof_begin(s);
     CK(jxamd_k_gs_sweep(g->nsl, g->perm, g->sptr, g->col, g->val, g->diag, f, fs, uo_zero ? NULL : u, s->T[l],
                         g->dep_ptr, g->dep_list, g->flag, s->epoch, 1, s->ticket, &s->tbase, s->err, s->stream));
+    prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
     s->epoch++;
+    pq = prof_begin(s);
     CK(jxamd_k_gs_sweep(h->nsl, h->perm, h->sptr, h->col, h->val, h->diag, f, fs, s->T[l], u, h->dep_ptr,
                         h->dep_list, h->flag, s->epoch, -1, s->ticket, &s->tbase, s->err, s->stream));
+    prof_end(s, pq, JXAMD_PROF_GS_SWEEP, h->abytes);
     return 0;
+}
+
+/* one threaded relax-6 call on level l: entry values uo (NULL: zeros) -> un */
+static int hsgs_chunk(jxamd_solver *s, int l, const double *f, int fs, const double *uo, double *un)
+{
+    dsell *g = &s->Gc[l];
+    int pq = prof_begin(s);
+    CK(jxamd_k_gs_chunk(s->nlev[l], s->nchunks[l], s->wstep[l], g->sptr, g->col, g->val, g->diag, f, fs, uo, un,
+                        s->stream));
+    prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
+    return 0;
+}
+
+/* the two relax-6 calls of one level visit (relax_order 1, par_relax_if.c:36-60);
+ * the result stays in u */
+static int relax_pair(jxamd_solver *s, int l, const double *f, int fs, double *u, int uo_zero)
+{
+    if (s->nchunks[l] > 1) {
+        CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
+        return hsgs_chunk(s, l, f, fs, s->T[l], u);
+    }
+    CK(hsgs(s, l, f, fs, u, uo_zero));
+    return hsgs(s, l, f, fs, u, 0);
+}
+
+/* a single relax-6 call; the result stays in u */
+static int relax_once(jxamd_solver *s, int l, const double *f, int fs, double *u, int uo_zero)
+{
+    if (s->nchunks[l] > 1) {
+        CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
+        return jxamd_k_copy(s->nlev[l], s->T[l], u, s->stream);
+    }
+    return hsgs(s, l, f, fs, u, uo_zero);
 }
 
 /* V-cycle on F[0] (possibly a strided view) into U[0].  u0_zero: U[0] is zero
@@ -721,28 +863,34 @@ static int vcycle(jxamd_solver *s, const double *f0, int fs0, int u0_zero)
     const int L = s->L;
     if (L == 1) {
         /* no coarsening: one relax-6 call (par_cycle.c:236-255) */
-        return hsgs(s, 0, f0, fs0, s->U[0], u0_zero);
+        return relax_once(s, 0, f0, fs0, s->U[0], u0_zero);
     }
     for (int l = 0; l < L - 1; l++) {
         const double *f = l == 0 ? f0 : s->F[l];
         const int fs = l == 0 ? fs0 : 1;
         const int zero = l == 0 ? u0_zero : 1;
-        CK(hsgs(s, l, f, fs, s->U[l], zero));
-        CK(hsgs(s, l, f, fs, s->U[l], 0));
+        CK(relax_pair(s, l, f, fs, s->U[l], zero));
         /* Vtemp = F - A U; F_{l+1} = P^T Vtemp */
         dsell *A = &s->Ares[l];
+        int pq = prof_begin(s);
         CK(jxamd_k_csr_spmv(A->n, A->sptr, A->col, A->val, s->U[l], 1, f, fs, s->Vt[l], 1, s->stream));
+        prof_end(s, pq, JXAMD_PROF_CSR_SPMV, A->abytes);
         dsell *R = &s->R[l];
+        pq = prof_begin(s);
         CK(jxamd_k_csr_spmv(R->n, R->sptr, R->col, R->val, s->Vt[l], 1, NULL, 1, s->F[l + 1], 0, s->stream));
+        prof_end(s, pq, JXAMD_PROF_CSR_SPMV, R->abytes);
     }
+    int pc = prof_begin(s);
     CK(jxamd_k_dense_gemv(s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
+    prof_end(s, pc, JXAMD_PROF_COARSE, 8.0 * s->nc * ((double)s->nc + 2.0));
     for (int l = L - 2; l >= 0; l--) {
         dsell *P = &s->P[l];
+        int pq = prof_begin(s);
         CK(jxamd_k_csr_spmv(P->n, P->sptr, P->col, P->val, s->U[l + 1], 1, NULL, 1, s->U[l], 2, s->stream));
+        prof_end(s, pq, JXAMD_PROF_CSR_SPMV, P->abytes);
         const double *f = l == 0 ? f0 : s->F[l];
         const int fs = l == 0 ? fs0 : 1;
-        CK(hsgs(s, l, f, fs, s->U[l], 0));
-        CK(hsgs(s, l, f, fs, s->U[l], 0));
+        CK(relax_pair(s, l, f, fs, s->U[l], 0));
     }
     return 0;
 }
@@ -752,9 +900,12 @@ static int bhgs_sweep(jxamd_solver *s, const double *rhs, const double *xo, cons
 {
     dbsr *d = &s->A;
     s->epoch++;
-    return jxamd_k_bhgs_sweep(d->b, d->gnsl, d->gperm, d->gsptr, d->gcol, d->gval, d->dinv, rhs, xo, xp,
-                              s->prm.pressure_index, xn, d->gdep_ptr, d->gdep_list, d->flag, s->epoch, w, omega,
-                              s->ticket, &s->tbase, s->err, s->stream);
+    int pq = prof_begin(s);
+    int e = jxamd_k_bhgs_sweep(d->b, d->gnsl, d->gperm, d->gsptr, d->gcol, d->gval, d->dinv, rhs, xo, xp,
+                               s->prm.pressure_index, xn, d->gdep_ptr, d->gdep_list, d->flag, s->epoch, w, omega,
+                               s->ticket, &s->tbase, s->err, s->stream);
+    prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, d->sweep_bytes);
+    return e;
 }
 
 /* x = M^{-1} r (jxf_cpr.c:424-610) */
@@ -828,7 +979,7 @@ int jxamd_cpr_precond_timed(jxamd_solver *s, const double *d_r, double *d_x, int
 int jxamd_op_hsgs(jxamd_solver *s, int level, const double *d_f, double *d_u)
 {
     if (!s || level < 0 || (s->L > 1 && level >= s->L - 1) || level >= s->L) return JXAMD_ERROR_ARG;
-    CK(hsgs(s, level, d_f, 1, d_u, 0));
+    CK(relax_once(s, level, d_f, 1, d_u, 0));
     return jxamd_stream_sync(s->stream);
 }
 
@@ -861,7 +1012,49 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
 /* ---------------------------------------------------------------------- */
 static int dev_dot(jxamd_solver *s, const double *u, double *y, const double *vprev, const double *hprev, double *out)
 {
-    return jxamd_k_dot_mgs(s->N, u, y, vprev, hprev, s->part, s->count, out, s->stream);
+    int pq = prof_begin(s);
+    int e = jxamd_k_dot_mgs(s->N, u, y, vprev, hprev, s->part, s->count, out, s->stream);
+    /* reads u and y (and v_prev, writing y) */
+    prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * s->N * (vprev ? (u == y ? 3.0 : 4.0) : (u == y ? 1.0 : 2.0)));
+    return e;
+}
+
+/* y = A x (mode 0) or y = rhs - A x (mode 1) with the stage-0 layout */
+static int bsr_apply(jxamd_solver *s, const double *x, const double *rhs, double *y, int mode)
+{
+    dbsr *A = &s->A;
+    int pq = prof_begin(s);
+    int e = jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, rhs, y, mode, 0.0, 0.0, s->stream);
+    prof_end(s, pq, JXAMD_PROF_BSR_SPMV, A->spmv_bytes + (mode == 1 ? 8.0 * s->N : 0.0));
+    return e;
+}
+
+int jxamd_solver_profile(jxamd_solver *s, int enable)
+{
+    if (!s) return JXAMD_ERROR_ARG;
+    prof_t *p = &s->prof;
+    CK(prof_flush(s));
+    if (enable && !p->ev0[0]) {
+        for (int q = 0; q < PROF_POOL; q++) {
+            CK(jxamd_event_create(&p->ev0[q]));
+            CK(jxamd_event_create(&p->ev1[q]));
+        }
+    }
+    memset(p->count, 0, sizeof(p->count));
+    memset(p->ms, 0, sizeof(p->ms));
+    memset(p->tbytes, 0, sizeof(p->tbytes));
+    p->on = enable ? 1 : 0;
+    return 0;
+}
+
+int jxamd_solver_profile_read(jxamd_solver *s, int kind, long *launches, double *ms, double *bytes)
+{
+    if (!s || kind < 0 || kind >= JXAMD_PROF_KINDS) return JXAMD_ERROR_ARG;
+    CK(prof_flush(s));
+    if (launches) *launches = s->prof.count[kind];
+    if (ms) *ms = s->prof.ms[kind];
+    if (bytes) *bytes = s->prof.tbytes[kind];
+    return 0;
 }
 
 static int host_norm(jxamd_solver *s, const double *v, double *nrm)
@@ -914,7 +1107,6 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     double t, gamma, r_norm = 0.0, b_norm = 0.0, den_norm, epsilon, r_norm_0 = 0.0;
     double normup, normdown, conv_factor = 0.0;
     s->t_prec_ms = 0.0;
-    dbsr *A = &s->A;
 
 #define TRY(x)                                                                                                         \
     do {                                                                                                               \
@@ -924,7 +1116,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
 
     TRY(jxamd_event_record(s->ev0, s->stream));
     /* P0 = b - A x */
-    TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, VEC(0), 1, 0.0, 0.0, s->stream));
+    TRY(bsr_apply(s, x, bvec, VEC(0), 1));
     TRY(host_norm(s, bvec, &b_norm));
     if (b_norm != 0.0 && (b_norm / b_norm) != (b_norm / b_norm)) { err = JXAMD_ERROR_GENERIC; goto done; }
     TRY(host_norm(s, VEC(0), &r_norm));
@@ -940,7 +1132,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
         if (r_norm == 0.0) goto done;
         if (r_norm <= epsilon) {
             /* false convergence 1 check (pgmres.c:1200-1241) */
-            TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, s->Rv, 1, 0.0, 0.0, s->stream));
+            TRY(bsr_apply(s, x, bvec, s->Rv, 1));
             TRY(host_norm(s, s->Rv, &r_norm));
             if (r_norm <= epsilon) { converged = 1; break; }
             if (s->prm.is_check_restarted == 0) break;
@@ -952,12 +1144,16 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             iter++;
             TRY(precond_timed(s, VEC(i - 1), s->Rv));
             nprec++;
-            TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, s->Rv, NULL, VEC(i), 0, 0.0, 0.0, s->stream));
+            TRY(bsr_apply(s, s->Rv, NULL, VEC(i), 0));
             /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
             TRY(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
             for (j = 1; j < i; j++) TRY(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
             TRY(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i));
-            TRY(jxamd_k_scale(N, h + i, 0.0, VEC(i), s->stream));
+            {
+                int pq = prof_begin(s);
+                TRY(jxamd_k_scale(N, h + i, 0.0, VEC(i), s->stream));
+                prof_end(s, pq, JXAMD_PROF_VEC, 16.0 * N);
+            }
             TRY(jxamd_d2h(hcol, h, sizeof(double) * (i + 1), s->stream));
             TRY(accumulate_precond_time(s)); /* also synchronises the stream */
             for (j = 0; j < i; j++) hh[j][i - 1] = hcol[j];
@@ -995,13 +1191,17 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             rs[k] = t / hh[k][k];
         }
         TRY(jxamd_h2d(ydev, rs, sizeof(double) * i, s->stream));
-        TRY(jxamd_k_lincomb(N, i, V, N, ydev, s->Wv, s->stream));
+        {
+            int pq = prof_begin(s);
+            TRY(jxamd_k_lincomb(N, i, V, N, ydev, s->Wv, s->stream));
+            prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 1));
+        }
         TRY(precond_timed(s, s->Wv, s->Rv));
         nprec++;
         TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Rv, x, s->stream));
         TRY(accumulate_precond_time(s));
         if (r_norm <= epsilon) {
-            TRY(jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, s->Rv, 1, 0.0, 0.0, s->stream));
+            TRY(bsr_apply(s, x, bvec, s->Rv, 1));
             TRY(host_norm(s, s->Rv, &r_norm));
             if (r_norm <= epsilon) { converged = 1; break; }
             if (s->prm.is_check_restarted == 1) {
@@ -1044,7 +1244,7 @@ done:
         st->t_precond_ms = s->t_prec_ms;
         if (!err) {
             double tr = 0.0;
-            int e2 = jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, bvec, s->Rv, 1, 0.0, 0.0, s->stream);
+            int e2 = bsr_apply(s, x, bvec, s->Rv, 1);
             if (!e2) e2 = host_norm(s, s->Rv, &tr);
             st->true_rel_res = b_norm > 0.0 ? tr / b_norm : tr;
         }

@@ -123,6 +123,56 @@ void cref_relax6(const cref_csr *A, const double *f, double *u)
     }
 }
 
+/* Hybrid SSOR with num_threads > 1, J/csrc/operation/relaxations/
+ * par_relax_6.c:132-223: rows split into num_threads contiguous chunks
+ * (the first n % T chunks one row longer); inside a chunk a forward then a
+ * backward GS sweep, neighbours outside the chunk read from the copy of u
+ * taken at entry.  Chunks are independent, so running them one after the
+ * other reproduces the threaded result. */
+void cref_relax6_threads(const cref_csr *A, const double *f, double *u, int num_threads)
+{
+    const int n = A->n;
+    if (num_threads <= 1) {
+        cref_relax6(A, f, u);
+        return;
+    }
+    double *tmp = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) tmp[i] = u[i];
+    for (int t = 0; t < num_threads; t++) {
+        int size = n / num_threads, rest = n - size * num_threads, ns, ne;
+        if (t < rest) {
+            ns = t * size + t;
+            ne = (t + 1) * size + t + 1;
+        } else {
+            ns = t * size + rest;
+            ne = (t + 1) * size + rest;
+        }
+        for (int i = ns; i < ne; i++) {
+            double d = A->a[A->ia[i]];
+            if (d != 0.0) {
+                double res = f[i];
+                for (int jj = A->ia[i] + 1; jj < A->ia[i + 1]; jj++) {
+                    int ii = A->ja[jj];
+                    res -= A->a[jj] * ((ii >= ns && ii < ne) ? u[ii] : tmp[ii]);
+                }
+                u[i] = res / d;
+            }
+        }
+        for (int i = ne - 1; i > ns - 1; i--) {
+            double d = A->a[A->ia[i]];
+            if (d != 0.0) {
+                double res = f[i];
+                for (int jj = A->ia[i] + 1; jj < A->ia[i + 1]; jj++) {
+                    int ii = A->ja[jj];
+                    res -= A->a[jj] * ((ii >= ns && ii < ne) ? u[ii] : tmp[ii]);
+                }
+                u[i] = res / d;
+            }
+        }
+    }
+    free(tmp);
+}
+
 /* Gaussian elimination without pivoting, J/csrc/operation/matops/par_gs.c:20-74 */
 int cref_gselim(double *A, double *x, int size)
 {
@@ -334,15 +384,15 @@ static int cref_pamg_cycle(const cref_hier *H, double **F, double **U, double *v
     for (;;) {
         if (L > 1) {
             if (cycle_param < 3) {
-                cref_relax6(&H->A[level], F[level], U[level]);
-                cref_relax6(&H->A[level], F[level], U[level]);
+                cref_relax6_threads(&H->A[level], F[level], U[level], H->relax_threads[level]);
+                cref_relax6_threads(&H->A[level], F[level], U[level], H->relax_threads[level]);
             } else {
                 err = cref_relax9(&H->A[level], F[level], U[level]);
                 if (err) return err;
             }
         } else {
             /* no coarsening: one user-type (6) sweep, par_cycle.c:236-255 */
-            cref_relax6(&H->A[level], F[level], U[level]);
+            cref_relax6_threads(&H->A[level], F[level], U[level], H->relax_threads[level]);
         }
         --lev_counter[level];
         if (lev_counter[level] >= 0 && level != L - 1) {

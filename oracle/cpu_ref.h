@@ -23,6 +23,7 @@ typedef struct {
     int num_levels;
     cref_csr A[CREF_MAX_LEVELS]; /* A[0] = pressure matrix */
     cref_csr P[CREF_MAX_LEVELS]; /* P[l]: n_l x n_{l+1}, l < num_levels-1 */
+    int relax_threads[CREF_MAX_LEVELS]; /* relax-6 thread count per level; <= 1: sequential */
 } cref_hier;
 
 typedef struct {
@@ -48,6 +49,7 @@ void cref_csr_matvec(const cref_csr *A, double alpha, const double *x, double be
 void cref_csr_matvecT(const cref_csr *A, double alpha, const double *x, double beta, double *y);
 double cref_dot(int n, const double *x, const double *y);
 void cref_relax6(const cref_csr *A, const double *f, double *u);
+void cref_relax6_threads(const cref_csr *A, const double *f, double *u, int num_threads);
 int cref_relax9(const cref_csr *A, const double *f, double *u);
 int cref_gselim(double *A, double *x, int n);
 void cref_bsr_block_matvec_stable(double alpha, const double *mat, const double *v, double beta, double *ov, int b);

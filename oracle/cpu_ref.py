@@ -19,7 +19,8 @@ class CSR(ctypes.Structure):
 
 
 class Hier(ctypes.Structure):
-    _fields_ = [("num_levels", ctypes.c_int), ("A", CSR * MAX_LEVELS), ("P", CSR * MAX_LEVELS)]
+    _fields_ = [("num_levels", ctypes.c_int), ("A", CSR * MAX_LEVELS), ("P", CSR * MAX_LEVELS),
+                ("relax_threads", ctypes.c_int * MAX_LEVELS)]
 
 
 class BSR(ctypes.Structure):
@@ -55,6 +56,7 @@ def lib():
             ("cref_csr_matvecT", [vp, f64, vp, f64, vp], None),
             ("cref_dot", [i32, vp, vp], f64),
             ("cref_relax6", [vp, vp, vp], None),
+            ("cref_relax6_threads", [vp, vp, vp, i32], None),
             ("cref_relax9", [vp, vp, vp], i32),
             ("cref_gselim", [vp, vp, i32], i32),
             ("cref_bsr_block_matvec_stable", [f64, vp, vp, f64, vp, i32], None),
@@ -106,6 +108,15 @@ class Oracle:
     def relax6(self, level, f, u):
         u = np.array(u, dtype=np.float64, copy=True)
         self.L.cref_relax6(ctypes.byref(self.hier.A[level]), _p(np.ascontiguousarray(f)), _p(u))
+        return u
+
+    def set_relax_threads(self, per_level):
+        for l, t in enumerate(per_level):
+            self.hier.relax_threads[l] = int(t)
+
+    def relax6_threads(self, level, f, u, threads):
+        u = np.array(u, dtype=np.float64, copy=True)
+        self.L.cref_relax6_threads(ctypes.byref(self.hier.A[level]), _p(np.ascontiguousarray(f)), _p(u), threads)
         return u
 
     def relax9(self, f):

@@ -35,7 +35,13 @@ CASES = {
     "nodec": (dict(jx.CONFIGS["C1"]), 1e-6, 100, 1, 0, 1, 30),
     "restart": (dict(jx.CONFIGS["C1"], seed=11), 1e-10, 60, 3, 1, 1, 3),
     "layered": (dict(jx.CONFIGS["C2"], nx=12, ny=20, nz=12, n_smooth_layers=5, seed=21), 1e-6, 100, 3, 1, 1, 30),
+    # the full 60x220x85 system: scalars and residual history only
+    "c2": (dict(jx.CONFIGS["C2"]), 1e-6, 100, 3, 1, 0, 30),
+    # lvl3 with OMP_NUM_THREADS=4: the threaded relax-6 (par_relax_6.c) golden vectors
+    "lvl3_t4": (dict(jx.CONFIGS["C1"], nx=20, ny=20, nz=10, seed=3), 1e-6, 100, 3, 1, 1, 30),
 }
+# OpenMP threads of the reference run (1 unless listed)
+THREADS = {"lvl3_t4": 4}
 # operator vectors not kept for these cases (the hierarchy and the solve are)
 SOLVE_ONLY = {"restart", "layered"}
 OPERATOR_KEYS = ("hsgs_", "coarse_", "vcycle_", "bhgs_", "cpr_", "CF")
@@ -64,20 +70,24 @@ def sha(a):
 
 
 def main():
+    only = set(sys.argv[1:])
     if not os.path.exists(DRIVER):
         sys.exit("build the reference first: make -C oracle ref")
-    env = dict(os.environ, OMP_NUM_THREADS="1")
     for name, (gp, rtol, max_iter, s2, dec, lvl, kdim) in CASES.items():
+        if only and name not in only:
+            continue
         A, rhs = jx.generate(**gp)
         tmp = tempfile.mkdtemp()
         jx.write_bsr_binary(os.path.join(tmp, "A.bin"), A)
         jx.write_rhs_text(os.path.join(tmp, "b.rhs"), rhs)
         dump = os.path.join(tmp, "o.dump")
+        env = dict(os.environ, OMP_NUM_THREADS=str(THREADS.get(name, 1)))
         r = subprocess.run([DRIVER, os.path.join(tmp, "A.bin"), os.path.join(tmp, "b.rhs"), dump, repr(rtol),
                             str(max_iter), str(s2), str(dec), str(lvl), str(kdim)], capture_output=True, text=True,
                            env=env, check=True)
         d = read_dump(dump)
-        meta = dict(rtol=rtol, max_iter=max_iter, stage2_maxit=s2, decouple=dec, k_dim=kdim)
+        meta = dict(rtol=rtol, max_iter=max_iter, stage2_maxit=s2, decouple=dec, k_dim=kdim,
+                    threads=THREADS.get(name, 1))
         rec = {k: v for k, v in d.items()}
         if name in SOLVE_ONLY:
             rec = {k: v for k, v in rec.items() if not k.startswith(OPERATOR_KEYS)}

@@ -91,6 +91,71 @@ def test_full_solve_vs_reference(name):
     s.close()
 
 
+def _threads_per_level(H, relax_threads=0, relax_chunk=0):
+    out = []
+    for ia, _, _, _ in H.A:
+        n = len(ia) - 1
+        T = relax_threads if relax_threads > 1 else (-(-n // relax_chunk) if relax_chunk > 0 else 1)
+        out.append(min(T, n))
+    return out
+
+
+def test_threaded_smoother_vs_reference():
+    """relax_threads = 4 against the OMP_NUM_THREADS=4 reference run."""
+    d, Ad, bd, H, meta, s = _solver("lvl3_t4", relax_threads=4)
+    u = dev(d["hsgs_u_in"])
+    s.hsgs(0, dev(d["hsgs_f"]).data_ptr(), u.data_ptr())
+    assert relerr(host(u), d["hsgs_u_out"]) < OP_TOL
+    uv = dev(np.zeros(len(d["vcycle_u"])))
+    s.vcycle(dev(d["vcycle_f"]).data_ptr(), uv.data_ptr())
+    assert relerr(host(uv), d["vcycle_u"]) < 1e-9
+    xc = dev(np.zeros(Ad.n))
+    s.precond(dev(d["cpr_r"]).data_ptr(), xc.data_ptr())
+    assert relerr(host(xc), d["cpr_x"]) < 1e-9
+    x, st, hist = s.solve(bd)
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    assert abs(st.iterations - int(d["iterations"][0])) <= 2 and st.converged == 1 and true <= meta["rtol"]
+    assert relerr(x, d["x_final"]) < X_TOL
+    assert s.error_flag() == 0
+    s.close()
+
+
+@pytest.mark.parametrize("name,chunk", [("c1", 8), ("tiny", 3), ("b4", 5), ("b2", 1), ("lvl3", 16), ("nodec", 64),
+                                        ("layered", 32)])
+def test_chunked_smoother_vs_oracle(name, chunk):
+    """relax_chunk = c: ceil(n_l / c) chunks per level, against the oracle's
+    threaded relax 6 with the same chunk counts; the solve stays within two
+    iterations of the one-thread reference."""
+    d, Ad, bd, H, meta, s = _solver(name, relax_chunk=chunk)
+    o = Oracle(Ad, H)
+    T = _threads_per_level(H, relax_chunk=chunk)
+    o.set_relax_threads(T)
+    rng = np.random.default_rng(chunk)
+    for l in range(max(H.num_levels - 1, 1)):
+        n = len(H.A[l][0]) - 1
+        f, u0 = rng.standard_normal(n), rng.standard_normal(n)
+        u = dev(u0)
+        s.hsgs(l, dev(f).data_ptr(), u.data_ptr())
+        assert relerr(host(u), o.relax6_threads(l, f, u0, T[l])) < OP_TOL
+    f = rng.standard_normal(len(H.A[0][0]) - 1)
+    uv = dev(np.zeros_like(f))
+    s.vcycle(dev(f).data_ptr(), uv.data_ptr())
+    assert relerr(host(uv), o.vcycle(f)) < 1e-9
+    r = rng.standard_normal(Ad.n)
+    p = o.params(stage2_maxit=int(meta["stage2_maxit"]))
+    xc = dev(np.zeros(Ad.n))
+    s.precond(dev(r).data_ptr(), xc.data_ptr())
+    assert relerr(host(xc), o.cpr(r, p)) < 1e-9
+    x, st, hist = s.solve(bd)
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    print(f"[gpu] {name} chunk {chunk}: iters {st.iterations} (reference one-thread {int(d['iterations'][0])}) "
+          f"true_rel {true:.3e}", flush=True)
+    assert abs(st.iterations - int(d["iterations"][0])) <= 2
+    assert st.converged == 1 and true <= meta["rtol"]
+    assert s.error_flag() == 0
+    s.close()
+
+
 def test_solve_is_deterministic_and_device_entry_agrees():
     d, Ad, bd, H, meta, s = _solver("lvl3")
     x1, st1, _ = s.solve(bd)
@@ -112,3 +177,31 @@ def test_nan_rhs_is_reported_as_error():
     x, st, _ = s.solve(bd)
     assert st.converged == 1
     s.close()
+
+
+def test_c2_host_setup_and_gpu_solve_vs_reference():
+    """Full 60x220x85 system: host-built hierarchy (bit-identical to the
+    reference's) and the GPU solve against the reference's own run."""
+    from tests.conftest import fixture_gen_params, load_fixture
+    d = load_fixture("c2")
+    A, rhs = jx.generate(**fixture_gen_params(d))
+    Ad, bd = jx.decouple_abf(A, rhs)
+    ia, ja, a = jx.extract_pressure(Ad)
+    H = jx.amg_setup(ia, ja, a)
+    assert [len(m[0]) - 1 for m in H.A] == list(d["level_sizes"])
+    print(f"[gpu] case=c2 nb={Ad.nb} levels={H.num_levels}", flush=True)
+    S = Ad.to_scipy()
+    ref_it = int(d["iterations"][0])
+    for chunk in (0, jx.BENCH_RELAX_CHUNK):
+        s = jx.Solver(Ad, H, jx.default_params(tol=1e-6, max_iter=100, relax_chunk=chunk))
+        x, st, hist = s.solve(bd)
+        true = np.linalg.norm(bd - S @ x) / np.linalg.norm(bd)
+        print(f"[gpu] c2 chunk {chunk}: iters {st.iterations} (reference {ref_it}) true_rel {true:.3e} "
+              f"solve {st.t_solve_ms:.1f} ms", flush=True)
+        assert abs(st.iterations - ref_it) <= 2
+        assert st.converged == 1 and true <= 1e-6
+        if chunk == 0:
+            n = min(len(hist), len(d["res_history"]))
+            assert relerr(hist[:n], d["res_history"][:n]) < 1e-6
+        assert s.error_flag() == 0
+        s.close()

@@ -139,3 +139,24 @@ def test_solver_rejects_bad_input_without_gpu_calls():
     hc2 = bad2.c_struct()
     rc = jx.lib().jxamd_solver_create(ctypes.byref(s), ctypes.byref(ac), ctypes.byref(hc2), ctypes.byref(p))
     assert rc == jx.ERROR_ARG
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_amg_setup_bit_exact_vs_reference(name):
+    """The host hierarchy (strength, HMIS, ext+i, RAP) equals the reference's."""
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    ia, ja, a = jx.extract_pressure(Ad)
+    G = jx.amg_setup(ia, ja, a)
+    assert G.num_levels == H.num_levels
+    for l in range(G.num_levels):
+        for x, y in [(G.A[l], H.A[l])] + ([(G.P[l], H.P[l])] if l < G.num_levels - 1 else []):
+            assert x[3] == y[3]
+            assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2])
+
+
+def test_amg_setup_rejects_diagonal_not_first():
+    ia = np.array([0, 2, 4], np.int32)
+    ja = np.array([1, 0, 1, 0], np.int32)
+    a = np.array([-1.0, 2.0, 2.0, -1.0])
+    with pytest.raises(RuntimeError):
+        jx.amg_setup(ia, ja, a)

@@ -92,6 +92,52 @@ def test_relax6_is_forward_then_backward_triangular_solve():
     assert relerr(got, u2) < 1e-13
 
 
+def test_threaded_relax6_bit_exact_vs_reference():
+    """OMP_NUM_THREADS=4 reference run: relax 6 splits every level into four
+    contiguous chunks (par_relax_6.c:132-223)."""
+    d, A, rhs, Ad, bd, H, meta = system("lvl3_t4")
+    assert int(meta["threads"]) == 4
+    o = Oracle(Ad, H)
+    assert np.array_equal(o.relax6_threads(0, d["hsgs_f"], d["hsgs_u_in"], 4), d["hsgs_u_out"])
+    # the one-thread sweep gives a different answer on the same inputs
+    assert not np.array_equal(o.relax6(0, d["hsgs_f"], d["hsgs_u_in"]), d["hsgs_u_out"])
+    o.set_relax_threads([4] * H.num_levels)
+    assert np.array_equal(o.vcycle(d["vcycle_f"]), d["vcycle_u"])
+    assert np.array_equal(o.cpr(d["cpr_r"]), d["cpr_x"])
+    x, st, hist, rc = o.gmres(bd, o.params(tol=meta["rtol"]))
+    # the threaded reference also sums its dot products in four parts
+    assert st.iterations == int(d["iterations"][0])
+    assert relerr(hist, d["res_history"]) < 1e-12
+
+
+@pytest.mark.parametrize("T", [2, 3, 7, 40])
+def test_threaded_relax6_is_block_jacobi_over_chunk_ssor(T):
+    """T chunks: forward then backward Gauss-Seidel on each chunk's diagonal
+    block, every coupling outside the chunk taken at the entry values; T = n
+    reduces to one Jacobi step."""
+    import jxpamg_all_amd as jx
+    A, csr = _small_pressure_system()
+    n = A.shape[0]
+    Ab = jx.BSRMatrix(1, 2, np.array([0, 1], np.int32), np.array([0], np.int32), np.eye(2).ravel())
+    o = Oracle(Ab, jx.Hierarchy([csr], []))
+    rng = np.random.default_rng(T)
+    f, u0 = rng.standard_normal(n), rng.standard_normal(n)
+    got = o.relax6_threads(0, f, u0, T)
+    size, rest = divmod(n, T)
+    expect = np.empty(n)
+    Ad = A.toarray()
+    for c in range(T):
+        ns = c * size + min(c, rest)
+        ne = ns + size + (1 if c < rest else 0)
+        B = Ad[ns:ne, ns:ne]
+        g = f[ns:ne] - Ad[ns:ne, :ns] @ u0[:ns] - Ad[ns:ne, ne:] @ u0[ne:]
+        u1 = sla.solve_triangular(np.tril(B), g - np.triu(B, 1) @ u0[ns:ne], lower=True)
+        expect[ns:ne] = sla.solve_triangular(np.triu(B), g - np.tril(B, -1) @ u1, lower=False)
+    assert relerr(got, expect) < 1e-12
+    jac = (f - (A - sp.diags(A.diagonal())) @ u0) / A.diagonal()
+    assert relerr(o.relax6_threads(0, f, u0, n), jac) < 1e-14
+
+
 def test_gselim_solves_diagonally_dominant_systems():
     import ctypes
     from oracle.cpu_ref import lib
