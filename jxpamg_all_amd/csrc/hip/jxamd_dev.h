@@ -38,6 +38,12 @@ int jxamd_k_bhgs_sweep(int b, int nsl, const int *perm, const int *sptr, const i
                        const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
                        double *xn, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, double w,
                        double omega, unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
+/* hybrid SSOR with short chunks (at most JXAMD_CHUNK_LDS_MAX rows): SELL-64
+ * of the off-diagonals in natural row order, diagonal in diag[i] */
+#define JXAMD_CHUNK_LDS_MAX 16
+int jxamd_gs_chunk_lds_rows(int n, int nchunks); /* longest chunk */
+int jxamd_k_gs_chunk_lds(int n, int nchunks, const int *sptr, const int *col, const double *val, const double *diag,
+                         const double *f, int fs, const double *uo, double *un, void *stream);
 int jxamd_k_gs_chunk(int n, int nchunks, const int *wstep, const int *sptr, const int *col, const double *val,
                      const double *diag, const double *f, int fs, const double *uo, double *un, void *stream);
 int jxamd_k_dense_gemv(int n, const double *M, const double *x, double *y, void *stream);

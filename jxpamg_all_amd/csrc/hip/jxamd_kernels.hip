@@ -365,6 +365,96 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same hybrid SSOR for short chunks (at most LMAX rows), as one pass over
+// the matrix.  Every coupling to a row outside the chunk is read at its entry
+// value in both the forward and the backward sweep, so it folds into the
+// right-hand side: g_i = f_i - sum_{j outside} a_ij uo_j.  What is left per
+// chunk is a len x len block B of in-chunk couplings, and both sweeps are
+// Gauss-Seidel on (D + B) u = g starting from the entry values (a later row
+// of the chunk is still at its entry value when the forward sweep reads it).
+//   phase A  one lane per row (SELL-64 rows in natural order, off-diagonals,
+//            diagonal in diag[i]): g_i, d_i, entry value and the row of B
+//            into LDS
+//   phase B  one lane per chunk: forward then backward sweep, u in registers
+//   phase C  one lane per row: un_i from LDS (coalesced store)
+// Block b owns chunks [b*cpb, (b+1)*cpb), at most 256 rows.  Rows with a zero
+// diagonal keep their entry value.
+// ---------------------------------------------------------------------------
+template <int LMAX>
+__global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cpb, const int *__restrict__ sptr,
+                                                      const int *__restrict__ col, const double *__restrict__ val,
+                                                      const double *__restrict__ diag, const double *__restrict__ f,
+                                                      int fs, const double *__restrict__ uo, double *__restrict__ un)
+{
+    constexpr int BST = LMAX * LMAX + 1; // odd stride between chunk blocks: spreads LDS banks
+    extern __shared__ double sm[];
+    double *g = sm, *ux = sm + 256, *dd = sm + 512, *B = sm + 768;
+    const int size = n / nchunks, rest = n - size * nchunks;
+    const int c0 = blockIdx.x * cpb;
+    const int c1 = min(c0 + cpb, nchunks);
+    const int row0 = c0 < rest ? c0 * (size + 1) : c0 * size + rest;
+    const int row1 = c1 < rest ? c1 * (size + 1) : c1 * size + rest;
+    const int r = threadIdx.x;
+    const int i = row0 + r;
+    if (i < row1) {
+        const int big = rest * (size + 1);
+        const int c = i < big ? i / (size + 1) : rest + (i - big) / size;
+        const int ns = c < rest ? c * (size + 1) : c * size + rest;
+        const int len = c < rest ? size + 1 : size;
+        double *Brow = B + (size_t)(c - c0) * BST + (i - ns) * LMAX;
+        for (int k = 0; k < len; k++) Brow[k] = 0.0;
+        double acc = f[(size_t)i * fs];
+        const int s = i / WAVE, lane = i % WAVE;
+        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+            const size_t e = (size_t)k * WAVE + lane;
+            const int j = col[e];
+            const double v = val[e];
+            if (j >= ns && j < ns + len) {
+                if (j != i) Brow[j - ns] += v;
+            } else if (uo) {
+                acc -= v * uo[j];
+            }
+        }
+        dd[r] = diag[i];
+        g[r] = acc;
+        ux[r] = uo ? uo[i] : 0.0;
+    }
+    __syncthreads();
+    const int c = c0 + (int)threadIdx.x;
+    if (c < c1) {
+        const int ns = c < rest ? c * (size + 1) : c * size + rest;
+        const int len = c < rest ? size + 1 : size;
+        const int rl = ns - row0;
+        const double *Bq = B + (size_t)(c - c0) * BST;
+        double u[LMAX];
+#pragma unroll
+        for (int t = 0; t < LMAX; t++) u[t] = t < len ? ux[rl + t] : 0.0;
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+            for (int tt = 0; tt < LMAX; tt++) {
+                const int t = pass == 0 ? tt : LMAX - 1 - tt;
+                if (t < len) {
+                    const double d = dd[rl + t];
+                    if (d != 0.0) {
+                        double x = g[rl + t];
+#pragma unroll
+                        for (int q = 0; q < LMAX; q++)
+                            if (q != t && q < len) x -= Bq[t * LMAX + q] * u[q];
+                        u[t] = x / d;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < LMAX; t++)
+            if (t < len) ux[rl + t] = u[t];
+    }
+    __syncthreads();
+    if (i < row1) un[i] = ux[r];
+}
+
 // dense y = M x (row-major n x n), one wavefront per row
 __global__ void __launch_bounds__(256) k_dense_gemv(int n, const double *__restrict__ M, const double *__restrict__ x,
                                                     double *__restrict__ y)
@@ -633,6 +723,32 @@ int jxamd_k_gs_chunk(int n, int nchunks, const int *wstep, const int *sptr, cons
     const int waves = (nchunks + WAVE - 1) / WAVE;
     k_gs_chunk<<<(waves + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, nchunks, wstep, sptr, col, val, diag, f, fs, uo,
                                                                   un);
+    return chk(hipGetLastError());
+}
+
+int jxamd_gs_chunk_lds_rows(int n, int nchunks)
+{
+    if (nchunks < 1 || n < nchunks) return 0;
+    const int size = n / nchunks;
+    return size + (n % nchunks ? 1 : 0);
+}
+
+int jxamd_k_gs_chunk_lds(int n, int nchunks, const int *sptr, const int *col, const double *val, const double *diag,
+                         const double *f, int fs, const double *uo, double *un, void *stream)
+{
+    if (n == 0) return 0;
+    const int maxlen = jxamd_gs_chunk_lds_rows(n, nchunks);
+    if (maxlen < 1 || maxlen > JXAMD_CHUNK_LDS_MAX) return JXAMD_ERROR_ARG;
+    const int cpb = 256 / maxlen;
+    const int grid = (nchunks + cpb - 1) / cpb;
+    hipStream_t st = (hipStream_t)stream;
+#define LAUNCH_LDS(LM)                                                                                                 \
+    k_gs_chunk_lds<LM><<<grid, 256, sizeof(double) * (768 + (size_t)cpb * (LM * LM + 1)), st>>>(                     \
+        n, nchunks, cpb, sptr, col, val, diag, f, fs, uo, un)
+    if (maxlen <= 4) LAUNCH_LDS(4);
+    else if (maxlen <= 8) LAUNCH_LDS(8);
+    else LAUNCH_LDS(16);
+#undef LAUNCH_LDS
     return chk(hipGetLastError());
 }
 

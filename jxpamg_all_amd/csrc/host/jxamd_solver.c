@@ -596,8 +596,19 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     if (s->prm.relax_threads > 1) T = s->prm.relax_threads;
     else if (s->prm.relax_chunk > 0) T = (n + s->prm.relax_chunk - 1) / s->prm.relax_chunk;
     if (T > n) T = n;
-    if (T > 1 && (l < s->L - 1 || s->L == 1)) {
-        /* threaded relax 6: wave w holds chunks 64w..64w+63; slice wstep[w]+t
+    if (T > 1 && (l < s->L - 1 || s->L == 1) && jxamd_gs_chunk_lds_rows(n, T) <= JXAMD_CHUNK_LDS_MAX) {
+        /* short chunks: one pass, off-diagonals in natural row order */
+        e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 1, 1);
+        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        CK(e);
+        s->nchunks[l] = T;
+        s->wstep[l] = NULL;
+        /* matrix once, f + entry values + results */
+        s->Gc[l].abytes = 12.0 * A->ia[n] + 4.0 * (n + 1) + 24.0 * n;
+    } else if (T > 1 && (l < s->L - 1 || s->L == 1)) {
+        /* long chunks: wave w holds chunks 64w..64w+63; slice wstep[w]+t
          * holds row ns(c)+t of each chunk c of the wave */
         const int size = n / T, rest = n - size * T;
         const int waves = (T + SELLW - 1) / SELLW;
@@ -828,8 +839,12 @@ static int hsgs_chunk(jxamd_solver *s, int l, const double *f, int fs, const dou
 {
     dsell *g = &s->Gc[l];
     int pq = prof_begin(s);
-    CK(jxamd_k_gs_chunk(s->nlev[l], s->nchunks[l], s->wstep[l], g->sptr, g->col, g->val, g->diag, f, fs, uo, un,
-                        s->stream));
+    if (s->wstep[l])
+        CK(jxamd_k_gs_chunk(s->nlev[l], s->nchunks[l], s->wstep[l], g->sptr, g->col, g->val, g->diag, f, fs, uo, un,
+                            s->stream));
+    else
+        CK(jxamd_k_gs_chunk_lds(s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->diag, f, fs, uo, un,
+                                s->stream));
     prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
     return 0;
 }
