@@ -171,6 +171,19 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *d_b, double *d_x, d
 /* x = M^{-1} r, device vectors.  Replaces JXF_CPRPrecond (jxf_cpr.c:424-610). */
 int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x);
 
+/* The CSR-array solve entry with the arguments of JXFPAMG_Solver_Interface
+ * (src/jxpamg_interface.c:118-257) minus the MPI communicator (one GPU, one
+ * process), for solver_id 220 = CPR-GMRES (src/jxpamg_interface.c:1967-2043).
+ * row = nb*num_functions node-major rows; ia/ja may be 1-based (decremented
+ * in place, :143-150) and each row's diagonal is swapped to the front in
+ * place (jxf_CSRMatrixReorder, seq_csr_matop.c:189-231).  Regroups into BSR
+ * with b = num_functions (2..4), builds the pressure-AMG hierarchy on the
+ * host, solves from x = 0 on the GPU and writes ser_x and *iter.  argv:
+ * -kdim, -maxiter (reference options), -rchunk, -rthreads (relax-6 chunking).
+ * Any other solver_id returns JXAMD_ERROR_ARG. */
+int jxamd_solver_interface(int argc, char *argv[], int row, int *ia, int *ja, double *a, double *ser_x,
+                           double *ser_b, double tol, int solver_id, int num_functions, int *iter);
+
 /* Single operators on device vectors (parity tests, benchmarks). */
 /* y = alpha*A*x + beta*y, jxf_BSRMatrixMatvec (seq_bsr_matvec.c:21-134) */
 int jxamd_op_bsr_matvec(jxamd_solver *s, double alpha, const double *d_x, double beta, double *d_y);

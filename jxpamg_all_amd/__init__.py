@@ -110,6 +110,7 @@ def _declare(L):
         "jxamd_amg_setup": [vp, vp, vp],
         "jxamd_hierarchy_free": [vp],
         "jxamd_cpr_precond_timed": [vp, vp, vp, i32, vp],
+        "jxamd_solver_interface": [i32, vp, i32, vp, vp, vp, vp, vp, f64, i32, i32, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -194,6 +195,23 @@ def generate(**kw):
     rhs = np.zeros(nb.value * b)
     check(L.jxamd_gen_reservoir(ctypes.byref(p), _p(ia), _p(ja), _p(a), _p(rhs)), "jxamd_gen_reservoir")
     return BSRMatrix(nb.value, b, ia, ja, a), rhs
+
+
+def solver_interface(row, ia, ja, a, ser_x, ser_b, tol, solver_id, num_functions, argv=()):
+    """JXFPAMG_Solver_Interface (src/jxpamg_interface.c:118-257), solver_id 220.
+    ia/ja/a are numpy arrays modified in place as by the reference (1-based
+    indices decremented, diagonal swapped to the front of each row); the
+    solution is written into ser_x.  Returns the iteration count."""
+    for arr, dt in ((ia, np.int32), (ja, np.int32), (a, np.float64), (ser_x, np.float64), (ser_b, np.float64)):
+        if arr.dtype != dt or not arr.flags.c_contiguous:
+            raise TypeError("solver_interface works in place on contiguous int32 / float64 arrays")
+    args = [b"jxamd"] + [str(v).encode() for v in argv]
+    cargv = (ctypes.c_char_p * (len(args) + 1))(*args, None)
+    it = ctypes.c_int(0)
+    check(lib().jxamd_solver_interface(len(args), ctypes.cast(cargv, ctypes.c_void_p), int(row), _p(ia), _p(ja),
+                                       _p(a), _p(ser_x), _p(ser_b), float(tol), int(solver_id), int(num_functions),
+                                       ctypes.byref(it)), "jxamd_solver_interface")
+    return it.value
 
 
 def decouple_abf(A, rhs=None):

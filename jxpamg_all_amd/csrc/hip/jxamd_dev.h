@@ -34,10 +34,12 @@ int jxamd_k_gs_sweep(int nsl, const int *perm, const int *sptr, const int *col, 
                      const double *f, int fs, const double *uo, double *un, const int *dep_ptr, const int *dep_list,
                      int *sflag, int epoch, int dir, unsigned long long *ticket, unsigned long long *base, int *err,
                      void *stream);
-int jxamd_k_bhgs_sweep(int b, int nsl, const int *perm, const int *sptr, const int *col, const double *val,
-                       const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
-                       double *xn, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, double w,
-                       double omega, unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
+int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,
+                       const double *val, const double *dinv, const double *rhs, const double *x0, const double *xp,
+                       int pidx, double *xs, long long xstride, double *xout, const int *dlo_ptr,
+                       const int *dlo_list, const int *dall_ptr, const int *dall_list, int *sflag, int epoch,
+                       double w, double omega, unsigned long long *ticket, unsigned long long *base, int *err,
+                       void *stream);
 /* hybrid SSOR with short chunks (at most JXAMD_CHUNK_LDS_MAX rows): SELL-64
  * of the off-diagonals in natural row order, diagonal in diag[i] */
 #define JXAMD_CHUNK_LDS_MAX 16

@@ -215,27 +215,42 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
     }
 }
 
-// One forward block hybrid SOR sweep on the BSR matrix (par_relax_bsr.c
-// semantics, one rank):
-//   tmp = omega*b_i - sum_{j!=i} A_ij x_j(cur) + (1-omega) sum_{j!=i} A_ij x_j(old)
-//   x_i <- (1 - w*omega) x_i(old) + w * Dinv_i tmp
-// Old values come from xo, or, when xo == nullptr, from the prolongated
-// pressure vector xp (pressure slot pidx, zeros elsewhere).  Writes xn.
-// Dependencies are the lower block columns (j < i).
+// K forward block hybrid SOR sweeps on the BSR matrix in one launch
+// (par_relax_bsr.c semantics, one rank), sweep k from x^(k-1) to x^(k):
+//   tmp = omega*b_i - sum_{j<i} A_ij x^(k)_j - sum_{j>i} A_ij x^(k-1)_j
+//         + (1-omega) sum_{j!=i} A_ij x^(k-1)_j
+//   x^(k)_i = (1 - w*omega) x^(k-1)_i + w * Dinv_i tmp
+// x^(-1) is x0, or, when x0 == nullptr, the prolongated pressure vector xp
+// (pressure slot pidx, zeros elsewhere).  x^(k) for k < K-1 goes to
+// xs + k*xstride, x^(K-1) to xout.
+// Work items are virtual slices v = k*nsl + s taken in the order vorder (the
+// identity if nullptr), a topological order of
+//   (k, s) after (k, s') for s' in dlo(s)   (slices of lower neighbours)
+//   (k, s) after (k-1, s'') for s'' in dall(s) (slices of all neighbours, s)
+// so sweep k+1 starts on a slice as soon as sweep k has passed its
+// neighbourhood: the K sweeps overlap instead of running back to back.
+// Flags: sflag[k*nsl + s].
 template <int B>
-__global__ void __launch_bounds__(256) k_bhgs_sweep(int nsl, const int *__restrict__ perm, const int *__restrict__ sptr,
-                                                    const int *__restrict__ col, const double *__restrict__ val,
-                                                    const double *__restrict__ dinv, const double *__restrict__ rhs,
-                                                    const double *xo, const double *__restrict__ xp, int pidx,
-                                                    double *xn, const int *__restrict__ dep_ptr,
-                                                    const int *__restrict__ dep_list, int *sflag, int epoch, double w,
-                                                    double omega, unsigned long long *ticket, unsigned long long base,
-                                                    int *err)
+__global__ void __launch_bounds__(256)
+k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__restrict__ perm,
+             const int *__restrict__ sptr, const int *__restrict__ col, const double *__restrict__ val,
+             const double *__restrict__ dinv, const double *__restrict__ rhs, const double *__restrict__ x0,
+             const double *__restrict__ xp, int pidx, double *xs, long long xstride, double *xout,
+             const int *__restrict__ dlo_ptr, const int *__restrict__ dlo_list, const int *__restrict__ dall_ptr,
+             const int *__restrict__ dall_list, int *sflag, int epoch, double w, double omega,
+             unsigned long long *ticket, unsigned long long base, int *err)
 {
     const int lane = threadIdx.x % WAVE;
+    const int K = nvs / nsl;
     for (;;) {
-        const int s = claim_slice(ticket, base, lane);
-        if (s < 0 || s >= nsl) break;
+        const int t = claim_slice(ticket, base, lane);
+        if (t < 0 || t >= nvs) break;
+        const int v = vorder ? vorder[t] : t;
+        const int k = v / nsl, s = v - k * nsl;
+        double *xn = k == K - 1 ? xout : xs + (size_t)k * xstride;
+        const double *xo = k == 0 ? x0 : xs + (size_t)(k - 1) * xstride;
+        const bool prev = k > 0; // old values written in this launch
+        if (prev) wait_slices(dall_ptr, dall_list, sflag + (size_t)(k - 1) * nsl, s, epoch, lane, err);
         const int p = s * WAVE + lane;
         const int i = perm[p];
         const int k0 = sptr[s], k1 = sptr[s + 1];
@@ -244,54 +259,65 @@ __global__ void __launch_bounds__(256) k_bhgs_sweep(int nsl, const int *__restri
         for (int r = 0; r < B; r++) { res0[r] = 0.0; res2[r] = 0.0; }
         if (i >= 0) {
             // all old-value terms, and the current-value terms of upper neighbours
-            for (int k = k0; k < k1; k++) {
-                const int j = col[(size_t)k * WAVE + lane];
+            for (int kk = k0; kk < k1; kk++) {
+                const int j = col[(size_t)kk * WAVE + lane];
                 if (j == i) continue; // padding slot
                 double xold[B];
 #pragma unroll
-                for (int t = 0; t < B; t++) xold[t] = xo ? xo[(size_t)j * B + t] : (t == pidx ? xp[j] : 0.0);
-                const double *v = val + (size_t)k * B * B * WAVE + lane;
+                for (int q = 0; q < B; q++)
+                    xold[q] = prev ? ld_coherent(xo + (size_t)j * B + q)
+                                   : (xo ? xo[(size_t)j * B + q] : (q == pidx ? xp[j] : 0.0));
+                const double *vv = val + (size_t)kk * B * B * WAVE + lane;
 #pragma unroll
                 for (int r = 0; r < B; r++) {
 #pragma unroll
-                    for (int t = 0; t < B; t++) {
-                        const double a = v[(r * B + t) * WAVE];
-                        res2[r] += a * xold[t];
-                        if (j > i) res0[r] -= a * xold[t];
+                    for (int q = 0; q < B; q++) {
+                        const double a = vv[(r * B + q) * WAVE];
+                        res2[r] += a * xold[q];
+                        if (j > i) res0[r] -= a * xold[q];
                     }
                 }
             }
         }
-        wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
+        double xi[B], rb[B], D[B * B];
         if (i >= 0) {
-            for (int k = k0; k < k1; k++) {
-                const int j = col[(size_t)k * WAVE + lane];
+#pragma unroll
+            for (int r = 0; r < B; r++) {
+                xi[r] = prev ? ld_coherent(xo + (size_t)i * B + r)
+                             : (xo ? xo[(size_t)i * B + r] : (r == pidx ? xp[i] : 0.0));
+                rb[r] = rhs[(size_t)i * B + r];
+            }
+#pragma unroll
+            for (int q = 0; q < B * B; q++) D[q] = dinv[(size_t)i * B * B + q];
+        }
+        wait_slices(dlo_ptr, dlo_list, sflag + (size_t)k * nsl, s, epoch, lane, err);
+        if (i >= 0) {
+            for (int kk = k0; kk < k1; kk++) {
+                const int j = col[(size_t)kk * WAVE + lane];
                 if (j >= i) continue;
                 double xc[B];
 #pragma unroll
-                for (int t = 0; t < B; t++) xc[t] = ld_coherent(xn + (size_t)j * B + t);
-                const double *v = val + (size_t)k * B * B * WAVE + lane;
+                for (int q = 0; q < B; q++) xc[q] = ld_coherent(xn + (size_t)j * B + q);
+                const double *vv = val + (size_t)kk * B * B * WAVE + lane;
 #pragma unroll
                 for (int r = 0; r < B; r++) {
 #pragma unroll
-                    for (int t = 0; t < B; t++) res0[r] -= v[(r * B + t) * WAVE] * xc[t];
+                    for (int q = 0; q < B; q++) res0[r] -= vv[(r * B + q) * WAVE] * xc[q];
                 }
             }
             double tmp[B];
 #pragma unroll
-            for (int r = 0; r < B; r++) tmp[r] = omega * rhs[(size_t)i * B + r] + res0[r] + (1.0 - omega) * res2[r];
+            for (int r = 0; r < B; r++) tmp[r] = omega * rb[r] + res0[r] + (1.0 - omega) * res2[r];
             const double prod = 1.0 - w * omega;
-            const double *D = dinv + (size_t)i * B * B;
 #pragma unroll
             for (int r = 0; r < B; r++) {
                 double o = 0.0;
 #pragma unroll
-                for (int t = 0; t < B; t++) o += D[r * B + t] * tmp[t];
-                const double xi = xo ? xo[(size_t)i * B + r] : (r == pidx ? xp[i] : 0.0);
-                st_coherent(xn + (size_t)i * B + r, xi * prod + w * o);
+                for (int q = 0; q < B; q++) o += D[r * B + q] * tmp[q];
+                st_coherent(xn + (size_t)i * B + r, xi[r] * prod + w * o);
             }
         }
-        st_flag_release(sflag + s, epoch);
+        st_flag_release(sflag + (size_t)k * nsl + s, epoch);
     }
 }
 
@@ -696,23 +722,33 @@ int jxamd_k_gs_sweep(int nsl, const int *perm, const int *sptr, const int *col, 
     return chk(hipGetLastError());
 }
 
-int jxamd_k_bhgs_sweep(int b, int nsl, const int *perm, const int *sptr, const int *col, const double *val,
-                       const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx,
-                       double *xn, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, double w,
-                       double omega, unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,
+                       const double *val, const double *dinv, const double *rhs, const double *x0, const double *xp,
+                       int pidx, double *xs, long long xstride, double *xout, const int *dlo_ptr,
+                       const int *dlo_list, const int *dall_ptr, const int *dall_list, int *sflag, int epoch,
+                       double w, double omega, unsigned long long *ticket, unsigned long long *base, int *err,
+                       void *stream)
 {
-    if (nsl == 0) return 0;
-    const int waves = jxamd_sweep_waves(nsl);
+    if (nsl == 0 || K < 1) return 0;
+    if (K > 1 && (!vorder || !xs || !dall_ptr || !dall_list)) return JXAMD_ERROR_ARG;
+    if ((long long)nsl * K > 0x7fffffffLL) return JXAMD_ERROR_ARG;
+    const int nvs = nsl * K;
+    const int waves = jxamd_sweep_waves(nvs);
     const int grid = (waves + 3) / 4;
     hipStream_t st = (hipStream_t)stream;
     const unsigned long long b0 = *base;
+#define LAUNCH_BHGS(BS)                                                                                                \
+    k_bhgs_multi<BS><<<grid, 256, 0, st>>>(nsl, nvs, vorder, perm, sptr, col, val, dinv, rhs, x0, xp, pidx, xs,       \
+                                           xstride, xout, dlo_ptr, dlo_list, dall_ptr, dall_list, sflag, epoch, w,     \
+                                           omega, ticket, b0, err)
     switch (b) {
-    case 2: k_bhgs_sweep<2><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, dep_ptr, dep_list, sflag, epoch, w, omega, ticket, b0, err); break;
-    case 3: k_bhgs_sweep<3><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, dep_ptr, dep_list, sflag, epoch, w, omega, ticket, b0, err); break;
-    case 4: k_bhgs_sweep<4><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, dep_ptr, dep_list, sflag, epoch, w, omega, ticket, b0, err); break;
+    case 2: LAUNCH_BHGS(2); break;
+    case 3: LAUNCH_BHGS(3); break;
+    case 4: LAUNCH_BHGS(4); break;
     default: return JXAMD_ERROR_ARG;
     }
-    *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
+#undef LAUNCH_BHGS
+    *base += (unsigned long long)nvs + (unsigned long long)grid * 4;
     return chk(hipGetLastError());
 }
 

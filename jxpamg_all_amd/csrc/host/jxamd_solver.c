@@ -46,9 +46,10 @@ typedef struct {
     int *sptr, *col;
     double *val;
     /* forward block-HGS order */
-    int gnsl;
+    int gnsl, K;             /* K: stage-2 sweeps fused into one launch */
     int *gperm, *gsptr, *gcol, *flag, *gdep_ptr, *gdep_list;
-    double *gval, *dinv;
+    int *gall_ptr, *gall_list, *gvorder; /* neighbourhood slices, sweep-overlapped order */
+    double *gval, *dinv, *xs;  /* xs: x^(k) of the first K-1 sweeps */
     double spmv_bytes, sweep_bytes; /* algorithmic bytes: y = A x, one block-HGS sweep */
 } dbsr;
 
@@ -162,12 +163,12 @@ void jxamd_params_default(jxamd_params *p)
  * that hold its rows' dependencies. */
 typedef struct {
     int npos, nsl, nlevels;
-    int *perm, *dep_ptr, *dep_list;
+    int *perm, *dep_ptr, *dep_list, *slev; /* slev: level of each slice */
 } hsched;
 
 static void hsched_free(hsched *h)
 {
-    free(h->perm); free(h->dep_ptr); free(h->dep_list);
+    free(h->perm); free(h->dep_ptr); free(h->dep_list); free(h->slev);
     memset(h, 0, sizeof(*h));
 }
 
@@ -198,8 +199,11 @@ static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *
     o->nsl = o->npos / SELLW;
     o->nlevels = nl;
     o->perm = (int *)malloc(sizeof(int) * (o->npos > 0 ? o->npos : 1));
+    o->slev = (int *)malloc(sizeof(int) * (o->nsl > 0 ? o->nsl : 1));
     int *sl = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
-    if (!o->perm || !sl) { free(lev); free(start); free(cnt); free(sl); return JXAMD_ERROR_MEMORY; }
+    if (!o->perm || !o->slev || !sl) { free(lev); free(start); free(cnt); free(sl); return JXAMD_ERROR_MEMORY; }
+    for (int l = 0; l < nl; l++)
+        for (long q = start[l] / SELLW; q < start[l + 1] / SELLW; q++) o->slev[q] = l;
     for (int p = 0; p < o->npos; p++) o->perm[p] = -1;
     for (int l = 0; l < nl; l++) cnt[l] = 0;
     for (int i = 0; i < n; i++) { /* rows in ascending order inside a level */
@@ -243,6 +247,92 @@ static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *
     o->dep_ptr[o->nsl] = (int)used;
     free(lev); free(start); free(cnt); free(sl); free(mark);
     if (bad) { hsched_free(o); return bad == 2 ? JXAMD_ERROR_MEMORY : JXAMD_ERROR_GENERIC; }
+    return 0;
+}
+
+/* Per slice of a schedule: the distinct slices holding any neighbour of its
+ * rows, and the slice itself. */
+static int build_nbhd(int n, const int *ia, const int *ja, const hsched *sc, int **optr, int **olist)
+{
+    const int nsl = sc->nsl;
+    int *sl = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int *mark = (int *)malloc(sizeof(int) * (nsl > 0 ? nsl : 1));
+    int *ptr = (int *)malloc(sizeof(int) * ((size_t)nsl + 1));
+    size_t cap = (size_t)nsl * 8 + 16, used = 0;
+    int *list = (int *)malloc(sizeof(int) * cap);
+    if (!sl || !mark || !ptr || !list) { free(sl); free(mark); free(ptr); free(list); return JXAMD_ERROR_MEMORY; }
+    for (int p = 0; p < sc->npos; p++)
+        if (sc->perm[p] >= 0) sl[sc->perm[p]] = p / SELLW;
+    for (int q = 0; q < nsl; q++) mark[q] = -1;
+    for (int q = 0; q < nsl; q++) {
+        ptr[q] = (int)used;
+        for (int l = -1; l < SELLW; l++) {
+            int i = l < 0 ? -1 : sc->perm[q * SELLW + l];
+            if (l >= 0 && i < 0) continue;
+            int k0 = l < 0 ? 0 : ia[i], k1 = l < 0 ? 1 : ia[i + 1];
+            for (int k = k0; k < k1; k++) {
+                int d = l < 0 ? q : sl[ja[k]];
+                if (mark[d] == q) continue;
+                mark[d] = q;
+                if (used == cap) {
+                    cap *= 2;
+                    int *t = (int *)realloc(list, sizeof(int) * cap);
+                    if (!t) { free(sl); free(mark); free(ptr); free(list); return JXAMD_ERROR_MEMORY; }
+                    list = t;
+                }
+                list[used++] = d;
+            }
+        }
+    }
+    ptr[nsl] = (int)used;
+    free(sl);
+    free(mark);
+    *optr = ptr;
+    *olist = list;
+    return 0;
+}
+
+/* Order of the K*nsl virtual slices (k, s) of K overlapped sweeps: by
+ * slev[s] + lag*k, then k, then s, with lag = 1 + the largest level step
+ * from a slice to a slice in its neighbourhood.  Every dependency of (k, s)
+ * -- (k, s') for s' in dep(s) and (k-1, s'') for s'' in nbhd(s) -- comes
+ * earlier; this is checked. */
+static int build_vorder(const hsched *sc, const int *nb_ptr, const int *nb_list, int K, int **out)
+{
+    const int nsl = sc->nsl;
+    int lag = 1;
+    for (int q = 0; q < nsl; q++)
+        for (int e = nb_ptr[q]; e < nb_ptr[q + 1]; e++)
+            if (sc->slev[nb_list[e]] - sc->slev[q] + 1 > lag) lag = sc->slev[nb_list[e]] - sc->slev[q] + 1;
+    const long nkeys = (long)sc->nlevels + (long)lag * (K - 1);
+    const size_t nv = (size_t)nsl * K;
+    long *cnt = (long *)calloc((size_t)nkeys + 1, sizeof(long));
+    int *ord = (int *)malloc(sizeof(int) * (nv > 0 ? nv : 1));
+    int *pos = (int *)malloc(sizeof(int) * (nv > 0 ? nv : 1));
+    if (!cnt || !ord || !pos) { free(cnt); free(ord); free(pos); return JXAMD_ERROR_MEMORY; }
+    for (int k = 0; k < K; k++)
+        for (int q = 0; q < nsl; q++) cnt[sc->slev[q] + (long)lag * k + 1]++;
+    for (long t = 0; t < nkeys; t++) cnt[t + 1] += cnt[t];
+    for (int k = 0; k < K; k++) /* k outer, q inner: stable within a key */
+        for (int q = 0; q < nsl; q++) {
+            long at = cnt[sc->slev[q] + (long)lag * k]++;
+            ord[at] = k * nsl + q;
+        }
+    for (size_t t = 0; t < nv; t++) pos[ord[t]] = (int)t;
+    int bad = 0;
+    for (int k = 0; k < K && !bad; k++)
+        for (int q = 0; q < nsl && !bad; q++) {
+            const int me = pos[k * nsl + q];
+            for (int e = sc->dep_ptr[q]; e < sc->dep_ptr[q + 1]; e++)
+                if (pos[k * nsl + sc->dep_list[e]] >= me) bad = 1;
+            if (k > 0)
+                for (int e = nb_ptr[q]; e < nb_ptr[q + 1]; e++)
+                    if (pos[(k - 1) * nsl + nb_list[e]] >= me) bad = 1;
+        }
+    free(cnt);
+    free(pos);
+    if (bad) { free(ord); return JXAMD_ERROR_GENERIC; }
+    *out = ord;
     return 0;
 }
 
@@ -567,7 +657,21 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         if (!rc) rc = jxamd_h2d(d->dinv, dinv, sizeof(double) * (size_t)nb * bb, s->stream);
         if (!rc) rc = upload_ints(&d->gdep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
         if (!rc) rc = upload_ints(&d->gdep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
-        if (!rc) rc = zero_ints(&d->flag, (size_t)sc.nsl, s->stream);
+        d->K = s->prm.stage2_maxit > 1 ? s->prm.stage2_maxit : 1;
+        if (!rc) rc = zero_ints(&d->flag, (size_t)sc.nsl * d->K, s->stream);
+        if (!rc && d->K > 1) {
+            int *nptr = NULL, *nlist = NULL, *ord = NULL;
+            rc = build_nbhd(nb, A->ia, A->ja, &sc, &nptr, &nlist);
+            if (!rc) rc = build_vorder(&sc, nptr, nlist, d->K, &ord);
+            if (!rc) rc = upload_ints(&d->gall_ptr, nptr, (size_t)sc.nsl + 1, s->stream);
+            if (!rc) rc = upload_ints(&d->gall_list, nlist, (size_t)nptr[sc.nsl], s->stream);
+            if (!rc) rc = upload_ints(&d->gvorder, ord, (size_t)sc.nsl * d->K, s->stream);
+            if (!rc) rc = jxamd_dev_malloc((void **)&d->xs, sizeof(double) * (size_t)(d->K - 1) * nb * b);
+            if (!rc) rc = jxamd_stream_sync(s->stream);
+            free(nptr);
+            free(nlist);
+            free(ord);
+        }
         if (!rc) rc = jxamd_stream_sync(s->stream);
         free(v);
         free(dinv);
@@ -783,6 +887,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(d->gperm); jxamd_dev_free(d->gsptr); jxamd_dev_free(d->gcol);
     jxamd_dev_free(d->gval); jxamd_dev_free(d->dinv); jxamd_dev_free(d->flag);
     jxamd_dev_free(d->gdep_ptr); jxamd_dev_free(d->gdep_list);
+    jxamd_dev_free(d->gall_ptr); jxamd_dev_free(d->gall_list); jxamd_dev_free(d->gvorder); jxamd_dev_free(d->xs);
     for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
         dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]); dsell_free(&s->Gc[l]);
         jxamd_dev_free(s->wstep[l]);
@@ -910,16 +1015,19 @@ static int vcycle(jxamd_solver *s, const double *f0, int fs0, int u0_zero)
     return 0;
 }
 
-static int bhgs_sweep(jxamd_solver *s, const double *rhs, const double *xo, const double *xp, double *xn, double w,
-                      double omega)
+/* K stage-2 sweeps from x0 (or from the prolongated xp if x0 is NULL) into
+ * xn; K is 1 or the fused count d->K */
+static int bhgs_run(jxamd_solver *s, int K, const double *rhs, const double *x0, const double *xp, double *xn,
+                    double w, double omega)
 {
     dbsr *d = &s->A;
     s->epoch++;
     int pq = prof_begin(s);
-    int e = jxamd_k_bhgs_sweep(d->b, d->gnsl, d->gperm, d->gsptr, d->gcol, d->gval, d->dinv, rhs, xo, xp,
-                               s->prm.pressure_index, xn, d->gdep_ptr, d->gdep_list, d->flag, s->epoch, w, omega,
-                               s->ticket, &s->tbase, s->err, s->stream);
-    prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, d->sweep_bytes);
+    int e = jxamd_k_bhgs_multi(d->b, d->gnsl, K, K > 1 ? d->gvorder : NULL, d->gperm, d->gsptr, d->gcol, d->gval,
+                               d->dinv, rhs, x0, xp, s->prm.pressure_index, d->xs, (long long)s->N, xn, d->gdep_ptr,
+                               d->gdep_list, d->gall_ptr, d->gall_list, d->flag, s->epoch, w, omega, s->ticket,
+                               &s->tbase, s->err, s->stream);
+    prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, K * d->sweep_bytes);
     return e;
 }
 
@@ -938,14 +1046,7 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
      * prolongated vector on the fly and the last one lands in x */
     const int m = s->prm.stage2_maxit;
     if (m == 0) return jxamd_k_prolong(s->nb, b, pi, s->U[0], x, s->stream);
-    double *dst = (m % 2 == 1) ? x : s->xtmp;
-    CK(bhgs_sweep(s, r, NULL, s->U[0], dst, s->prm.stage2_weight, s->prm.stage2_omega));
-    for (int it = 1; it < m; it++) {
-        double *src = dst;
-        dst = (src == x) ? s->xtmp : x;
-        CK(bhgs_sweep(s, r, src, NULL, dst, s->prm.stage2_weight, s->prm.stage2_omega));
-    }
-    return 0;
+    return bhgs_run(s, m, r, NULL, s->U[0], x, s->prm.stage2_weight, s->prm.stage2_omega);
 }
 
 int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x)
@@ -1017,7 +1118,7 @@ int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u)
 int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, double omega)
 {
     if (!s) return JXAMD_ERROR_ARG;
-    CK(bhgs_sweep(s, d_b, d_x, NULL, s->xtmp, w, omega));
+    CK(bhgs_run(s, 1, d_b, d_x, NULL, s->xtmp, w, omega));
     CK(jxamd_k_copy(s->N, s->xtmp, d_x, s->stream));
     return jxamd_stream_sync(s->stream);
 }

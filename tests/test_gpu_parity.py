@@ -205,3 +205,51 @@ def test_c2_host_setup_and_gpu_solve_vs_reference():
             assert relerr(hist[:n], d["res_history"][:n]) < 1e-6
         assert s.error_flag() == 0
         s.close()
+
+
+def _node_major_csr(A, one_based=True, seed=0):
+    """BSR -> CSR with b consecutive rows per cell, entries shuffled inside rows."""
+    S = A.to_scipy().tocsr()
+    S.sort_indices()
+    rng = np.random.default_rng(seed)
+    ia, ja, a = [0], [], []
+    for i in range(S.shape[0]):
+        cols, vals = S.indices[S.indptr[i]:S.indptr[i + 1]], S.data[S.indptr[i]:S.indptr[i + 1]]
+        order = rng.permutation(len(cols))
+        ja += list(cols[order])
+        a += list(vals[order])
+        ia.append(len(ja))
+    ia, ja = np.array(ia, np.int32), np.array(ja, np.int32)
+    if one_based:
+        ia += 1
+        ja += 1
+    return ia, ja, np.array(a)
+
+
+def test_solver_interface_case_220():
+    """JXFPAMG_Solver_Interface arguments, solver 220: node-major 1-based CSR
+    in, BSR CPR-GMRES (no decoupling, one stage-2 sweep) on the GPU."""
+    from tests.conftest import fixture_gen_params, load_fixture
+    d = load_fixture("c1")
+    A, rhs = jx.generate(**fixture_gen_params(d))
+    ia, ja, a = _node_major_csr(A)
+    x = np.zeros(A.n)
+    print("[gpu] case=interface-220 c1", flush=True)
+    iters = jx.solver_interface(A.n, ia, ja, a, x, rhs.copy(), 1e-6, 220, A.b)
+    # in-place effects of the reference entry: 0-based, diagonal first
+    assert ia[0] == 0 and all(ja[ia[i]] == i for i in range(A.n))
+    true = np.linalg.norm(rhs - A.to_scipy() @ x) / np.linalg.norm(rhs)
+    assert true <= 1e-6
+    # same system through the BSR entry and through the oracle
+    Ap = jx.extract_pressure(A)
+    H = jx.amg_setup(*Ap)
+    s = jx.Solver(A, H, jx.default_params(tol=1e-6, max_iter=1000, stage2_maxit=1))
+    x2, st, _ = s.solve(rhs)
+    s.close()
+    assert st.iterations == iters and relerr(x, x2) < 1e-12
+    o = Oracle(A, H)
+    xo, sto, _, _ = o.gmres(rhs, o.params(tol=1e-6, max_iter=1000, stage2_maxit=1))
+    print(f"[gpu] interface-220: iters {iters} oracle {sto.iterations} true_rel {true:.3e}", flush=True)
+    assert abs(iters - sto.iterations) <= 2
+    with pytest.raises(RuntimeError):
+        jx.solver_interface(A.n, ia, ja, a, x, rhs.copy(), 1e-6, 22, A.b)

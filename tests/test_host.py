@@ -154,6 +154,20 @@ def test_amg_setup_bit_exact_vs_reference(name):
             assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2])
 
 
+def test_solver_interface_rejects_bad_arguments_on_the_host():
+    """Case-220 entry: unknown solver ids, block sizes and shapes fail before
+    any device work; a 1-based CSR is left 0-based only once it is accepted."""
+    A, rhs = jx.generate(nx=4, ny=4, nz=2, block_size=3, seed=1)
+    S = A.to_scipy().tocsr()
+    S.sort_indices()
+    ia, ja, a = (S.indptr + 1).astype(np.int32), (S.indices + 1).astype(np.int32), S.data.copy()
+    x = np.zeros(A.n)
+    for sid, nf in [(22, 3), (220, 1), (220, 5), (220, 5 if A.n % 5 else 7)]:
+        with pytest.raises(RuntimeError):
+            jx.solver_interface(A.n, ia, ja, a, x, rhs.copy(), 1e-6, sid, nf)
+    assert ia[0] == 1  # rejected before the in-place decrement
+
+
 def test_amg_setup_rejects_diagonal_not_first():
     ia = np.array([0, 2, 4], np.int32)
     ja = np.array([1, 0, 1, 0], np.int32)
