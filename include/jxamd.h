@@ -137,6 +137,12 @@ typedef struct {
      * relax_chunk = c > 0 uses ceil(n_l / c) chunks on level l instead.
      * Both 0/1: the one-thread sequential sweeps. */
     int relax_threads, relax_chunk;
+    /* Stage-1 (pressure AMG V-cycle) precision; no reference counterpart in
+     * the FP64 library (the reference's mixed driver, src/test_bsr_cprgmres_hybrid.c:217-439,
+     * runs stage 1 in FP32): 0 FP64; 1 FP32 level matrices, vectors and
+     * accumulation; 2 FP16 level matrices with a power-of-two scale per row,
+     * FP32 vectors and accumulation.  Stage 2 and GMRES stay FP64. */
+    int precond_precision;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread

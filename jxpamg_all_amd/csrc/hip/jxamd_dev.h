@@ -27,13 +27,19 @@ int jxamd_last_error(void);
 
 int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
                      const double *rhs, double *y, int mode, double alpha, double beta, void *stream);
-int jxamd_k_csr_spmv(int n, const int *sptr, const int *col, const double *val, const double *x, int xs,
-                     const double *rhs, int rs, double *y, int mode, void *stream);
+/* storage / vector type codes of the V-cycle (matrix values vt, vectors xt):
+ * (F64, F64), (F32, F32), (F16 with a float power-of-two scale per stored
+ * row, F32) */
+#define JXAMD_T_F64 0
+#define JXAMD_T_F32 1
+#define JXAMD_T_F16 2
+int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
+                     const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream);
 int jxamd_sweep_waves(int nsl);
-int jxamd_k_gs_sweep(int nsl, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
-                     const double *f, int fs, const double *uo, double *un, const int *dep_ptr, const int *dep_list,
-                     int *sflag, int epoch, int dir, unsigned long long *ticket, unsigned long long *base, int *err,
-                     void *stream);
+int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
+                     const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                     const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
+                     unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
 int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,
                        const double *val, const double *dinv, const double *rhs, const double *x0, const double *xp,
                        int pidx, double *xs, long long xstride, double *xout, const int *dlo_ptr,
@@ -44,11 +50,16 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
  * of the off-diagonals in natural row order, diagonal in diag[i] */
 #define JXAMD_CHUNK_LDS_MAX 16
 int jxamd_gs_chunk_lds_rows(int n, int nchunks); /* longest chunk */
-int jxamd_k_gs_chunk_lds(int n, int nchunks, const int *sptr, const int *col, const double *val, const double *diag,
-                         const double *f, int fs, const double *uo, double *un, void *stream);
-int jxamd_k_gs_chunk(int n, int nchunks, const int *wstep, const int *sptr, const int *col, const double *val,
-                     const double *diag, const double *f, int fs, const double *uo, double *un, void *stream);
-int jxamd_k_dense_gemv(int n, const double *M, const double *x, double *y, void *stream);
+int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, const int *col, const void *val,
+                         const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                         void *stream);
+int jxamd_k_gs_chunk(int vt, int xt, int n, int nchunks, const int *wstep, const int *sptr, const int *col,
+                     const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
+                     void *un, void *stream);
+int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream);
+int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream);
+int jxamd_k_widen(int n, const float *x, double *y, void *stream);
+int jxamd_k_fill_f32(int n, float *y, float v, void *stream);
 int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream);
 int jxamd_k_prolong(int nb, int b, int pidx, const double *xp, double *x, void *stream);
 int jxamd_k_fill(size_t n, double *y, double v, void *stream);

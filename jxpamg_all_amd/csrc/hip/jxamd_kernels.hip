@@ -15,6 +15,7 @@
 // Sync-free sweeps (k_gs_sweep, k_bhgs_sweep) run persistent wavefronts over
 // level-pure slices with per-slice completion flags; every wait is bounded
 // and an overflow is recorded in err[0] (see the comment at the kernels).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -44,6 +45,23 @@ __device__ __forceinline__ void st_coherent(double *p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ bool flag_ready(int f, int epoch) { return (int)(f - epoch) >= 0; }
+__device__ __forceinline__ float ld_coherent(const float *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(float *p, float v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Level matrices of the V-cycle are stored as V (double, float, or __half
+// with a power-of-two scale per stored row: a_ij = rsc[p] * h_ij); vectors
+// and accumulation are X (double with double, float otherwise).
+__device__ __forceinline__ double cvt(double v) { return v; }
+__device__ __forceinline__ float cvt(float v) { return v; }
+__device__ __forceinline__ float cvt(__half v) { return __half2float(v); }
+template <typename V> struct row_scaled { static constexpr bool value = false; };
+template <> struct row_scaled<__half> { static constexpr bool value = true; };
 
 // XCD-aware tile order for streaming kernels: consecutive tiles land on the
 // same XCD (dispatch is round-robin over the 8 XCDs), so the x entries a
@@ -109,21 +127,24 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
 // x and rhs may be strided views (xs, rs) so the pressure block of a BSR
 // vector is read in place.
 // ---------------------------------------------------------------------------
+template <typename V, typename X>
 __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__ sptr, const int *__restrict__ col,
-                                                  const double *__restrict__ val, const double *__restrict__ x, int xs,
-                                                  const double *__restrict__ rhs, int rs, double *__restrict__ y, int mode)
+                                                  const V *__restrict__ val, const float *__restrict__ rsc,
+                                                  const X *__restrict__ x, int xs, const X *__restrict__ rhs, int rs,
+                                                  X *__restrict__ y, int mode)
 {
     const int s = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     const int nsl = (n + WAVE - 1) / WAVE;
     if (s >= nsl) return;
     const int row = s * WAVE + lane;
-    double acc = 0.0;
+    X acc = 0;
     for (int k = sptr[s]; k < sptr[s + 1]; k++) {
         const size_t e = (size_t)k * WAVE + lane;
-        acc += val[e] * x[(size_t)col[e] * xs];
+        acc += (X)cvt(val[e]) * x[(size_t)col[e] * xs];
     }
     if (row >= n) return;
+    if (row_scaled<V>::value) acc *= (X)rsc[row];
     if (mode == 0) y[row] = acc;
     else if (mode == 1) y[row] = rhs[(size_t)row * rs] - acc;
     else y[row] += acc;
@@ -174,12 +195,14 @@ __device__ __forceinline__ void wait_slices(const int *__restrict__ dep_ptr, con
 //   dir > 0: forward (dependencies j < i), dir < 0: backward (j > i).
 //   Old values from uo (zeros if uo == nullptr), new values into un.
 //   f may be strided (fs).  A row with a zero diagonal keeps its old value.
+template <typename V, typename X>
 __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict__ perm, const int *__restrict__ sptr,
-                                                  const int *__restrict__ col, const double *__restrict__ val,
-                                                  const double *__restrict__ diag, const double *__restrict__ f, int fs,
-                                                  const double *uo, double *un, const int *__restrict__ dep_ptr,
-                                                  const int *__restrict__ dep_list, int *sflag, int epoch, int dir,
-                                                  unsigned long long *ticket, unsigned long long base, int *err)
+                                                  const int *__restrict__ col, const V *__restrict__ val,
+                                                  const float *__restrict__ rsc, const X *__restrict__ diag,
+                                                  const X *__restrict__ f, int fs, const X *uo, X *un,
+                                                  const int *__restrict__ dep_ptr, const int *__restrict__ dep_list,
+                                                  int *sflag, int epoch, int dir, unsigned long long *ticket,
+                                                  unsigned long long base, int *err)
 {
     const int lane = threadIdx.x % WAVE;
     for (;;) {
@@ -188,17 +211,18 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
         const int p = s * WAVE + lane;
         const int i = perm[p];
         const int k0 = sptr[s], k1 = sptr[s + 1];
-        double res = 0.0, d = 1.0, uold_i = 0.0;
+        X res = 0, d = 1, uold_i = 0, sc = 1;
         if (i >= 0) {
             d = diag[p];
             res = f[(size_t)i * fs];
             if (uo) uold_i = uo[i];
+            if (row_scaled<V>::value) sc = (X)rsc[p];
             // terms on old values need no wait
             for (int k = k0; k < k1; k++) {
                 const size_t e = (size_t)k * WAVE + lane;
                 const int j = col[e];
                 const bool dep = dir > 0 ? (j < i) : (j > i);
-                if (!dep && j != i && uo) res -= val[e] * uo[j];
+                if (!dep && j != i && uo) res -= sc * (X)cvt(val[e]) * uo[j];
             }
         }
         wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
@@ -207,9 +231,9 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
                 const size_t e = (size_t)k * WAVE + lane;
                 const int j = col[e];
                 const bool dep = dir > 0 ? (j < i) : (j > i);
-                if (dep) res -= val[e] * ld_coherent(un + j);
+                if (dep) res -= sc * (X)cvt(val[e]) * ld_coherent(un + j);
             }
-            st_coherent(un + i, d != 0.0 ? res / d : uold_i);
+            st_coherent(un + i, d != (X)0 ? res / d : uold_i);
         }
         st_flag_release(sflag + s, epoch);
     }
@@ -334,11 +358,12 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
 // the t-th row of each of the wave's 64 chunks (SELL-64, off-diagonals, the
 // diagonal in diag[p]).  No synchronisation between lanes.
 // ---------------------------------------------------------------------------
+template <typename V, typename X>
 __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int *__restrict__ wstep,
                                                   const int *__restrict__ sptr, const int *__restrict__ col,
-                                                  const double *__restrict__ val, const double *__restrict__ diag,
-                                                  const double *__restrict__ f, int fs, const double *__restrict__ uo,
-                                                  double *__restrict__ un)
+                                                  const V *__restrict__ val, const float *__restrict__ rsc,
+                                                  const X *__restrict__ diag, const X *__restrict__ f, int fs,
+                                                  const X *__restrict__ uo, X *__restrict__ un)
 {
     const int w = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
@@ -357,35 +382,39 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
         if (t >= len) continue;
         const int i = ns + t;
         const int s = s0 + t;
-        const double d = diag[(size_t)s * WAVE + lane];
-        double res = f[(size_t)i * fs];
+        const size_t p = (size_t)s * WAVE + lane;
+        const X d = diag[p];
+        const X sc = row_scaled<V>::value ? (X)rsc[p] : (X)1;
+        X res = f[(size_t)i * fs];
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const size_t e = (size_t)k * WAVE + lane;
             const int j = col[e];
             if (j == i) continue; // padding slot
-            double xj;
+            X xj;
             if (j >= ns && j < i) xj = un[j];
-            else xj = uo ? uo[j] : 0.0;
-            res -= val[e] * xj;
+            else xj = uo ? uo[j] : (X)0;
+            res -= sc * (X)cvt(val[e]) * xj;
         }
-        un[i] = d != 0.0 ? res / d : (uo ? uo[i] : 0.0);
+        un[i] = d != (X)0 ? res / d : (uo ? uo[i] : (X)0);
     }
     // backward sweep
     for (int t = nsteps - 1; t >= 0; t--) {
         if (t >= len) continue;
         const int i = ns + t;
         const int s = s0 + t;
-        const double d = diag[(size_t)s * WAVE + lane];
-        if (d == 0.0) continue;
-        double res = f[(size_t)i * fs];
+        const size_t p = (size_t)s * WAVE + lane;
+        const X d = diag[p];
+        if (d == (X)0) continue;
+        const X sc = row_scaled<V>::value ? (X)rsc[p] : (X)1;
+        X res = f[(size_t)i * fs];
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const size_t e = (size_t)k * WAVE + lane;
             const int j = col[e];
             if (j == i) continue;
-            double xj;
+            X xj;
             if (j >= ns && j < ne) xj = un[j];
-            else xj = uo ? uo[j] : 0.0;
-            res -= val[e] * xj;
+            else xj = uo ? uo[j] : (X)0;
+            res -= sc * (X)cvt(val[e]) * xj;
         }
         un[i] = res / d;
     }
@@ -407,15 +436,17 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
 // Block b owns chunks [b*cpb, (b+1)*cpb), at most 256 rows.  Rows with a zero
 // diagonal keep their entry value.
 // ---------------------------------------------------------------------------
-template <int LMAX>
+template <int LMAX, typename V, typename X>
 __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cpb, const int *__restrict__ sptr,
-                                                      const int *__restrict__ col, const double *__restrict__ val,
-                                                      const double *__restrict__ diag, const double *__restrict__ f,
-                                                      int fs, const double *__restrict__ uo, double *__restrict__ un)
+                                                      const int *__restrict__ col, const V *__restrict__ val,
+                                                      const float *__restrict__ rsc, const X *__restrict__ diag,
+                                                      const X *__restrict__ f, int fs, const X *__restrict__ uo,
+                                                      X *__restrict__ un)
 {
     constexpr int BST = LMAX * LMAX + 1; // odd stride between chunk blocks: spreads LDS banks
-    extern __shared__ double sm[];
-    double *g = sm, *ux = sm + 256, *dd = sm + 512, *B = sm + 768;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    X *sm = reinterpret_cast<X *>(smraw);
+    X *g = sm, *ux = sm + 256, *dd = sm + 512, *B = sm + 768;
     const int size = n / nchunks, rest = n - size * nchunks;
     const int c0 = blockIdx.x * cpb;
     const int c1 = min(c0 + cpb, nchunks);
@@ -428,14 +459,15 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         const int c = i < big ? i / (size + 1) : rest + (i - big) / size;
         const int ns = c < rest ? c * (size + 1) : c * size + rest;
         const int len = c < rest ? size + 1 : size;
-        double *Brow = B + (size_t)(c - c0) * BST + (i - ns) * LMAX;
-        for (int k = 0; k < len; k++) Brow[k] = 0.0;
-        double acc = f[(size_t)i * fs];
+        X *Brow = B + (size_t)(c - c0) * BST + (i - ns) * LMAX;
+        for (int k = 0; k < len; k++) Brow[k] = 0;
+        X acc = f[(size_t)i * fs];
+        const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
         const int s = i / WAVE, lane = i % WAVE;
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const size_t e = (size_t)k * WAVE + lane;
             const int j = col[e];
-            const double v = val[e];
+            const X v = sc * (X)cvt(val[e]);
             if (j >= ns && j < ns + len) {
                 if (j != i) Brow[j - ns] += v;
             } else if (uo) {
@@ -444,7 +476,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         }
         dd[r] = diag[i];
         g[r] = acc;
-        ux[r] = uo ? uo[i] : 0.0;
+        ux[r] = uo ? uo[i] : (X)0;
     }
     __syncthreads();
     const int c = c0 + (int)threadIdx.x;
@@ -452,19 +484,19 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         const int ns = c < rest ? c * (size + 1) : c * size + rest;
         const int len = c < rest ? size + 1 : size;
         const int rl = ns - row0;
-        const double *Bq = B + (size_t)(c - c0) * BST;
-        double u[LMAX];
+        const X *Bq = B + (size_t)(c - c0) * BST;
+        X u[LMAX];
 #pragma unroll
-        for (int t = 0; t < LMAX; t++) u[t] = t < len ? ux[rl + t] : 0.0;
+        for (int t = 0; t < LMAX; t++) u[t] = t < len ? ux[rl + t] : (X)0;
 #pragma unroll
         for (int pass = 0; pass < 2; pass++) {
 #pragma unroll
             for (int tt = 0; tt < LMAX; tt++) {
                 const int t = pass == 0 ? tt : LMAX - 1 - tt;
                 if (t < len) {
-                    const double d = dd[rl + t];
-                    if (d != 0.0) {
-                        double x = g[rl + t];
+                    const X d = dd[rl + t];
+                    if (d != (X)0) {
+                        X x = g[rl + t];
 #pragma unroll
                         for (int q = 0; q < LMAX; q++)
                             if (q != t && q < len) x -= Bq[t * LMAX + q] * u[q];
@@ -482,20 +514,39 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
 }
 
 // dense y = M x (row-major n x n), one wavefront per row
-__global__ void __launch_bounds__(256) k_dense_gemv(int n, const double *__restrict__ M, const double *__restrict__ x,
-                                                    double *__restrict__ y)
+template <typename X>
+__global__ void __launch_bounds__(256) k_dense_gemv(int n, const X *__restrict__ M, const X *__restrict__ x,
+                                                    X *__restrict__ y)
 {
     const int row = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (row >= n) return;
-    double acc = 0.0;
-    for (int c = lane; c < n; c += WAVE) acc += M[(size_t)row * n + c] * x[c];
+    X acc = 0;
+    for (int j = lane; j < n; j += WAVE) acc += M[(size_t)row * n + j] * x[j];
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) y[row] = acc;
 }
 
-// y[i] = x[i*stride + off]
+// y[i] = (float) x[i*stride + off]: restriction of an FP64 residual into an FP32 level vector
+__global__ void k_gather_cast(int n, const double *__restrict__ x, int stride, int off, float *__restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)x[(size_t)i * stride + off];
+}
+
+__global__ void k_widen(int n, const float *__restrict__ x, double *__restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (double)x[i];
+}
+
+__global__ void k_fill_f32(int n, float *__restrict__ y, float v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
 __global__ void k_gather_stride(int n, const double *__restrict__ x, int stride, int off, double *__restrict__ y)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -692,13 +743,24 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
     return chk(hipGetLastError());
 }
 
-int jxamd_k_csr_spmv(int n, const int *sptr, const int *col, const double *val, const double *x, int xs,
-                     const double *rhs, int rs, double *y, int mode, void *stream)
+// (vt, xt) dispatch: F64/F64, F32/F32, F16/F32
+#define DISPATCH_VX(vt, xt, CALL)                                                                                      \
+    do {                                                                                                               \
+        if ((vt) == JXAMD_T_F64 && (xt) == JXAMD_T_F64) { typedef double V_; typedef double X_; CALL; }                \
+        else if ((vt) == JXAMD_T_F32 && (xt) == JXAMD_T_F32) { typedef float V_; typedef float X_; CALL; }             \
+        else if ((vt) == JXAMD_T_F16 && (xt) == JXAMD_T_F32) { typedef __half V_; typedef float X_; CALL; }            \
+        else return JXAMD_ERROR_ARG;                                                                                   \
+    } while (0)
+
+int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
+                     const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream)
 {
     const int nsl = (n + WAVE - 1) / WAVE;
     const int grid = (nsl + 3) / 4;
     if (grid == 0) return 0;
-    k_csr_spmv<<<grid, 256, 0, (hipStream_t)stream>>>(n, sptr, col, val, x, xs, rhs, rs, y, mode);
+    DISPATCH_VX(vt, xt,
+                (k_csr_spmv<V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                    n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y, mode)));
     return chk(hipGetLastError());
 }
 
@@ -708,16 +770,18 @@ int jxamd_sweep_waves(int nsl)
     return nsl < cap ? (nsl > 0 ? nsl : 1) : cap;
 }
 
-int jxamd_k_gs_sweep(int nsl, const int *perm, const int *sptr, const int *col, const double *val, const double *diag,
-                     const double *f, int fs, const double *uo, double *un, const int *dep_ptr, const int *dep_list,
-                     int *sflag, int epoch, int dir, unsigned long long *ticket, unsigned long long *base, int *err,
-                     void *stream)
+int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
+                     const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                     const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
+                     unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
 {
     if (nsl == 0) return 0;
     const int waves = jxamd_sweep_waves(nsl);
     const int grid = (waves + 3) / 4;
-    k_gs_sweep<<<grid, 256, 0, (hipStream_t)stream>>>(nsl, perm, sptr, col, val, diag, f, fs, uo, un, dep_ptr,
-                                                      dep_list, sflag, epoch, dir, ticket, *base, err);
+    DISPATCH_VX(vt, xt,
+                (k_gs_sweep<V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                    nsl, perm, sptr, col, (const V_ *)val, rsc, (const X_ *)diag, (const X_ *)f, fs, (const X_ *)uo,
+                    (X_ *)un, dep_ptr, dep_list, sflag, epoch, dir, ticket, *base, err)));
     *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
     return chk(hipGetLastError());
 }
@@ -752,13 +816,16 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
     return chk(hipGetLastError());
 }
 
-int jxamd_k_gs_chunk(int n, int nchunks, const int *wstep, const int *sptr, const int *col, const double *val,
-                     const double *diag, const double *f, int fs, const double *uo, double *un, void *stream)
+int jxamd_k_gs_chunk(int vt, int xt, int n, int nchunks, const int *wstep, const int *sptr, const int *col,
+                     const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
+                     void *un, void *stream)
 {
     if (n == 0 || nchunks < 1) return 0;
     const int waves = (nchunks + WAVE - 1) / WAVE;
-    k_gs_chunk<<<(waves + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, nchunks, wstep, sptr, col, val, diag, f, fs, uo,
-                                                                  un);
+    DISPATCH_VX(vt, xt,
+                (k_gs_chunk<V_, X_><<<(waves + 3) / 4, 256, 0, (hipStream_t)stream>>>(
+                    n, nchunks, wstep, sptr, col, (const V_ *)val, rsc, (const X_ *)diag, (const X_ *)f, fs,
+                    (const X_ *)uo, (X_ *)un)));
     return chk(hipGetLastError());
 }
 
@@ -769,8 +836,9 @@ int jxamd_gs_chunk_lds_rows(int n, int nchunks)
     return size + (n % nchunks ? 1 : 0);
 }
 
-int jxamd_k_gs_chunk_lds(int n, int nchunks, const int *sptr, const int *col, const double *val, const double *diag,
-                         const double *f, int fs, const double *uo, double *un, void *stream)
+int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, const int *col, const void *val,
+                         const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                         void *stream)
 {
     if (n == 0) return 0;
     const int maxlen = jxamd_gs_chunk_lds_rows(n, nchunks);
@@ -779,8 +847,10 @@ int jxamd_k_gs_chunk_lds(int n, int nchunks, const int *sptr, const int *col, co
     const int grid = (nchunks + cpb - 1) / cpb;
     hipStream_t st = (hipStream_t)stream;
 #define LAUNCH_LDS(LM)                                                                                                 \
-    k_gs_chunk_lds<LM><<<grid, 256, sizeof(double) * (768 + (size_t)cpb * (LM * LM + 1)), st>>>(                     \
-        n, nchunks, cpb, sptr, col, val, diag, f, fs, uo, un)
+    DISPATCH_VX(vt, xt,                                                                                                \
+                (k_gs_chunk_lds<LM, V_, X_><<<grid, 256, sizeof(X_) * (768 + (size_t)cpb * (LM * LM + 1)), st>>>(    \
+                    n, nchunks, cpb, sptr, col, (const V_ *)val, rsc, (const X_ *)diag, (const X_ *)f, fs,            \
+                    (const X_ *)uo, (X_ *)un)))
     if (maxlen <= 4) LAUNCH_LDS(4);
     else if (maxlen <= 8) LAUNCH_LDS(8);
     else LAUNCH_LDS(16);
@@ -788,10 +858,37 @@ int jxamd_k_gs_chunk_lds(int n, int nchunks, const int *sptr, const int *col, co
     return chk(hipGetLastError());
 }
 
-int jxamd_k_dense_gemv(int n, const double *M, const double *x, double *y, void *stream)
+int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream)
 {
     if (n == 0) return 0;
-    k_dense_gemv<<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, M, x, y);
+    if (xt == JXAMD_T_F64)
+        k_dense_gemv<double><<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, (const double *)M, (const double *)x,
+                                                                            (double *)y);
+    else if (xt == JXAMD_T_F32)
+        k_dense_gemv<float><<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, (const float *)M, (const float *)x,
+                                                                           (float *)y);
+    else return JXAMD_ERROR_ARG;
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream)
+{
+    if (n == 0) return 0;
+    k_gather_cast<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, x, stride, off, y);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_widen(int n, const float *x, double *y, void *stream)
+{
+    if (n == 0) return 0;
+    k_widen<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, x, y);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_fill_f32(int n, float *y, float v, void *stream)
+{
+    if (n == 0) return 0;
+    k_fill_f32<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, y, v);
     return chk(hipGetLastError());
 }
 

@@ -37,7 +37,9 @@ typedef struct {
     int n, ncols, nsl;
     int *sptr, *col, *perm, *flag;
     int *dep_ptr, *dep_list; /* sweep layouts: dependency slices per slice */
-    double *val, *diag;
+    void *val, *diag;        /* val: storage type vt; diag: vector type xt */
+    float *rsc;              /* vt == F16: power-of-two scale per position */
+    int vt, xt;
     double abytes;           /* algorithmic HBM bytes of one launch over it */
 } dsell;
 
@@ -78,9 +80,12 @@ struct jxamd_solver_s {
     int *wstep[JXAMD_MAX_LEVELS];
     int nchunks[JXAMD_MAX_LEVELS]; /* 0: sequential sweeps on this level */
     dsell R[JXAMD_MAX_LEVELS], P[JXAMD_MAX_LEVELS];
-    double *cinv;
+    void *cinv;            /* coarsest inverse, type xt */
     int nc;
-    double *F[JXAMD_MAX_LEVELS], *U[JXAMD_MAX_LEVELS], *T[JXAMD_MAX_LEVELS], *Vt[JXAMD_MAX_LEVELS];
+    int vt, xt;            /* V-cycle storage / vector types (JXAMD_T_*) */
+    size_t xsz;            /* bytes per V-cycle vector entry */
+    void *F[JXAMD_MAX_LEVELS], *U[JXAMD_MAX_LEVELS], *T[JXAMD_MAX_LEVELS], *Vt[JXAMD_MAX_LEVELS];
+    double *up64;          /* FP64 copy of the stage-1 result when xt != F64 */
     double *xtmp;          /* stage-2 ping-pong partner, N */
     double *V;             /* (k_dim+1) Krylov vectors, N each */
     double *Rv, *Wv, *Bd, *Xd; /* GMRES work, host-solve staging */
@@ -432,26 +437,117 @@ static int zero_ints(int **dst, size_t n, void *st)
 
 /* sched != NULL: a sweep layout; uploads its dependency lists and a zeroed
  * per-slice flag array */
-static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsched *sched)
+/* IEEE binary16 bits of f, round to nearest even (values here are scaled
+ * into [-1, 1], so overflow cannot occur; tiny values become subnormal or 0) */
+static uint16_t half_bits(float f)
+{
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* rounds past 65504 */
+    if (ax < 0x38800000u) { /* below 2^-14: subnormal half */
+        if (ax < 0x33000000u) return (uint16_t)sign; /* below 2^-25 */
+        const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+        const int shift = 126 - (int)(ax >> 23); /* 14..24 */
+        uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (q & 1u))) q++;
+        return (uint16_t)(sign | q);
+    }
+    uint32_t h = ((ax >> 13) - (112u << 10));
+    const uint32_t rem = ax & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+}
+
+static size_t type_size(int t) { return t == JXAMD_T_F64 ? 8 : (t == JXAMD_T_F32 ? 4 : 2); }
+
+/* algorithmic bytes of one pass over a level matrix: values (+ F16 row
+ * scales), indices, row pointer, plus nvec vector entries per row */
+static double level_bytes(const dsell *d, long nnz, int n, double nvec)
+{
+    double b = (double)nnz * (type_size(d->vt) + 4.0) + 4.0 * (n + 1) + nvec * type_size(d->xt) * n;
+    if (d->vt == JXAMD_T_F16) b += 4.0 * n;
+    return b;
+}
+
+/* Upload a host SELL layout, values converted to storage type vt and the
+ * diagonal to vector type xt. */
+static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsched *sched, int vt, int xt)
 {
     memset(d, 0, sizeof(*d));
     d->n = h->n;
     d->ncols = ncols;
     d->nsl = h->nsl;
+    d->vt = vt;
+    d->xt = xt;
     size_t slots = (size_t)(h->nsl ? h->sptr[h->nsl] : 0) * SELLW;
     CK(jxamd_dev_malloc((void **)&d->sptr, sizeof(int) * (h->nsl + 1)));
     CK(jxamd_dev_malloc((void **)&d->col, sizeof(int) * (slots ? slots : 1)));
-    CK(jxamd_dev_malloc((void **)&d->val, sizeof(double) * (slots ? slots : 1)));
+    CK(jxamd_dev_malloc(&d->val, type_size(vt) * (slots ? slots : 1)));
     CK(jxamd_h2d(d->sptr, h->sptr, sizeof(int) * (h->nsl + 1), st));
     CK(jxamd_h2d(d->col, h->col, sizeof(int) * slots, st));
-    CK(jxamd_h2d(d->val, h->val, sizeof(double) * slots, st));
+    if (vt == JXAMD_T_F64) {
+        CK(jxamd_h2d(d->val, h->val, sizeof(double) * slots, st));
+    } else {
+        void *tmp = malloc(type_size(vt) * (slots ? slots : 1));
+        float *sc = NULL;
+        if (!tmp) return JXAMD_ERROR_MEMORY;
+        if (vt == JXAMD_T_F32) {
+            for (size_t e = 0; e < slots; e++) ((float *)tmp)[e] = (float)h->val[e];
+        } else {
+            const size_t npos = (size_t)h->nsl * SELLW;
+            sc = (float *)malloc(sizeof(float) * (npos ? npos : 1));
+            if (!sc) { free(tmp); return JXAMD_ERROR_MEMORY; }
+            for (int q = 0; q < h->nsl; q++)
+                for (int l = 0; l < SELLW; l++) {
+                    double mx = 0.0;
+                    for (int k = h->sptr[q]; k < h->sptr[q + 1]; k++) {
+                        double v = fabs(h->val[(size_t)k * SELLW + l]);
+                        if (v > mx) mx = v;
+                    }
+                    /* smallest power of two >= the row's largest magnitude */
+                    int ex = 0;
+                    if (mx > 0.0) {
+                        double m = frexp(mx, &ex); /* mx = m * 2^ex, m in [0.5, 1) */
+                        if (m == 0.5) ex -= 1;
+                    }
+                    const float scale = mx > 0.0 ? (float)ldexp(1.0, ex) : 1.0f;
+                    sc[(size_t)q * SELLW + l] = scale;
+                    for (int k = h->sptr[q]; k < h->sptr[q + 1]; k++) {
+                        size_t e = (size_t)k * SELLW + l;
+                        ((uint16_t *)tmp)[e] = half_bits((float)(h->val[e] / scale));
+                    }
+                }
+            int rc = jxamd_dev_malloc((void **)&d->rsc, sizeof(float) * (npos ? npos : 1));
+            if (!rc) rc = jxamd_h2d(d->rsc, sc, sizeof(float) * npos, st);
+            if (rc) { free(tmp); free(sc); return rc; }
+        }
+        int rc = jxamd_h2d(d->val, tmp, type_size(vt) * slots, st);
+        if (!rc) rc = jxamd_stream_sync(st);
+        free(tmp);
+        free(sc);
+        CK(rc);
+    }
     if (h->perm) {
         CK(jxamd_dev_malloc((void **)&d->perm, sizeof(int) * (h->n ? h->n : 1)));
         CK(jxamd_h2d(d->perm, h->perm, sizeof(int) * h->n, st));
     }
     if (h->diag) {
-        CK(jxamd_dev_malloc((void **)&d->diag, sizeof(double) * (h->n ? h->n : 1)));
-        CK(jxamd_h2d(d->diag, h->diag, sizeof(double) * h->n, st));
+        CK(jxamd_dev_malloc(&d->diag, type_size(xt) * (h->n ? h->n : 1)));
+        if (xt == JXAMD_T_F64) {
+            CK(jxamd_h2d(d->diag, h->diag, sizeof(double) * h->n, st));
+        } else {
+            float *t = (float *)malloc(sizeof(float) * (h->n ? h->n : 1));
+            if (!t) return JXAMD_ERROR_MEMORY;
+            for (int p = 0; p < h->n; p++) t[p] = (float)h->diag[p];
+            int rc = jxamd_h2d(d->diag, t, sizeof(float) * h->n, st);
+            if (!rc) rc = jxamd_stream_sync(st);
+            free(t);
+            CK(rc);
+        }
     }
     if (sched) {
         CK(upload_ints(&d->dep_ptr, sched->dep_ptr, (size_t)sched->nsl + 1, st));
@@ -465,7 +561,7 @@ static void dsell_free(dsell *d)
 {
     jxamd_dev_free(d->sptr); jxamd_dev_free(d->col); jxamd_dev_free(d->val);
     jxamd_dev_free(d->perm); jxamd_dev_free(d->diag); jxamd_dev_free(d->flag);
-    jxamd_dev_free(d->dep_ptr); jxamd_dev_free(d->dep_list);
+    jxamd_dev_free(d->dep_ptr); jxamd_dev_free(d->dep_list); jxamd_dev_free(d->rsc);
     memset(d, 0, sizeof(*d));
 }
 
@@ -690,12 +786,12 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     int e;
     /* natural order, full rows: residual F - A U */
     e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 0, 0);
-    if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL);
+    if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL, s->vt, s->xt);
     if (!e) e = jxamd_stream_sync(s->stream);
     hsell_free(&h);
     CK(e);
     /* CSR SpMV bytes: nnz (value + index) + row pointer + x + rhs + y */
-    s->Ares[l].abytes = 12.0 * A->ia[n] + 4.0 * (n + 1) + 24.0 * n;
+    s->Ares[l].abytes = level_bytes(&s->Ares[l], A->ia[n], n, 3.0);
     int T = 0;
     if (s->prm.relax_threads > 1) T = s->prm.relax_threads;
     else if (s->prm.relax_chunk > 0) T = (n + s->prm.relax_chunk - 1) / s->prm.relax_chunk;
@@ -703,14 +799,14 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     if (T > 1 && (l < s->L - 1 || s->L == 1) && jxamd_gs_chunk_lds_rows(n, T) <= JXAMD_CHUNK_LDS_MAX) {
         /* short chunks: one pass, off-diagonals in natural row order */
         e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 1, 1);
-        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL);
+        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt);
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         CK(e);
         s->nchunks[l] = T;
         s->wstep[l] = NULL;
         /* matrix once, f + entry values + results */
-        s->Gc[l].abytes = 12.0 * A->ia[n] + 4.0 * (n + 1) + 24.0 * n;
+        s->Gc[l].abytes = level_bytes(&s->Gc[l], A->ia[n], n, 3.0);
     } else if (T > 1 && (l < s->L - 1 || s->L == 1)) {
         /* long chunks: wave w holds chunks 64w..64w+63; slice wstep[w]+t
          * holds row ns(c)+t of each chunk c of the wave */
@@ -735,7 +831,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
                     perm[(size_t)(wstep[w] + t) * SELLW + ln] = row;
                 }
         e = hsell_build(&h, npos, A->ia, A->ja, A->a, perm, 1, 1);
-        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL);
+        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt);
         if (!e) e = upload_ints(&s->wstep[l], wstep, (size_t)waves + 1, s->stream);
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
@@ -743,7 +839,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         CK(e);
         s->nchunks[l] = T;
         /* forward + backward pass over the matrix, old + new + rhs vectors */
-        s->Gc[l].abytes = 2.0 * (12.0 * A->ia[n] + 4.0 * (n + 1)) + 24.0 * n;
+        s->Gc[l].abytes = 2.0 * level_bytes(&s->Gc[l], A->ia[n], n, 0.0) + 3.0 * type_size(s->xt) * n;
     } else if (l < s->L - 1 || s->L == 1) {
         /* forward and backward sweep orders, off-diagonal entries */
         for (int dir = 1; dir >= -1; dir -= 2) {
@@ -752,20 +848,21 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
             int *perm = sc.perm;
             sc.perm = NULL; /* owned by h below */
             e = hsell_build(&h, sc.npos, A->ia, A->ja, A->a, perm, 1, 1);
-            if (!e) e = dsell_upload(dir > 0 ? &s->Gf[l] : &s->Gb[l], &h, n, s->stream, &sc);
+            if (!e) e = dsell_upload(dir > 0 ? &s->Gf[l] : &s->Gb[l], &h, n, s->stream, &sc, s->vt, s->xt);
             hsell_free(&h);
             hsched_free(&sc);
             CK(e);
             /* sweep: matrix once + old x + new x + rhs */
-            (dir > 0 ? &s->Gf[l] : &s->Gb[l])->abytes = 12.0 * A->ia[n] + 4.0 * (n + 1) + 24.0 * n;
+            (dir > 0 ? &s->Gf[l] : &s->Gb[l])->abytes = level_bytes(dir > 0 ? &s->Gf[l] : &s->Gb[l], A->ia[n], n, 3.0);
         }
     }
     if (l < s->L - 1) {
         /* interpolation P (n x nc) and restriction R = P^T (nc x n) */
         const jxamd_csr *Pm = &H->P[l];
         e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL);
-        s->P[l].abytes = 12.0 * Pm->ia[Pm->n] + 4.0 * (Pm->n + 1) + 8.0 * Pm->ncols + 16.0 * Pm->n;
+        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL, s->vt, s->xt);
+        /* y += P x: x once, y read and written */
+        s->P[l].abytes = level_bytes(&s->P[l], Pm->ia[Pm->n], Pm->n, 2.0) + (double)type_size(s->xt) * Pm->ncols;
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         CK(e);
@@ -773,8 +870,8 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         double *ra;
         CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
         e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL);
-        s->R[l].abytes = 12.0 * Pm->ia[Pm->n] + 4.0 * (Pm->ncols + 1) + 8.0 * Pm->n + 8.0 * Pm->ncols;
+        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL, s->vt, s->xt);
+        s->R[l].abytes = level_bytes(&s->R[l], Pm->ia[Pm->n], Pm->ncols, 1.0) + (double)type_size(s->xt) * Pm->n;
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         free(ri); free(rj); free(ra);
@@ -786,8 +883,12 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         for (int i = 0; i < n; i++)
             for (int k = A->ia[i]; k < A->ia[i + 1]; k++) M[(size_t)i * n + A->ja[k]] = A->a[k];
         e = dense_inverse(M, n);
-        if (!e) e = jxamd_dev_malloc((void **)&s->cinv, sizeof(double) * ((size_t)n * n + 1));
-        if (!e) e = jxamd_h2d(s->cinv, M, sizeof(double) * (size_t)n * n, s->stream);
+        if (!e && s->xt != JXAMD_T_F64) {
+            float *Mf = (float *)M; /* in place: element k of the float array lies before element k of the double one */
+            for (size_t k = 0; k < (size_t)n * n; k++) Mf[k] = (float)M[k];
+        }
+        if (!e) e = jxamd_dev_malloc(&s->cinv, s->xsz * ((size_t)n * n + 1));
+        if (!e) e = jxamd_h2d(s->cinv, M, s->xsz * (size_t)n * n, s->stream);
         if (!e) e = jxamd_stream_sync(s->stream);
         free(M);
         s->nc = n;
@@ -805,6 +906,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->k_dim < 1 || p->pressure_index < 0 || p->pressure_index >= A->b) return JXAMD_ERROR_ARG;
     if (p->stage1_maxit < 0 || p->stage2_maxit < 0) return JXAMD_ERROR_ARG;
     if (p->relax_threads < 0 || p->relax_chunk < 0) return JXAMD_ERROR_ARG;
+    if (p->precond_precision < 0 || p->precond_precision > 2) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -830,6 +932,9 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     s->b = A->b;
     s->N = (size_t)A->nb * A->b;
     s->L = H->num_levels;
+    s->vt = p->precond_precision == 2 ? JXAMD_T_F16 : (p->precond_precision == 1 ? JXAMD_T_F32 : JXAMD_T_F64);
+    s->xt = p->precond_precision == 0 ? JXAMD_T_F64 : JXAMD_T_F32;
+    s->xsz = type_size(s->xt);
     int e = jxamd_stream_create(&s->stream);
     if (!e) e = jxamd_event_create(&s->ev0);
     if (!e) e = jxamd_event_create(&s->ev1);
@@ -842,11 +947,12 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     }
     for (int l = 0; l < s->L && !e; l++) {
         size_t n = (size_t)s->nlev[l] + 1;
-        e = jxamd_dev_malloc((void **)&s->F[l], sizeof(double) * n);
-        if (!e) e = jxamd_dev_malloc((void **)&s->U[l], sizeof(double) * n);
-        if (!e) e = jxamd_dev_malloc((void **)&s->T[l], sizeof(double) * n);
-        if (!e) e = jxamd_dev_malloc((void **)&s->Vt[l], sizeof(double) * n);
+        e = jxamd_dev_malloc(&s->F[l], s->xsz * n);
+        if (!e) e = jxamd_dev_malloc(&s->U[l], s->xsz * n);
+        if (!e) e = jxamd_dev_malloc(&s->T[l], s->xsz * n);
+        if (!e) e = jxamd_dev_malloc(&s->Vt[l], s->xsz * n);
     }
+    if (!e && s->xt != JXAMD_T_F64) e = jxamd_dev_malloc((void **)&s->up64, sizeof(double) * ((size_t)s->nb + 1));
     const size_t N = s->N;
     if (!e) e = jxamd_dev_malloc((void **)&s->xtmp, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->V, sizeof(double) * N * (size_t)(p->k_dim + 1));
@@ -894,7 +1000,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
         dsell_free(&s->R[l]); dsell_free(&s->P[l]);
         jxamd_dev_free(s->F[l]); jxamd_dev_free(s->U[l]); jxamd_dev_free(s->T[l]); jxamd_dev_free(s->Vt[l]);
     }
-    jxamd_dev_free(s->cinv);
+    jxamd_dev_free(s->cinv); jxamd_dev_free(s->up64);
     jxamd_dev_free(s->xtmp); jxamd_dev_free(s->V); jxamd_dev_free(s->Rv); jxamd_dev_free(s->Wv);
     jxamd_dev_free(s->Bd); jxamd_dev_free(s->Xd);
     jxamd_dev_free(s->dsc); jxamd_dev_free(s->part); jxamd_dev_free(s->count);
@@ -923,40 +1029,41 @@ int jxamd_solver_error_flag(jxamd_solver *s, int *flag)
 
 /* one relax-6 call: forward sweep u -> T, backward sweep T -> u.
  * uo_zero: the incoming u is known to be zero (first visit of a level). */
-static int hsgs(jxamd_solver *s, int l, const double *f, int fs, double *u, int uo_zero)
+static int hsgs(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
     dsell *g = &s->Gf[l], *h = &s->Gb[l];
     s->epoch++;
     int pq = prof_begin(s);
-    CK(jxamd_k_gs_sweep(g->nsl, g->perm, g->sptr, g->col, g->val, g->diag, f, fs, uo_zero ? NULL : u, s->T[l],
-                        g->dep_ptr, g->dep_list, g->flag, s->epoch, 1, s->ticket, &s->tbase, s->err, s->stream));
+    CK(jxamd_k_gs_sweep(g->vt, g->xt, g->nsl, g->perm, g->sptr, g->col, g->val, g->rsc, g->diag, f, fs,
+                        uo_zero ? NULL : u, s->T[l], g->dep_ptr, g->dep_list, g->flag, s->epoch, 1, s->ticket,
+                        &s->tbase, s->err, s->stream));
     prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
     s->epoch++;
     pq = prof_begin(s);
-    CK(jxamd_k_gs_sweep(h->nsl, h->perm, h->sptr, h->col, h->val, h->diag, f, fs, s->T[l], u, h->dep_ptr,
-                        h->dep_list, h->flag, s->epoch, -1, s->ticket, &s->tbase, s->err, s->stream));
+    CK(jxamd_k_gs_sweep(h->vt, h->xt, h->nsl, h->perm, h->sptr, h->col, h->val, h->rsc, h->diag, f, fs, s->T[l], u,
+                        h->dep_ptr, h->dep_list, h->flag, s->epoch, -1, s->ticket, &s->tbase, s->err, s->stream));
     prof_end(s, pq, JXAMD_PROF_GS_SWEEP, h->abytes);
     return 0;
 }
 
 /* one threaded relax-6 call on level l: entry values uo (NULL: zeros) -> un */
-static int hsgs_chunk(jxamd_solver *s, int l, const double *f, int fs, const double *uo, double *un)
+static int hsgs_chunk(jxamd_solver *s, int l, const void *f, int fs, const void *uo, void *un)
 {
     dsell *g = &s->Gc[l];
     int pq = prof_begin(s);
     if (s->wstep[l])
-        CK(jxamd_k_gs_chunk(s->nlev[l], s->nchunks[l], s->wstep[l], g->sptr, g->col, g->val, g->diag, f, fs, uo, un,
-                            s->stream));
+        CK(jxamd_k_gs_chunk(g->vt, g->xt, s->nlev[l], s->nchunks[l], s->wstep[l], g->sptr, g->col, g->val, g->rsc,
+                            g->diag, f, fs, uo, un, s->stream));
     else
-        CK(jxamd_k_gs_chunk_lds(s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->diag, f, fs, uo, un,
-                                s->stream));
+        CK(jxamd_k_gs_chunk_lds(g->vt, g->xt, s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->rsc, g->diag, f,
+                                fs, uo, un, s->stream));
     prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
     return 0;
 }
 
 /* the two relax-6 calls of one level visit (relax_order 1, par_relax_if.c:36-60);
  * the result stays in u */
-static int relax_pair(jxamd_solver *s, int l, const double *f, int fs, double *u, int uo_zero)
+static int relax_pair(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
     if (s->nchunks[l] > 1) {
         CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
@@ -967,18 +1074,26 @@ static int relax_pair(jxamd_solver *s, int l, const double *f, int fs, double *u
 }
 
 /* a single relax-6 call; the result stays in u */
-static int relax_once(jxamd_solver *s, int l, const double *f, int fs, double *u, int uo_zero)
+static int relax_once(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
     if (s->nchunks[l] > 1) {
         CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
-        return jxamd_k_copy(s->nlev[l], s->T[l], u, s->stream);
+        return jxamd_d2d(u, s->T[l], s->xsz * s->nlev[l], s->stream);
     }
     return hsgs(s, l, f, fs, u, uo_zero);
 }
 
-/* V-cycle on F[0] (possibly a strided view) into U[0].  u0_zero: U[0] is zero
- * on entry (the CPR stage-1 start x_p := 0, jxf_cpr.c:448). */
-static int vcycle(jxamd_solver *s, const double *f0, int fs0, int u0_zero)
+static int csr_launch(jxamd_solver *s, const dsell *A, const void *x, const void *rhs, int rs, void *y, int mode)
+{
+    int pq = prof_begin(s);
+    CK(jxamd_k_csr_spmv(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode, s->stream));
+    prof_end(s, pq, JXAMD_PROF_CSR_SPMV, A->abytes);
+    return 0;
+}
+
+/* V-cycle on f0 (level-0 right-hand side of the V-cycle's vector type,
+ * possibly a strided view) into U[0]. */
+static int vcycle(jxamd_solver *s, const void *f0, int fs0, int u0_zero)
 {
     const int L = s->L;
     if (L == 1) {
@@ -986,29 +1101,20 @@ static int vcycle(jxamd_solver *s, const double *f0, int fs0, int u0_zero)
         return relax_once(s, 0, f0, fs0, s->U[0], u0_zero);
     }
     for (int l = 0; l < L - 1; l++) {
-        const double *f = l == 0 ? f0 : s->F[l];
+        const void *f = l == 0 ? f0 : s->F[l];
         const int fs = l == 0 ? fs0 : 1;
         const int zero = l == 0 ? u0_zero : 1;
         CK(relax_pair(s, l, f, fs, s->U[l], zero));
         /* Vtemp = F - A U; F_{l+1} = P^T Vtemp */
-        dsell *A = &s->Ares[l];
-        int pq = prof_begin(s);
-        CK(jxamd_k_csr_spmv(A->n, A->sptr, A->col, A->val, s->U[l], 1, f, fs, s->Vt[l], 1, s->stream));
-        prof_end(s, pq, JXAMD_PROF_CSR_SPMV, A->abytes);
-        dsell *R = &s->R[l];
-        pq = prof_begin(s);
-        CK(jxamd_k_csr_spmv(R->n, R->sptr, R->col, R->val, s->Vt[l], 1, NULL, 1, s->F[l + 1], 0, s->stream));
-        prof_end(s, pq, JXAMD_PROF_CSR_SPMV, R->abytes);
+        CK(csr_launch(s, &s->Ares[l], s->U[l], f, fs, s->Vt[l], 1));
+        CK(csr_launch(s, &s->R[l], s->Vt[l], NULL, 1, s->F[l + 1], 0));
     }
     int pc = prof_begin(s);
-    CK(jxamd_k_dense_gemv(s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
-    prof_end(s, pc, JXAMD_PROF_COARSE, 8.0 * s->nc * ((double)s->nc + 2.0));
+    CK(jxamd_k_dense_gemv(s->xt, s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
+    prof_end(s, pc, JXAMD_PROF_COARSE, (double)s->xsz * s->nc * ((double)s->nc + 2.0));
     for (int l = L - 2; l >= 0; l--) {
-        dsell *P = &s->P[l];
-        int pq = prof_begin(s);
-        CK(jxamd_k_csr_spmv(P->n, P->sptr, P->col, P->val, s->U[l + 1], 1, NULL, 1, s->U[l], 2, s->stream));
-        prof_end(s, pq, JXAMD_PROF_CSR_SPMV, P->abytes);
-        const double *f = l == 0 ? f0 : s->F[l];
+        CK(csr_launch(s, &s->P[l], s->U[l + 1], NULL, 1, s->U[l], 2));
+        const void *f = l == 0 ? f0 : s->F[l];
         const int fs = l == 0 ? fs0 : 1;
         CK(relax_pair(s, l, f, fs, s->U[l], 0));
     }
@@ -1035,18 +1141,31 @@ static int bhgs_run(jxamd_solver *s, int K, const double *rhs, const double *x0,
 static int cpr_apply(jxamd_solver *s, const double *r, double *x)
 {
     const int b = s->b, pi = s->prm.pressure_index;
+    /* stage 1: the restricted residual is read in place (stride b), or cast
+     * into the V-cycle's precision */
+    const void *f0 = r + pi;
+    int fs0 = b;
+    if (s->xt != JXAMD_T_F64 && s->prm.stage1_maxit > 0) {
+        CK(jxamd_k_gather_cast(s->nlev[0], r, b, pi, (float *)s->F[0], s->stream));
+        f0 = s->F[0];
+        fs0 = 1;
+    }
     int zero = 1;
-    /* stage 1: the restricted residual is read in place (stride b) */
     for (int it = 0; it < s->prm.stage1_maxit; it++) {
-        CK(vcycle(s, r + pi, b, zero));
+        CK(vcycle(s, f0, fs0, zero));
         zero = 0;
     }
-    if (s->prm.stage1_maxit == 0) CK(jxamd_k_fill(s->nlev[0], s->U[0], 0.0, s->stream));
+    double *xp = (double *)s->U[0];
+    if (s->xt != JXAMD_T_F64) {
+        xp = s->up64;
+        if (s->prm.stage1_maxit > 0) CK(jxamd_k_widen(s->nlev[0], (const float *)s->U[0], xp, s->stream));
+    }
+    if (s->prm.stage1_maxit == 0) CK(jxamd_k_fill(s->nlev[0], xp, 0.0, s->stream));
     /* prolongation (type 0) + stage-2 sweeps; the first sweep reads the
      * prolongated vector on the fly and the last one lands in x */
     const int m = s->prm.stage2_maxit;
-    if (m == 0) return jxamd_k_prolong(s->nb, b, pi, s->U[0], x, s->stream);
-    return bhgs_run(s, m, r, NULL, s->U[0], x, s->prm.stage2_weight, s->prm.stage2_omega);
+    if (m == 0) return jxamd_k_prolong(s->nb, b, pi, xp, x, s->stream);
+    return bhgs_run(s, m, r, NULL, xp, x, s->prm.stage2_weight, s->prm.stage2_omega);
 }
 
 int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x)
@@ -1092,26 +1211,48 @@ int jxamd_cpr_precond_timed(jxamd_solver *s, const double *d_r, double *d_x, int
     return 0;
 }
 
+/* FP64 device vectors in and out of the V-cycle's precision */
+static int to_level(jxamd_solver *s, int n, const double *src, void *dst)
+{
+    if (s->xt == JXAMD_T_F64) return jxamd_k_copy(n, src, (double *)dst, s->stream);
+    return jxamd_k_gather_cast(n, src, 1, 0, (float *)dst, s->stream);
+}
+
+static int from_level(jxamd_solver *s, int n, const void *src, double *dst)
+{
+    if (s->xt == JXAMD_T_F64) return jxamd_k_copy(n, (const double *)src, dst, s->stream);
+    return jxamd_k_widen(n, (const float *)src, dst, s->stream);
+}
+
 int jxamd_op_hsgs(jxamd_solver *s, int level, const double *d_f, double *d_u)
 {
     if (!s || level < 0 || (s->L > 1 && level >= s->L - 1) || level >= s->L) return JXAMD_ERROR_ARG;
-    CK(relax_once(s, level, d_f, 1, d_u, 0));
+    const int n = s->nlev[level];
+    CK(to_level(s, n, d_f, s->F[level]));
+    CK(to_level(s, n, d_u, s->U[level]));
+    CK(relax_once(s, level, s->F[level], 1, s->U[level], 0));
+    CK(from_level(s, n, s->U[level], d_u));
     return jxamd_stream_sync(s->stream);
 }
 
 int jxamd_op_coarse_solve(jxamd_solver *s, const double *d_f, double *d_u)
 {
     if (!s || s->L < 2) return JXAMD_ERROR_ARG;
-    CK(jxamd_k_dense_gemv(s->nc, s->cinv, d_f, d_u, s->stream));
+    const int L = s->L;
+    CK(to_level(s, s->nc, d_f, s->F[L - 1]));
+    CK(jxamd_k_dense_gemv(s->xt, s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
+    CK(from_level(s, s->nc, s->U[L - 1], d_u));
     return jxamd_stream_sync(s->stream);
 }
 
 int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u)
 {
     if (!s) return JXAMD_ERROR_ARG;
-    CK(jxamd_k_copy(s->nlev[0], d_u, s->U[0], s->stream));
-    CK(vcycle(s, d_f, 1, 0));
-    CK(jxamd_k_copy(s->nlev[0], s->U[0], d_u, s->stream));
+    const int n = s->nlev[0];
+    CK(to_level(s, n, d_f, s->F[0]));
+    CK(to_level(s, n, d_u, s->U[0]));
+    CK(vcycle(s, s->F[0], 1, 0));
+    CK(from_level(s, n, s->U[0], d_u));
     return jxamd_stream_sync(s->stream);
 }
 

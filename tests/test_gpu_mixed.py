@@ -1,0 +1,122 @@
+"""GPU mixed-precision stage 1 (pressure-AMG V-cycle in FP32, or FP16 with a
+power-of-two scale per stored row and FP32 accumulation), against the FP64
+oracle.
+
+The reference's own mixed driver (src/test_bsr_cprgmres_hybrid.c) cannot be
+built here (its FP64 twin library is absent), so the mixed results are
+parity-unpinned against the reference; they are held to: operator error
+against the FP64 oracle within the stated precision bounds, the exact FP16
+quantisation reproduced on the CPU, the whole solve's true FP64 residual
+<= rtol and the iteration count within two of the plain FP64 path."""
+import numpy as np
+import pytest
+
+import jxpamg_all_amd as jx
+from oracle.cpu_ref import Oracle
+from tests._common import relerr, system
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+FP32_OP_TOL = 2e-5    # one smoother call / coarse solve, relative to max |expected|
+FP32_CYCLE_TOL = 1e-4  # whole V-cycle (errors compound over levels)
+FP16_CYCLE_TOL = 5e-2  # whole V-cycle with FP16 level matrices (2^-11 per entry)
+
+
+def dev(x):
+    return torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _solver(name, prec, **kw):
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    p = jx.default_params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]), tol=meta["rtol"],
+                          stage2_maxit=int(meta["stage2_maxit"]), precond_precision=prec, **kw)
+    print(f"[gpu] case={name} precision={prec} {kw}", flush=True)
+    return d, Ad, bd, H, meta, jx.Solver(Ad, H, p)
+
+
+def _fp16_offdiag(H, level):
+    """Level matrix as the FP16 smoother layout stores it: off-diagonals as
+    float16(a / s_i) * s_i with s_i the smallest power of two >= max_j |a_ij|
+    over the row's off-diagonals, diagonal rounded to FP32."""
+    ia, ja, a, nc = H.A[level]
+    q = a.copy()
+    for i in range(len(ia) - 1):
+        k0, k1 = ia[i], ia[i + 1]
+        q[k0] = np.float32(a[k0])
+        off = a[k0 + 1:k1]
+        mx = np.max(np.abs(off)) if len(off) else 0.0
+        if mx > 0:
+            m, e = np.frexp(mx)
+            if m == 0.5:
+                e -= 1
+            sc = np.ldexp(1.0, int(e))
+            q[k0 + 1:k1] = (off / sc).astype(np.float32).astype(np.float16).astype(np.float64) * sc
+    return ia, ja, q, nc
+
+
+@pytest.mark.parametrize("name,chunk", [("c1", 0), ("lvl3", 0), ("lvl3", 4), ("layered", 8)])
+def test_fp32_stage1_operators_and_solve(name, chunk):
+    d, Ad, bd, H, meta, s = _solver(name, 1, relax_chunk=chunk)
+    o = Oracle(Ad, H)
+    T = [1] * H.num_levels
+    if chunk:
+        T = [min(len(m[0]) - 1, -(-(len(m[0]) - 1) // chunk)) for m in H.A]
+        o.set_relax_threads(T)
+    rng = np.random.default_rng(1)
+    n0 = len(H.A[0][0]) - 1
+    f, u0 = rng.standard_normal(n0), rng.standard_normal(n0)
+    u = dev(u0)
+    s.hsgs(0, dev(f).data_ptr(), u.data_ptr())
+    assert relerr(host(u), o.relax6_threads(0, f, u0, T[0])) < FP32_OP_TOL
+    uv = dev(np.zeros(n0))
+    s.vcycle(dev(f).data_ptr(), uv.data_ptr())
+    assert relerr(host(uv), o.vcycle(f)) < FP32_CYCLE_TOL
+    plain = jx.Solver(Ad, H, jx.default_params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]),
+                                               tol=meta["rtol"], stage2_maxit=int(meta["stage2_maxit"]),
+                                               relax_chunk=chunk))
+    _, st64, _ = plain.solve(bd)
+    plain.close()
+    x, st, hist = s.solve(bd)
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    print(f"[gpu] {name} fp32 chunk {chunk}: iters {st.iterations} (fp64 {st64.iterations}, reference "
+          f"{int(d['iterations'][0])}) true_rel {true:.3e}", flush=True)
+    assert st.converged == 1 and true <= meta["rtol"]
+    assert abs(st.iterations - st64.iterations) <= 2
+    assert s.error_flag() == 0
+    s.close()
+
+
+@pytest.mark.parametrize("name,chunk", [("b4", 4), ("lvl3", 8)])
+def test_fp16_stage1_quantisation_and_solve(name, chunk):
+    d, Ad, bd, H, meta, s = _solver(name, 2, relax_chunk=chunk)
+    T = [min(len(m[0]) - 1, -(-(len(m[0]) - 1) // chunk)) for m in H.A]
+    # the smoother with FP16 storage equals the FP64 smoother on the quantised matrix
+    Hq = jx.Hierarchy([_fp16_offdiag(H, 0)] + list(H.A[1:]), list(H.P))
+    oq = Oracle(Ad, Hq)
+    rng = np.random.default_rng(2)
+    n0 = len(H.A[0][0]) - 1
+    f, u0 = rng.standard_normal(n0), rng.standard_normal(n0)
+    u = dev(u0)
+    s.hsgs(0, dev(f).data_ptr(), u.data_ptr())
+    got = host(u)
+    assert relerr(got, oq.relax6_threads(0, f, u0, T[0])) < FP32_OP_TOL
+    o = Oracle(Ad, H)
+    o.set_relax_threads(T)
+    uv = dev(np.zeros(n0))
+    s.vcycle(dev(f).data_ptr(), uv.data_ptr())
+    assert relerr(host(uv), o.vcycle(f)) < FP16_CYCLE_TOL
+    x, st, hist = s.solve(bd)
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    print(f"[gpu] {name} fp16 chunk {chunk}: iters {st.iterations} (reference {int(d['iterations'][0])}) "
+          f"true_rel {true:.3e}", flush=True)
+    assert st.converged == 1 and true <= meta["rtol"]
+    assert abs(st.iterations - int(d["iterations"][0])) <= 2
+    assert s.error_flag() == 0
+    s.close()
