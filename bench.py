@@ -121,7 +121,7 @@ def main():
     achieved = (by_l / n_l) / (per_launch_ms * 1e-3) / 1e9
     kernel_names = {"gs_sweep": "k_gs_sweep", "bhgs_sweep": "k_bhgs_sweep", "bsr_spmv": "k_bsr_spmv",
                     "csr_spmv": "k_csr_spmv", "coarse_solve": "k_dense_gemv", "mgs": "k_dot_mgs",
-                    "vector_ops": "k_scale/k_axpy/k_lincomb"}
+                    "vector_ops": "k_scale/k_axpy/k_lincomb", "bilu_solve": "k_btri"}
     traffic = read_traffic(kernel_names[dom])
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,

@@ -120,6 +120,13 @@ void jxamd_amg_params_default(jxamd_amg_params *p);
 int jxamd_amg_setup(const jxamd_csr *A0, const jxamd_amg_params *p, jxamd_hierarchy *H);
 void jxamd_hierarchy_free(jxamd_hierarchy *H);
 
+/* Exact block ILU(0) on A's block pattern (diagonal block first, then
+ * ascending block columns): lu[nnzb*b*b] holds L (strictly lower blocks, unit
+ * diagonal implied) and U (diagonal and upper blocks), dinv[nb*b*b] the
+ * inverted diagonal blocks of U.  Stands in for the reference's iterative
+ * BILU setup (jxfpamg-0220/csrc/ilu/par_bilu_setup.c:17, type 3). */
+int jxamd_bilu0_factor(int nb, int b, const int *ia, const int *ja, const double *a, double *lu, double *dinv);
+
 /* Parameters.  CPR keys mirror JXF_CPRSetParameter / JXF_CPRSetRealParameter
  * (jxfpamg-0220/csrc/cpr/jxf_cpr.c:226-262) plus the stage-2 weights fixed at
  * jxf_cpr.c:547-549; GMRES keys mirror the JXF_GMRESSet* setters
@@ -128,7 +135,8 @@ typedef struct {
     int k_dim, max_iter, is_check_restarted;
     double tol, a_tol;
     int pressure_index, stage1_maxit, stage2_maxit;
-    int stage1_solver_type, stage2_solver_type, print_level;
+    int stage1_solver_type, stage2_solver_type, print_level; /* stage2_solver_type: 2 block hybrid SOR
+                                                               * (jxf_cpr.c:573), 5 BILU(0) corrections */
     double threshold, stage2_weight, stage2_omega;
     /* AMG smoother (relax 6) threading, jxfpamg-0220/csrc/operation/
      * relaxations/par_relax_6.c: relax_threads = T > 1 is the reference's
@@ -199,6 +207,9 @@ int jxamd_op_hsgs(jxamd_solver *s, int level, const double *d_f, double *d_u);
 int jxamd_op_coarse_solve(jxamd_solver *s, const double *d_f, double *d_u);
 /* one V-cycle from u (jxf_PAMGSolve with max_iter 1, par_amg_solve.c:36) */
 int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u);
+/* z = (LU)^{-1} r with the exact BILU(0) factors of A (stage2_solver_type 5;
+ * the reference's iterative BILU: jxfpamg-0220/csrc/ilu/par_bilu_solve.c:360-366) */
+int jxamd_op_bilu_solve(jxamd_solver *s, const double *d_r, double *d_z);
 /* one block hybrid-SOR sweep, jxf_ParBSRHGSRelax forward (par_relax_bsr.c:193-573) */
 int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, double omega);
 /* y = A*x launched reps times back to back on the solver's stream; the mean
@@ -218,7 +229,8 @@ enum {
     JXAMD_PROF_COARSE = 4,     /* coarsest-level dense solve */
     JXAMD_PROF_MGS = 5,        /* fused Gram-Schmidt dots / norms */
     JXAMD_PROF_VEC = 6,        /* scale, axpy, linear combinations, copies */
-    JXAMD_PROF_KINDS = 7
+    JXAMD_PROF_BILU = 7,       /* stage-2 BILU(0) block triangular solves */
+    JXAMD_PROF_KINDS = 8
 };
 int jxamd_solver_profile(jxamd_solver *s, int enable); /* enable also clears the counters */
 int jxamd_solver_profile_read(jxamd_solver *s, int kind, long *launches, double *ms, double *bytes);

@@ -112,6 +112,8 @@ def _declare(L):
         "jxamd_hierarchy_free": [vp],
         "jxamd_cpr_precond_timed": [vp, vp, vp, i32, vp],
         "jxamd_solver_interface": [i32, vp, i32, vp, vp, vp, vp, vp, f64, i32, i32, vp],
+        "jxamd_op_bilu_solve": [vp, vp, vp],
+        "jxamd_bilu0_factor": [i32, i32, vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -213,6 +215,15 @@ def solver_interface(row, ia, ja, a, ser_x, ser_b, tol, solver_id, num_functions
                                        _p(a), _p(ser_x), _p(ser_b), float(tol), int(solver_id), int(num_functions),
                                        ctypes.byref(it)), "jxamd_solver_interface")
     return it.value
+
+
+def bilu0_factor(A):
+    """Exact block ILU(0) factors of a BSR matrix: (lu on A's pattern, dinv)."""
+    bb = A.b * A.b
+    lu = np.zeros(A.nnzb * bb)
+    dinv = np.zeros(A.nb * bb)
+    check(lib().jxamd_bilu0_factor(A.nb, A.b, _p(A.ia), _p(A.ja), _p(A.a), _p(lu), _p(dinv)), "jxamd_bilu0_factor")
+    return lu, dinv
 
 
 def decouple_abf(A, rhs=None):
@@ -391,6 +402,9 @@ class Solver:
     def vcycle(self, d_f, d_u):
         check(self._L.jxamd_op_vcycle(self._s, ctypes.c_void_p(d_f), ctypes.c_void_p(d_u)), "jxamd_op_vcycle")
 
+    def bilu_solve(self, d_r, d_z):
+        check(self._L.jxamd_op_bilu_solve(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_z)), "jxamd_op_bilu_solve")
+
     def bhgs(self, d_x, d_b, w, omega):
         check(self._L.jxamd_op_bhgs(self._s, ctypes.c_void_p(d_x), ctypes.c_void_p(d_b), w, omega),
               "jxamd_op_bhgs")
@@ -407,7 +421,7 @@ class Solver:
                                               ctypes.byref(ms)), "jxamd_cpr_precond_timed")
         return ms.value
 
-    PROF_KINDS = ["gs_sweep", "bhgs_sweep", "bsr_spmv", "csr_spmv", "coarse_solve", "mgs", "vector_ops"]
+    PROF_KINDS = ["gs_sweep", "bhgs_sweep", "bsr_spmv", "csr_spmv", "coarse_solve", "mgs", "vector_ops", "bilu_solve"]
 
     def profile(self, enable=True):
         check(self._L.jxamd_solver_profile(self._s, 1 if enable else 0), "jxamd_solver_profile")

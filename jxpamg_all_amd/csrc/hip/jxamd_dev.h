@@ -46,6 +46,10 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
                        const int *dlo_list, const int *dall_ptr, const int *dall_list, int *sflag, int epoch,
                        double w, double omega, unsigned long long *ticket, unsigned long long *base, int *err,
                        void *stream);
+/* BILU(0) block triangular solve (vt: F64 or F32 factor storage) */
+int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
+                 const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
+                 int epoch, unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
 /* hybrid SSOR with short chunks (at most JXAMD_CHUNK_LDS_MAX rows): SELL-64
  * of the off-diagonals in natural row order, diagonal in diag[i] */
 #define JXAMD_CHUNK_LDS_MAX 16

@@ -345,6 +345,67 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
     }
 }
 
+// One block triangular solve of BILU(0), sync-free over level-pure slices:
+//   out_i = Dinv_i (rhs_i - sum_j M_ij out_j)   (Dinv_i = I when dinv == nullptr)
+// where the slice layout holds only the blocks M_ij of one triangle (the
+// dependencies: lower blocks for the forward L solve, upper blocks for the
+// backward U solve), stored as V (double or float), vectors double.
+template <int B, typename V>
+__global__ void __launch_bounds__(256) k_btri(int nsl, const int *__restrict__ perm, const int *__restrict__ sptr,
+                                              const int *__restrict__ col, const V *__restrict__ val,
+                                              const V *__restrict__ dinv, const double *__restrict__ rhs, double *out,
+                                              const int *__restrict__ dep_ptr, const int *__restrict__ dep_list,
+                                              int *sflag, int epoch, unsigned long long *ticket,
+                                              unsigned long long base, int *err)
+{
+    const int lane = threadIdx.x % WAVE;
+    for (;;) {
+        const int s = claim_slice(ticket, base, lane);
+        if (s < 0 || s >= nsl) break;
+        const int p = s * WAVE + lane;
+        const int i = perm[p];
+        const int k0 = sptr[s], k1 = sptr[s + 1];
+        double acc[B], D[B * B];
+        if (i >= 0) {
+#pragma unroll
+            for (int r = 0; r < B; r++) acc[r] = rhs[(size_t)i * B + r];
+            if (dinv) {
+#pragma unroll
+                for (int q = 0; q < B * B; q++) D[q] = (double)dinv[(size_t)i * B * B + q];
+            }
+        }
+        wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
+        if (i >= 0) {
+            for (int k = k0; k < k1; k++) {
+                const int j = col[(size_t)k * WAVE + lane];
+                if (j == i) continue; // padding slot
+                double xc[B];
+#pragma unroll
+                for (int q = 0; q < B; q++) xc[q] = ld_coherent(out + (size_t)j * B + q);
+                const V *vv = val + (size_t)k * B * B * WAVE + lane;
+#pragma unroll
+                for (int r = 0; r < B; r++) {
+#pragma unroll
+                    for (int q = 0; q < B; q++) acc[r] -= (double)vv[(r * B + q) * WAVE] * xc[q];
+                }
+            }
+            if (dinv) {
+#pragma unroll
+                for (int r = 0; r < B; r++) {
+                    double o = 0.0;
+#pragma unroll
+                    for (int q = 0; q < B; q++) o += D[r * B + q] * acc[q];
+                    st_coherent(out + (size_t)i * B + r, o);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < B; r++) st_coherent(out + (size_t)i * B + r, acc[r]);
+            }
+        }
+        st_flag_release(sflag + s, epoch);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Hybrid SSOR with T threads (relax 6, weights 1; par_relax_6.c:132-223):
 // rows split into T contiguous chunks (first n % T chunks one row longer);
@@ -813,6 +874,39 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
     }
 #undef LAUNCH_BHGS
     *base += (unsigned long long)nvs + (unsigned long long)grid * 4;
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
+                 const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
+                 int epoch, unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+{
+    if (nsl == 0) return 0;
+    if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32) return JXAMD_ERROR_ARG;
+    const int waves = jxamd_sweep_waves(nsl);
+    const int grid = (waves + 3) / 4;
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned long long b0 = *base;
+#define LAUNCH_BTRI(BS, VT)                                                                                            \
+    k_btri<BS, VT><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, (const VT *)val, (const VT *)dinv, rhs, out, dep_ptr,  \
+                                         dep_list, sflag, epoch, ticket, b0, err)
+    if (vt == JXAMD_T_F64) {
+        switch (b) {
+        case 2: LAUNCH_BTRI(2, double); break;
+        case 3: LAUNCH_BTRI(3, double); break;
+        case 4: LAUNCH_BTRI(4, double); break;
+        default: return JXAMD_ERROR_ARG;
+        }
+    } else {
+        switch (b) {
+        case 2: LAUNCH_BTRI(2, float); break;
+        case 3: LAUNCH_BTRI(3, float); break;
+        case 4: LAUNCH_BTRI(4, float); break;
+        default: return JXAMD_ERROR_ARG;
+        }
+    }
+#undef LAUNCH_BTRI
+    *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
     return chk(hipGetLastError());
 }
 

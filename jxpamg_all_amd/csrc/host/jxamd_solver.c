@@ -55,6 +55,15 @@ typedef struct {
     double spmv_bytes, sweep_bytes; /* algorithmic bytes: y = A x, one block-HGS sweep */
 } dbsr;
 
+/* one BILU(0) triangle on the device: level-ordered block SELL of the
+ * dependency blocks (L: j < i, U: j > i), U's inverted diagonal per row */
+typedef struct {
+    int nsl, vt;
+    int *perm, *sptr, *col, *dep_ptr, *dep_list, *flag;
+    void *val, *dinv;
+    double bytes;
+} dtri;
+
 /* per-kernel-class device timing (HIP events around each launch) */
 #define PROF_POOL 4096
 typedef struct {
@@ -86,6 +95,8 @@ struct jxamd_solver_s {
     size_t xsz;            /* bytes per V-cycle vector entry */
     void *F[JXAMD_MAX_LEVELS], *U[JXAMD_MAX_LEVELS], *T[JXAMD_MAX_LEVELS], *Vt[JXAMD_MAX_LEVELS];
     double *up64;          /* FP64 copy of the stage-1 result when xt != F64 */
+    dtri Lf, Ub;           /* stage-2 BILU(0) factors (stage2_solver_type 5) */
+    double *bz;            /* BILU(0) intermediate vector, N */
     double *xtmp;          /* stage-2 ping-pong partner, N */
     double *V;             /* (k_dim+1) Krylov vectors, N each */
     double *Rv, *Wv, *Bd, *Xd; /* GMRES work, host-solve staging */
@@ -778,6 +789,111 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
     return 0;
 }
 
+static void dtri_free(dtri *t)
+{
+    jxamd_dev_free(t->perm); jxamd_dev_free(t->sptr); jxamd_dev_free(t->col); jxamd_dev_free(t->dep_ptr);
+    jxamd_dev_free(t->dep_list); jxamd_dev_free(t->flag); jxamd_dev_free(t->val); jxamd_dev_free(t->dinv);
+    memset(t, 0, sizeof(*t));
+}
+
+/* upload n doubles as type vt (F64 or F32) */
+static int upload_real(void **dst, const double *src, size_t n, int vt, void *st)
+{
+    CK(jxamd_dev_malloc(dst, type_size(vt) * (n ? n : 1)));
+    if (vt == JXAMD_T_F64) return jxamd_h2d(*dst, src, sizeof(double) * n, st);
+    float *t = (float *)malloc(sizeof(float) * (n ? n : 1));
+    if (!t) return JXAMD_ERROR_MEMORY;
+    for (size_t k = 0; k < n; k++) t[k] = (float)src[k];
+    int rc = jxamd_h2d(*dst, t, sizeof(float) * n, st);
+    if (!rc) rc = jxamd_stream_sync(st);
+    free(t);
+    return rc;
+}
+
+/* one triangle of the BILU(0) factors: dir > 0 the unit lower L (forward
+ * level order), dir < 0 the upper U with its inverted diagonal (backward) */
+static int build_tri(jxamd_solver *s, const jxamd_bsr *A, const double *lu, const double *dinv, int dir, dtri *t)
+{
+    const int nb = A->nb, b = A->b, bb = b * b;
+    memset(t, 0, sizeof(*t));
+    t->vt = s->prm.precond_precision > 0 ? JXAMD_T_F32 : JXAMD_T_F64;
+    int *fia = (int *)malloc(sizeof(int) * ((size_t)nb + 1));
+    int *fja = (int *)malloc(sizeof(int) * (size_t)(A->nnzb > 0 ? A->nnzb : 1));
+    int *fidx = (int *)malloc(sizeof(int) * (size_t)(A->nnzb > 0 ? A->nnzb : 1));
+    double *dummy = (double *)calloc((size_t)(A->nnzb > 0 ? A->nnzb : 1), sizeof(double));
+    if (!fia || !fja || !fidx || !dummy) { free(fia); free(fja); free(fidx); free(dummy); return JXAMD_ERROR_MEMORY; }
+    long nf = 0;
+    fia[0] = 0;
+    for (int i = 0; i < nb; i++) {
+        for (int k = A->ia[i]; k < A->ia[i + 1]; k++)
+            if (dir > 0 ? A->ja[k] < i : A->ja[k] > i) { fja[nf] = A->ja[k]; fidx[nf] = k; nf++; }
+        fia[i + 1] = (int)nf;
+    }
+    hsched sc;
+    int e = build_schedule(nb, A->ia, A->ja, dir, &sc);
+    hsell h;
+    memset(&h, 0, sizeof(h));
+    int *perm = NULL;
+    if (!e) {
+        perm = sc.perm;
+        sc.perm = NULL; /* owned by h */
+        e = hsell_build(&h, sc.npos, fia, fja, dummy, perm, 0, 1);
+    }
+    double *v = NULL;
+    size_t slots = 0;
+    if (!e) {
+        slots = (size_t)(h.nsl ? h.sptr[h.nsl] : 0);
+        v = (double *)calloc(slots * bb * SELLW + 1, sizeof(double));
+        if (!v) e = JXAMD_ERROR_MEMORY;
+    }
+    if (!e) {
+        for (int q = 0; q < h.nsl; q++) {
+            const int k0 = h.sptr[q], w = h.sptr[q + 1] - k0;
+            for (int l = 0; l < SELLW; l++) {
+                const int i = perm[q * SELLW + l];
+                if (i < 0) continue;
+                const int beg = fia[i], len = fia[i + 1] - beg;
+                for (int k = 0; k < w && k < len; k++) {
+                    const double *blk = lu + (size_t)fidx[beg + k] * bb;
+                    for (int e2 = 0; e2 < bb; e2++) v[((size_t)(k0 + k) * bb + e2) * SELLW + l] = blk[e2];
+                }
+            }
+        }
+        t->nsl = h.nsl;
+        e = upload_ints(&t->perm, perm, (size_t)sc.npos, s->stream);
+        if (!e) e = upload_ints(&t->sptr, h.sptr, (size_t)h.nsl + 1, s->stream);
+        if (!e) e = upload_ints(&t->col, h.col, slots * SELLW, s->stream);
+        if (!e) e = upload_real(&t->val, v, slots * bb * SELLW, t->vt, s->stream);
+        if (!e && dir < 0) e = upload_real(&t->dinv, dinv, (size_t)nb * bb, t->vt, s->stream);
+        if (!e) e = upload_ints(&t->dep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
+        if (!e) e = upload_ints(&t->dep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
+        if (!e) e = zero_ints(&t->flag, (size_t)sc.nsl, s->stream);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        /* dependency blocks + indices, rhs read, result written, dependency values read */
+        t->bytes = (double)nf * (type_size(t->vt) * bb + 4.0) + 24.0 * nb * b + (dir < 0 ? (double)type_size(t->vt) * nb * bb : 0.0);
+    }
+    free(v);
+    hsell_free(&h); /* frees perm */
+    hsched_free(&sc);
+    free(fia); free(fja); free(fidx); free(dummy);
+    return e;
+}
+
+static int build_bilu(jxamd_solver *s, const jxamd_bsr *A)
+{
+    const size_t bb = (size_t)A->b * A->b;
+    double *lu = (double *)malloc(sizeof(double) * (size_t)A->nnzb * bb);
+    double *dinv = (double *)malloc(sizeof(double) * (size_t)A->nb * bb);
+    if (!lu || !dinv) { free(lu); free(dinv); return JXAMD_ERROR_MEMORY; }
+    int e = jxamd_bilu0_factor(A->nb, A->b, A->ia, A->ja, A->a, lu, dinv);
+    if (!e) e = build_tri(s, A, lu, dinv, 1, &s->Lf);
+    if (!e) e = build_tri(s, A, lu, dinv, -1, &s->Ub);
+    if (!e) e = jxamd_dev_malloc((void **)&s->bz, sizeof(double) * s->N);
+    free(lu);
+    free(dinv);
+    return e;
+}
+
 static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
 {
     const jxamd_csr *A = &H->A[l];
@@ -907,6 +1023,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->stage1_maxit < 0 || p->stage2_maxit < 0) return JXAMD_ERROR_ARG;
     if (p->relax_threads < 0 || p->relax_chunk < 0) return JXAMD_ERROR_ARG;
     if (p->precond_precision < 0 || p->precond_precision > 2) return JXAMD_ERROR_ARG;
+    if (p->stage2_solver_type != 2 && p->stage2_solver_type != 5) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -941,6 +1058,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_event_create(&s->ev2);
     if (!e) e = jxamd_event_create(&s->ev3);
     if (!e) e = build_bsr(s, A);
+    if (!e && p->stage2_solver_type == 5) e = build_bilu(s, A);
     for (int l = 0; l < s->L && !e; l++) {
         s->nlev[l] = H->A[l].n;
         e = build_level(s, l, H);
@@ -1001,6 +1119,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
         jxamd_dev_free(s->F[l]); jxamd_dev_free(s->U[l]); jxamd_dev_free(s->T[l]); jxamd_dev_free(s->Vt[l]);
     }
     jxamd_dev_free(s->cinv); jxamd_dev_free(s->up64);
+    dtri_free(&s->Lf); dtri_free(&s->Ub); jxamd_dev_free(s->bz);
     jxamd_dev_free(s->xtmp); jxamd_dev_free(s->V); jxamd_dev_free(s->Rv); jxamd_dev_free(s->Wv);
     jxamd_dev_free(s->Bd); jxamd_dev_free(s->Xd);
     jxamd_dev_free(s->dsc); jxamd_dev_free(s->part); jxamd_dev_free(s->count);
@@ -1137,6 +1256,25 @@ static int bhgs_run(jxamd_solver *s, int K, const double *rhs, const double *x0,
     return e;
 }
 
+static int bsr_apply(jxamd_solver *s, const double *x, const double *rhs, double *y, int mode);
+
+static int tri_run(jxamd_solver *s, const dtri *t, const double *rhs, double *out)
+{
+    s->epoch++;
+    int pq = prof_begin(s);
+    CK(jxamd_k_btri(s->b, t->vt, t->nsl, t->perm, t->sptr, t->col, t->val, t->dinv, rhs, out, t->dep_ptr,
+                    t->dep_list, t->flag, s->epoch, s->ticket, &s->tbase, s->err, s->stream));
+    prof_end(s, pq, JXAMD_PROF_BILU, t->bytes);
+    return 0;
+}
+
+/* z = (LU)^{-1} r; r and z may not alias s->bz */
+static int bilu_solve(jxamd_solver *s, const double *r, double *z)
+{
+    CK(tri_run(s, &s->Lf, r, s->bz));
+    return tri_run(s, &s->Ub, s->bz, z);
+}
+
 /* x = M^{-1} r (jxf_cpr.c:424-610) */
 static int cpr_apply(jxamd_solver *s, const double *r, double *x)
 {
@@ -1164,7 +1302,17 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
     /* prolongation (type 0) + stage-2 sweeps; the first sweep reads the
      * prolongated vector on the fly and the last one lands in x */
     const int m = s->prm.stage2_maxit;
-    if (m == 0) return jxamd_k_prolong(s->nb, b, pi, xp, x, s->stream);
+    if (m == 0 || s->prm.stage2_solver_type == 5) CK(jxamd_k_prolong(s->nb, b, pi, xp, x, s->stream));
+    if (m == 0) return 0;
+    if (s->prm.stage2_solver_type == 5) {
+        /* BILU(0) corrections: x += (LU)^{-1} (r - A x) */
+        for (int it = 0; it < m; it++) {
+            CK(bsr_apply(s, x, r, s->xtmp, 1));
+            CK(bilu_solve(s, s->xtmp, s->xtmp));
+            CK(jxamd_k_axpy(s->N, NULL, 1.0, 0, s->xtmp, x, s->stream));
+        }
+        return 0;
+    }
     return bhgs_run(s, m, r, NULL, xp, x, s->prm.stage2_weight, s->prm.stage2_omega);
 }
 
@@ -1253,6 +1401,13 @@ int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u)
     CK(to_level(s, n, d_u, s->U[0]));
     CK(vcycle(s, s->F[0], 1, 0));
     CK(from_level(s, n, s->U[0], d_u));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_op_bilu_solve(jxamd_solver *s, const double *d_r, double *d_z)
+{
+    if (!s || s->prm.stage2_solver_type != 5 || !d_r || !d_z || d_r == d_z) return JXAMD_ERROR_ARG;
+    CK(bilu_solve(s, d_r, d_z));
     return jxamd_stream_sync(s->stream);
 }
 

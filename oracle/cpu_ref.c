@@ -453,6 +453,131 @@ int cref_pamg_solve(const cref_hier *H, const double *f, double *u)
  * stage1_maxit AMG solves, prolongate type 0 (overwrite, :127-133), then
  * stage2_maxit forward block HGS sweeps on A x = r.  The debug residuals
  * printed at :518-524 and :599-605 change nothing and are not repeated. */
+/* ---------------------------------------------------------------------
+ * BILU(0) -- no reference restatement: the reference's BILU
+ * (J/csrc/ilu/par_bilu_setup.c:17, jxf_ILUSetupFGPBILU_v3 :4996) is an
+ * iterative fine-grained variant and is not on the test_bsr_cprgmres path.
+ * This is the textbook exact block ILU(0) (IKJ order, no fill outside A's
+ * pattern), pinned by its defining properties in the tests.
+ * --------------------------------------------------------------------- */
+
+/* In-place inverse of a b x b block, Gauss-Jordan with partial pivoting;
+ * returns 1 if singular. */
+static int block_inverse(double *M, int b)
+{
+    double inv[16], a[16];
+    for (int i = 0; i < b * b; i++) a[i] = M[i];
+    for (int i = 0; i < b; i++)
+        for (int j = 0; j < b; j++) inv[i * b + j] = (i == j) ? 1.0 : 0.0;
+    for (int c = 0; c < b; c++) {
+        int pr = c;
+        for (int r = c + 1; r < b; r++)
+            if (fabs(a[r * b + c]) > fabs(a[pr * b + c])) pr = r;
+        if (a[pr * b + c] == 0.0) return 1;
+        if (pr != c)
+            for (int j = 0; j < b; j++) {
+                double t = a[c * b + j]; a[c * b + j] = a[pr * b + j]; a[pr * b + j] = t;
+                t = inv[c * b + j]; inv[c * b + j] = inv[pr * b + j]; inv[pr * b + j] = t;
+            }
+        double d = 1.0 / a[c * b + c];
+        for (int j = 0; j < b; j++) { a[c * b + j] *= d; inv[c * b + j] *= d; }
+        for (int r = 0; r < b; r++) {
+            if (r == c) continue;
+            double f = a[r * b + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < b; j++) { a[r * b + j] -= f * a[c * b + j]; inv[r * b + j] -= f * inv[c * b + j]; }
+        }
+    }
+    for (int i = 0; i < b * b; i++) M[i] = inv[i];
+    return 0;
+}
+
+/* lu (same pattern as A): strictly lower blocks hold L (unit block lower),
+ * the diagonal and upper blocks hold U; dinv[i] = U_ii^{-1}.  Columns of a
+ * row must be ascending after the diagonal block. */
+int cref_bilu0_factor(const cref_bsr *A, double *lu, double *dinv)
+{
+    const int nb = A->nb, b = A->b, bb = b * b;
+    int *pos = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+    if (!pos) return 2;
+    for (int j = 0; j < nb; j++) pos[j] = -1;
+    for (long k = 0; k < (long)A->ia[nb] * bb; k++) lu[k] = A->a[k];
+    int err = 0;
+    for (int i = 0; i < nb && !err; i++) {
+        for (int k = A->ia[i]; k < A->ia[i + 1]; k++) pos[A->ja[k]] = k;
+        for (int kk = A->ia[i] + 1; kk < A->ia[i + 1]; kk++) {
+            const int k = A->ja[kk];
+            if (k > i) break;
+            /* L_ik = A_ik U_kk^{-1} */
+            double t[16];
+            const double *Dk = dinv + (long)k * bb;
+            double *Lik = lu + (long)kk * bb;
+            for (int r = 0; r < b; r++)
+                for (int c = 0; c < b; c++) {
+                    double acc = 0.0;
+                    for (int q = 0; q < b; q++) acc += Lik[r * b + q] * Dk[q * b + c];
+                    t[r * b + c] = acc;
+                }
+            for (int e = 0; e < bb; e++) Lik[e] = t[e];
+            /* A_ij -= L_ik U_kj for j > k in row k that are in row i */
+            for (int jj = A->ia[k] + 1; jj < A->ia[k + 1]; jj++) {
+                const int j = A->ja[jj];
+                if (j <= k || pos[j] < 0) continue;
+                double *Aij = lu + (long)pos[j] * bb;
+                const double *Ukj = lu + (long)jj * bb;
+                for (int r = 0; r < b; r++)
+                    for (int c = 0; c < b; c++) {
+                        double acc = 0.0;
+                        for (int q = 0; q < b; q++) acc += Lik[r * b + q] * Ukj[q * b + c];
+                        Aij[r * b + c] -= acc;
+                    }
+            }
+        }
+        double *Di = dinv + (long)i * bb;
+        for (int e = 0; e < bb; e++) Di[e] = lu[(long)A->ia[i] * bb + e];
+        if (block_inverse(Di, b)) err = 1;
+        for (int k = A->ia[i]; k < A->ia[i + 1]; k++) pos[A->ja[k]] = -1;
+    }
+    free(pos);
+    return err;
+}
+
+/* z = (LU)^{-1} r: y_i = r_i - sum_{k<i} L_ik y_k, then
+ * z_i = U_ii^{-1} (y_i - sum_{j>i} U_ij z_j) */
+void cref_bilu0_solve(const cref_bsr *A, const double *lu, const double *dinv, const double *r, double *z)
+{
+    const int nb = A->nb, b = A->b, bb = b * b;
+    for (int i = 0; i < nb; i++) {
+        double acc[4];
+        for (int q = 0; q < b; q++) acc[q] = r[(long)i * b + q];
+        for (int kk = A->ia[i] + 1; kk < A->ia[i + 1]; kk++) {
+            const int k = A->ja[kk];
+            if (k > i) continue;
+            const double *L = lu + (long)kk * bb;
+            for (int q = 0; q < b; q++)
+                for (int c = 0; c < b; c++) acc[q] -= L[q * b + c] * z[(long)k * b + c];
+        }
+        for (int q = 0; q < b; q++) z[(long)i * b + q] = acc[q];
+    }
+    for (int i = nb - 1; i >= 0; i--) {
+        double acc[4];
+        for (int q = 0; q < b; q++) acc[q] = z[(long)i * b + q];
+        for (int jj = A->ia[i] + 1; jj < A->ia[i + 1]; jj++) {
+            const int j = A->ja[jj];
+            if (j < i) continue;
+            const double *U = lu + (long)jj * bb;
+            for (int q = 0; q < b; q++)
+                for (int c = 0; c < b; c++) acc[q] -= U[q * b + c] * z[(long)j * b + c];
+        }
+        const double *D = dinv + (long)i * bb;
+        for (int q = 0; q < b; q++) {
+            double o = 0.0;
+            for (int c = 0; c < b; c++) o += D[q * b + c] * acc[c];
+            z[(long)i * b + q] = o;
+        }
+    }
+}
+
 int cref_cpr_precond(const cref_bsr *A, const cref_hier *H, const cref_params *p, const double *r, double *x)
 {
     const int nb = A->nb, b = A->b, pi = p->pressure_index;
@@ -464,7 +589,22 @@ int cref_cpr_precond(const cref_bsr *A, const cref_hier *H, const cref_params *p
     for (int it = 0; it < p->stage1_maxit && !err; it++) err = cref_pamg_solve(H, rp, xp);
     for (int i = 0; i < nb; i++)
         for (int j = 0; j < b; j++) x[(long)i * b + j] = (j == pi) ? xp[i] : 0.0;
-    for (int it = 0; it < p->stage2_maxit; it++) cref_bsr_hgs(A, x, r, work, p->stage2_weight, p->stage2_omega, 1);
+    if (p->stage2_solver_type == 5) {
+        /* BILU(0) corrections from the prolongated pressure correction */
+        const long n = (long)nb * b;
+        double *res = (double *)malloc(sizeof(double) * n), *z = (double *)malloc(sizeof(double) * n);
+        for (int it = 0; it < p->stage2_maxit; it++) {
+            memcpy(res, r, sizeof(double) * n);
+            cref_bsr_matvec(A, -1.0, x, 1.0, res);
+            cref_bilu0_solve(A, p->bilu_lu, p->bilu_dinv, res, z);
+            for (long q = 0; q < n; q++) x[q] += z[q];
+        }
+        free(res);
+        free(z);
+    } else {
+        for (int it = 0; it < p->stage2_maxit; it++)
+            cref_bsr_hgs(A, x, r, work, p->stage2_weight, p->stage2_omega, 1);
+    }
     free(rp);
     free(xp);
     free(work);

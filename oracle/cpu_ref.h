@@ -37,6 +37,10 @@ typedef struct {
     double tol, a_tol;
     int pressure_index, stage1_maxit, stage2_maxit;
     double stage2_weight, stage2_omega;
+    /* 2: block hybrid SOR sweeps (the reference's BGS); 5: BILU(0)
+     * corrections x += (LU)^{-1}(r - A x) with the factors below */
+    int stage2_solver_type;
+    const double *bilu_lu, *bilu_dinv;
 } cref_params;
 
 typedef struct {
@@ -56,6 +60,8 @@ void cref_bsr_block_matvec_stable(double alpha, const double *mat, const double 
 void cref_bsr_block_invmatvec_stable(const double *mat, const double *v, double *ov, int b);
 void cref_bsr_hgs(const cref_bsr *A, double *x, const double *rhs, double *vtemp, double w, double omega, int fwd);
 int cref_pamg_solve(const cref_hier *H, const double *f, double *u);
+int cref_bilu0_factor(const cref_bsr *A, double *lu, double *dinv);
+void cref_bilu0_solve(const cref_bsr *A, const double *lu, const double *dinv, const double *r, double *z);
 int cref_cpr_precond(const cref_bsr *A, const cref_hier *H, const cref_params *p, const double *r, double *x);
 int cref_gmres_solve(const cref_bsr *A, const cref_hier *H, const cref_params *p, const double *b, double *x,
                      double *res_history, cref_stats *st);

@@ -32,7 +32,8 @@ class Params(ctypes.Structure):
     _fields_ = [("k_dim", ctypes.c_int), ("max_iter", ctypes.c_int), ("is_check_restarted", ctypes.c_int),
                 ("tol", ctypes.c_double), ("a_tol", ctypes.c_double), ("pressure_index", ctypes.c_int),
                 ("stage1_maxit", ctypes.c_int), ("stage2_maxit", ctypes.c_int), ("stage2_weight", ctypes.c_double),
-                ("stage2_omega", ctypes.c_double)]
+                ("stage2_omega", ctypes.c_double), ("stage2_solver_type", ctypes.c_int), ("bilu_lu", ctypes.c_void_p),
+                ("bilu_dinv", ctypes.c_void_p)]
 
 
 class Stats(ctypes.Structure):
@@ -63,6 +64,8 @@ def lib():
             ("cref_bsr_block_invmatvec_stable", [vp, vp, vp, i32], None),
             ("cref_bsr_hgs", [vp, vp, vp, vp, f64, f64, i32], None),
             ("cref_pamg_solve", [vp, vp, vp], i32),
+            ("cref_bilu0_factor", [vp, vp, vp], i32),
+            ("cref_bilu0_solve", [vp, vp, vp, vp, vp], None),
             ("cref_cpr_precond", [vp, vp, vp, vp, vp], i32),
             ("cref_gmres_solve", [vp, vp, vp, vp, vp, vp, vp], i32),
         ]:
@@ -94,11 +97,31 @@ class Oracle:
                 self.hier.P[l] = CSR(len(ia) - 1, nc, _p(ia).value, _p(ja).value, _p(a).value)
                 self.keep += [ia, ja, a]
 
-    @staticmethod
-    def params(k_dim=30, max_iter=100, is_check_restarted=1, tol=1e-6, a_tol=0.0, pressure_index=0,
-               stage1_maxit=1, stage2_maxit=3, stage2_weight=0.7, stage2_omega=0.9):
+    def params(self, k_dim=30, max_iter=100, is_check_restarted=1, tol=1e-6, a_tol=0.0, pressure_index=0,
+               stage1_maxit=1, stage2_maxit=3, stage2_weight=0.7, stage2_omega=0.9, stage2_solver_type=2):
+        lu = dinv = None
+        if stage2_solver_type == 5:
+            lu, dinv = self.bilu0()
+            lu, dinv = _p(lu).value, _p(dinv).value
         return Params(k_dim, max_iter, is_check_restarted, tol, a_tol, pressure_index, stage1_maxit, stage2_maxit,
-                      stage2_weight, stage2_omega)
+                      stage2_weight, stage2_omega, stage2_solver_type, lu, dinv)
+
+    def bilu0(self):
+        """Exact block ILU(0) factors (lu on A's pattern, dinv = U_ii^{-1}), cached."""
+        if getattr(self, "_bilu", None) is None:
+            bb = self.A.b * self.A.b
+            lu = np.zeros(self.A.nnzb * bb)
+            dinv = np.zeros(self.A.nb * bb)
+            rc = self.L.cref_bilu0_factor(ctypes.byref(self.bsr), _p(lu), _p(dinv))
+            assert rc == 0, "singular pivot block in BILU(0)"
+            self._bilu = (lu, dinv)
+        return self._bilu
+
+    def bilu0_solve(self, r):
+        lu, dinv = self.bilu0()
+        z = np.zeros(self.A.n)
+        self.L.cref_bilu0_solve(ctypes.byref(self.bsr), _p(lu), _p(dinv), _p(np.ascontiguousarray(r)), _p(z))
+        return z
 
     def matvec(self, alpha, x, beta, y):
         y = np.array(y, dtype=np.float64, copy=True)

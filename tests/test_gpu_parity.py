@@ -253,3 +253,30 @@ def test_solver_interface_case_220():
     assert abs(iters - sto.iterations) <= 2
     with pytest.raises(RuntimeError):
         jx.solver_interface(A.n, ia, ja, a, x, rhs.copy(), 1e-6, 22, A.b)
+
+
+@pytest.mark.parametrize("name,prec", [("c1", 0), ("b4", 0), ("b2", 0), ("lvl3", 0), ("lvl3", 1)])
+def test_bilu0_stage2_vs_oracle(name, prec):
+    """Stage-2 solver type 5: x += (LU)^{-1}(r - A x) with exact BILU(0)
+    factors (FP64, or FP32 storage with prec = 1)."""
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    p = jx.default_params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]), tol=meta["rtol"],
+                          stage2_maxit=1, stage2_solver_type=5, precond_precision=prec)
+    print(f"[gpu] case={name} bilu0 precision={prec}", flush=True)
+    s = jx.Solver(Ad, H, p)
+    o = Oracle(Ad, H)
+    r = np.random.default_rng(4).standard_normal(Ad.n)
+    z = dev(np.zeros(Ad.n))
+    s.bilu_solve(dev(r).data_ptr(), z.data_ptr())
+    assert relerr(host(z), o.bilu0_solve(r)) < (OP_TOL if prec == 0 else 1e-5)
+    x, st, hist = s.solve(bd)
+    xo, sto, _, _ = o.gmres(bd, o.params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]), tol=meta["rtol"],
+                                         stage2_maxit=1, stage2_solver_type=5))
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    print(f"[gpu] {name} bilu0: iters {st.iterations} (oracle {sto.iterations}) true_rel {true:.3e}", flush=True)
+    assert st.converged == 1 and true <= meta["rtol"]
+    assert abs(st.iterations - sto.iterations) <= (0 if prec == 0 else 2)
+    if prec == 0:
+        assert relerr(x, xo) < X_TOL
+    assert s.error_flag() == 0
+    s.close()

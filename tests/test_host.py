@@ -174,3 +174,23 @@ def test_amg_setup_rejects_diagonal_not_first():
     a = np.array([-1.0, 2.0, 2.0, -1.0])
     with pytest.raises(RuntimeError):
         jx.amg_setup(ia, ja, a)
+
+
+@pytest.mark.parametrize("name", ["c1", "b4", "b2", "nodec"])
+def test_bilu0_factor_matches_oracle(name):
+    from oracle.cpu_ref import Oracle
+    from tests._common import system
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    lu, dinv = jx.bilu0_factor(Ad)
+    lu_o, dinv_o = Oracle(Ad).bilu0()
+    assert np.array_equal(lu, lu_o) and np.array_equal(dinv, dinv_o)
+
+
+def test_bilu0_factor_rejects_unordered_rows():
+    A, rhs = jx.generate(nx=4, ny=4, nz=2, block_size=3, seed=1)
+    ja = A.ja.copy()
+    k0 = A.ia[5]
+    ja[k0 + 1], ja[k0 + 2] = ja[k0 + 2], ja[k0 + 1]
+    B = jx.BSRMatrix(A.nb, A.b, A.ia, ja, A.a)
+    with pytest.raises(RuntimeError):
+        jx.bilu0_factor(B)

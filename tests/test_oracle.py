@@ -232,3 +232,74 @@ def test_vcycle_is_linear_and_reduces_error():
     Ap = sp.csr_matrix((a, ja, ia))
     e = np.linalg.norm(f1 - Ap @ u1) / np.linalg.norm(f1)
     assert e < 0.5
+
+
+def _bsr_parts(A, lu):
+    """(L unit block lower, U block upper incl. diagonal) as scipy CSR."""
+    b = A.b
+    rows = np.repeat(np.arange(A.nb), np.diff(A.ia))
+    blocks = lu.reshape(-1, b, b)
+    lower = A.ja < rows
+    Lb = sp.bsr_matrix((blocks[lower], A.ja[lower], np.concatenate([[0], np.cumsum(np.bincount(rows[lower],
+                        minlength=A.nb))])), shape=(A.n, A.n)).tocsr() + sp.identity(A.n, format="csr")
+    up = ~lower
+    Ub = sp.bsr_matrix((blocks[up], A.ja[up], np.concatenate([[0], np.cumsum(np.bincount(rows[up],
+                        minlength=A.nb))])), shape=(A.n, A.n)).tocsr()
+    return Lb, Ub
+
+
+def test_bilu0_is_exact_lu_without_fill():
+    """A block-tridiagonal matrix has no fill: BILU(0) is its exact block LU,
+    so one BILU(0) solve is A^{-1} r."""
+    import jxpamg_all_amd as jx
+    rng = np.random.default_rng(3)
+    nb, b = 40, 3
+    ia, ja, blocks = [0], [], []
+    for i in range(nb):
+        cols = [i] + [j for j in (i - 1, i + 1) if 0 <= j < nb]
+        for j in cols:
+            blk = rng.uniform(-1, 1, (b, b))
+            if j == i:
+                blk += 8 * np.eye(b)
+            ja.append(j)
+            blocks.append(blk)
+        ia.append(len(ja))
+    A = jx.BSRMatrix(nb, b, np.array(ia), np.array(ja), np.array(blocks).ravel())
+    o = Oracle(A)
+    r = rng.standard_normal(A.n)
+    z = o.bilu0_solve(r)
+    assert relerr(z, spla.spsolve(A.to_scipy().tocsc(), r)) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["c1", "b4", "b2"])
+def test_bilu0_product_matches_a_on_its_pattern(name):
+    """ILU(0) defining property: (L U)_ij = A_ij on every stored block."""
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    As = Ad  # diagonal block first, then ascending block columns
+    o = Oracle(As)
+    lu, dinv = o.bilu0()
+    Lb, Ub = _bsr_parts(As, lu)
+    prod = (Lb @ Ub).tocsr()
+    S = As.to_scipy().tocsr()
+    mask = S.copy()
+    mask.data[:] = 1.0
+    on_pattern = prod.multiply(mask)
+    assert abs(on_pattern - S).max() <= 1e-12 * abs(S).max()
+    # dinv holds the inverted diagonal blocks of U
+    b = As.b
+    Ud = Ub.toarray()
+    for i in range(0, As.nb, max(1, As.nb // 17)):
+        D = Ud[i * b:(i + 1) * b, i * b:(i + 1) * b]
+        assert np.allclose(dinv.reshape(-1, b, b)[i] @ D, np.eye(b), atol=1e-12)
+    # the solve applies (L U)^{-1}
+    r = np.random.default_rng(1).standard_normal(As.n)
+    z = o.bilu0_solve(r)
+    assert relerr(Lb @ (Ub @ z), r) < 1e-12
+
+
+def test_cpr_with_bilu0_stage2_converges():
+    d, A, rhs, Ad, bd, H, meta = system("lvl3")
+    o = Oracle(Ad, H)
+    x, st, hist, rc = o.gmres(bd, o.params(tol=1e-8, stage2_maxit=1, stage2_solver_type=5))
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    assert rc == 0 and st.converged == 1 and true <= 1e-8
