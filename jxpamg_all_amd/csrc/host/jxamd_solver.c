@@ -266,9 +266,10 @@ static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *
     return 0;
 }
 
-/* Per slice of a schedule: the distinct slices holding any neighbour of its
- * rows, and the slice itself. */
-static int build_nbhd(int n, const int *ia, const int *ja, const hsched *sc, int **optr, int **olist)
+/* Per slice of a schedule (rows by position in perm): the distinct slices
+ * holding any neighbour of its rows, and the slice itself. */
+static int build_nbhd(int n, const int *ia, const int *ja, const hsched *sc, const int *perm, int **optr,
+                      int **olist)
 {
     const int nsl = sc->nsl;
     int *sl = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
@@ -278,12 +279,12 @@ static int build_nbhd(int n, const int *ia, const int *ja, const hsched *sc, int
     int *list = (int *)malloc(sizeof(int) * cap);
     if (!sl || !mark || !ptr || !list) { free(sl); free(mark); free(ptr); free(list); return JXAMD_ERROR_MEMORY; }
     for (int p = 0; p < sc->npos; p++)
-        if (sc->perm[p] >= 0) sl[sc->perm[p]] = p / SELLW;
+        if (perm[p] >= 0) sl[perm[p]] = p / SELLW;
     for (int q = 0; q < nsl; q++) mark[q] = -1;
     for (int q = 0; q < nsl; q++) {
         ptr[q] = (int)used;
         for (int l = -1; l < SELLW; l++) {
-            int i = l < 0 ? -1 : sc->perm[q * SELLW + l];
+            int i = l < 0 ? -1 : perm[q * SELLW + l];
             if (l >= 0 && i < 0) continue;
             int k0 = l < 0 ? 0 : ia[i], k1 = l < 0 ? 1 : ia[i + 1];
             for (int k = k0; k < k1; k++) {
@@ -768,7 +769,7 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         if (!rc) rc = zero_ints(&d->flag, (size_t)sc.nsl * d->K, s->stream);
         if (!rc && d->K > 1) {
             int *nptr = NULL, *nlist = NULL, *ord = NULL;
-            rc = build_nbhd(nb, A->ia, A->ja, &sc, &nptr, &nlist);
+            rc = build_nbhd(nb, A->ia, A->ja, &sc, perm, &nptr, &nlist);
             if (!rc) rc = build_vorder(&sc, nptr, nlist, d->K, &ord);
             if (!rc) rc = upload_ints(&d->gall_ptr, nptr, (size_t)sc.nsl + 1, s->stream);
             if (!rc) rc = upload_ints(&d->gall_list, nlist, (size_t)nptr[sc.nsl], s->stream);

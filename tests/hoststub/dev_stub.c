@@ -1,0 +1,93 @@
+/*
+ * dev_stub.c -- TEST INFRASTRUCTURE: a host-memory stand-in for the device
+ * layer (jxamd_dev.h) so the solver's host code (layouts, schedules, level
+ * builds, argument checks, the GMRES control loop) runs on a machine without
+ * a GPU, under AddressSanitizer.  Kernel launchers do nothing and return
+ * success; device memory is host memory.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "jxamd_dev.h"
+
+int jxamd_dev_count(int *n) { *n = 1; return 0; }
+int jxamd_dev_malloc(void **p, size_t bytes) { *p = calloc(1, bytes ? bytes : 1); return *p ? 0 : JXAMD_ERROR_MEMORY; }
+int jxamd_dev_free(void *p) { free(p); return 0; }
+int jxamd_h2d(void *dst, const void *src, size_t bytes, void *stream) { (void)stream; if (bytes) memcpy(dst, src, bytes); return 0; }
+int jxamd_d2h(void *dst, const void *src, size_t bytes, void *stream) { (void)stream; if (bytes) memcpy(dst, src, bytes); return 0; }
+int jxamd_d2d(void *dst, const void *src, size_t bytes, void *stream) { (void)stream; if (bytes) memmove(dst, src, bytes); return 0; }
+static int token;
+int jxamd_stream_create(void **s) { *s = &token; return 0; }
+int jxamd_stream_destroy(void *s) { (void)s; return 0; }
+int jxamd_stream_sync(void *s) { (void)s; return 0; }
+int jxamd_event_create(void **e) { *e = &token; return 0; }
+int jxamd_event_destroy(void *e) { (void)e; return 0; }
+int jxamd_event_record(void *e, void *s) { (void)e; (void)s; return 0; }
+int jxamd_event_elapsed(float *ms, void *e0, void *e1) { (void)e0; (void)e1; *ms = 0.f; return 0; }
+int jxamd_last_error(void) { return 0; }
+
+int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
+                     const double *rhs, double *y, int mode, double alpha, double beta, void *stream)
+{ (void)b; (void)nb; (void)sptr; (void)col; (void)val; (void)x; (void)rhs; (void)y; (void)mode; (void)alpha; (void)beta; (void)stream; return 0; }
+int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
+                     const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream)
+{ (void)vt; (void)xt; (void)n; (void)sptr; (void)col; (void)val; (void)rsc; (void)x; (void)xs; (void)rhs; (void)rs; (void)y; (void)mode; (void)stream; return 0; }
+int jxamd_sweep_waves(int nsl) { return nsl < 2048 ? (nsl > 0 ? nsl : 1) : 2048; }
+int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
+                     const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                     const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
+                     unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+{ (void)vt; (void)xt; (void)nsl; (void)perm; (void)sptr; (void)col; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)dep_ptr; (void)dep_list; (void)sflag; (void)epoch; (void)dir; (void)ticket; (void)base; (void)err; (void)stream; return 0; }
+int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,
+                       const double *val, const double *dinv, const double *rhs, const double *x0, const double *xp,
+                       int pidx, double *xs, long long xstride, double *xout, const int *dlo_ptr,
+                       const int *dlo_list, const int *dall_ptr, const int *dall_list, int *sflag, int epoch,
+                       double w, double omega, unsigned long long *ticket, unsigned long long *base, int *err,
+                       void *stream)
+{
+    (void)b; (void)perm; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)x0; (void)xp; (void)pidx;
+    (void)xstride; (void)xout; (void)dlo_ptr; (void)dlo_list; (void)sflag; (void)epoch; (void)w; (void)omega;
+    (void)ticket; (void)base; (void)err; (void)stream;
+    if (K > 1 && (!vorder || !xs || !dall_ptr || !dall_list)) return JXAMD_ERROR_ARG;
+    (void)nsl;
+    return 0;
+}
+int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
+                 const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
+                 int epoch, unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+{ (void)b; (void)vt; (void)nsl; (void)perm; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)out; (void)dep_ptr; (void)dep_list; (void)sflag; (void)epoch; (void)ticket; (void)base; (void)err; (void)stream; return 0; }
+int jxamd_gs_chunk_lds_rows(int n, int nchunks)
+{
+    if (nchunks < 1 || n < nchunks) return 0;
+    return n / nchunks + (n % nchunks ? 1 : 0);
+}
+int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, const int *col, const void *val,
+                         const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                         void *stream)
+{ (void)vt; (void)xt; (void)n; (void)nchunks; (void)sptr; (void)col; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
+int jxamd_k_gs_chunk(int vt, int xt, int n, int nchunks, const int *wstep, const int *sptr, const int *col,
+                     const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
+                     void *un, void *stream)
+{ (void)vt; (void)xt; (void)n; (void)nchunks; (void)wstep; (void)sptr; (void)col; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
+int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream)
+{ (void)xt; (void)n; (void)M; (void)x; (void)y; (void)stream; return 0; }
+int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream)
+{ (void)n; (void)x; (void)stride; (void)off; (void)y; (void)stream; return 0; }
+int jxamd_k_widen(int n, const float *x, double *y, void *stream) { (void)n; (void)x; (void)y; (void)stream; return 0; }
+int jxamd_k_fill_f32(int n, float *y, float v, void *stream) { (void)n; (void)y; (void)v; (void)stream; return 0; }
+int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream)
+{ (void)n; (void)x; (void)stride; (void)off; (void)y; (void)stream; return 0; }
+int jxamd_k_prolong(int nb, int b, int pidx, const double *xp, double *x, void *stream)
+{ (void)nb; (void)b; (void)pidx; (void)xp; (void)x; (void)stream; return 0; }
+int jxamd_k_fill(size_t n, double *y, double v, void *stream) { (void)n; (void)y; (void)v; (void)stream; return 0; }
+int jxamd_k_copy(size_t n, const double *x, double *y, void *stream) { (void)n; (void)x; (void)y; (void)stream; return 0; }
+int jxamd_k_axpy(size_t n, const double *pa, double a, int neg, const double *x, double *y, void *stream)
+{ (void)n; (void)pa; (void)a; (void)neg; (void)x; (void)y; (void)stream; return 0; }
+int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *stream)
+{ (void)n; (void)pnorm2; (void)a; (void)y; (void)stream; return 0; }
+int jxamd_dot_grid(size_t n) { (void)n; return 1; }
+int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
+                    unsigned int *count, double *out, void *stream)
+{ (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }
+int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream)
+{ (void)n; (void)m; (void)V; (void)ld; (void)y; (void)w; (void)stream; return 0; }

@@ -1,0 +1,62 @@
+/*
+ * drive.c -- TEST INFRASTRUCTURE: builds solvers for a generated system under
+ * every layout / precision / stage-2 option with the host-memory device stub,
+ * so AddressSanitizer checks the host layout code.  Exit status 0 = all
+ * creations and destructions succeeded.
+ *   drive nx ny nz b
+ */
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "jxamd.h"
+
+int main(int argc, char **argv)
+{
+    if (argc < 5) return 2;
+    jxamd_gen_params g = {atoi(argv[1]), atoi(argv[2]), atoi(argv[3]), atoi(argv[4]), JXAMD_FIELD_LAYERED, 1.0, 3.0,
+                          2, 100.0, 0.1, 2, 1e3, 7};
+    int nb, nnzb;
+    if (jxamd_gen_sizes(&g, &nb, &nnzb)) return 3;
+    const int b = g.block_size, bb = b * b;
+    int *ia = malloc(sizeof(int) * (nb + 1)), *ja = malloc(sizeof(int) * nnzb);
+    double *a = malloc(sizeof(double) * (size_t)nnzb * bb), *rhs = malloc(sizeof(double) * nb * b);
+    double *x = calloc((size_t)nb * b, sizeof(double));
+    if (jxamd_gen_reservoir(&g, ia, ja, a, rhs)) return 4;
+    if (jxamd_bsr_decouple_abf(nb, b, ia, ja, a, rhs)) return 5;
+    int *pia = malloc(sizeof(int) * (nb + 1)), *pja = malloc(sizeof(int) * nnzb);
+    double *pa = malloc(sizeof(double) * nnzb);
+    if (jxamd_bsr_extract_pressure(nb, b, ia, ja, a, 0, pia, pja, pa)) return 6;
+    jxamd_csr A0 = {nb, nb, nnzb, pia, pja, pa};
+    jxamd_amg_params ap;
+    jxamd_amg_params_default(&ap);
+    jxamd_hierarchy H;
+    if (jxamd_amg_setup(&A0, &ap, &H)) return 7;
+    jxamd_bsr A = {nb, nb, nnzb, b, ia, ja, a};
+    const int chunks[] = {0, 1, 3, 16, 40};
+    int fails = 0, runs = 0;
+    for (int ci = 0; ci < 5; ci++)
+        for (int prec = 0; prec <= 2; prec++)
+            for (int s2 = 0; s2 < 4; s2++) {
+                jxamd_params p;
+                jxamd_params_default(&p);
+                p.relax_chunk = chunks[ci];
+                p.precond_precision = prec;
+                p.stage2_solver_type = s2 < 3 ? 2 : 5;
+                p.stage2_maxit = s2 < 3 ? s2 + 1 : 1;
+                if (ci == 4) { p.relax_chunk = 0; p.relax_threads = 4; }
+                jxamd_solver *s = NULL;
+                int rc = jxamd_solver_create(&s, &A, &H, &p);
+                jxamd_stats st;
+                if (!rc) rc = jxamd_solver_solve(s, rhs, x, NULL, &st);
+                if (s) jxamd_solver_destroy(s);
+                runs++;
+                if (rc) {
+                    fails++;
+                    fprintf(stderr, "chunk %d prec %d stage2 %d: rc %d\n", chunks[ci], prec, s2, rc);
+                }
+            }
+    jxamd_hierarchy_free(&H);
+    free(ia); free(ja); free(a); free(rhs); free(x); free(pia); free(pja); free(pa);
+    printf("levels %d runs %d fails %d\n", H.num_levels, runs, fails);
+    return fails ? 1 : 0;
+}
