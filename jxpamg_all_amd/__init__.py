@@ -152,8 +152,10 @@ CONFIGS = {
 
 
 # rows per relax-6 chunk on the GPU for the large workloads: the reference's
-# threaded hybrid SSOR (par_relax_6.c) with ceil(n_l / 16) chunks on level l
-BENCH_RELAX_CHUNK = 16
+# threaded hybrid SSOR (par_relax_6.c) with ceil(n_l / 4) chunks on level l
+# (on the 60x220x85 system: 9 iterations against 8 for one thread, and the
+# fastest of chunk lengths 1..16)
+BENCH_RELAX_CHUNK = 4
 
 
 def gen_params(**kw):
