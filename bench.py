@@ -119,7 +119,7 @@ def main():
     n_l, ms_l, by_l = prof[dom]
     per_launch_ms = ms_l / n_l
     achieved = (by_l / n_l) / (per_launch_ms * 1e-3) / 1e9
-    kernel_names = {"gs_sweep": "k_gs_sweep", "bhgs_sweep": "k_bhgs_sweep", "bsr_spmv": "k_bsr_spmv",
+    kernel_names = {"gs_sweep": "k_gs_chunk_lds" if chunk else "k_gs_sweep", "bhgs_sweep": "k_bhgs_multi", "bsr_spmv": "k_bsr_spmv",
                     "csr_spmv": "k_csr_spmv", "coarse_solve": "k_dense_gemv", "mgs": "k_dot_mgs",
                     "vector_ops": "k_scale/k_axpy/k_lincomb", "bilu_solve": "k_btri"}
     traffic = read_traffic(kernel_names[dom])
