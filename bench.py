@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=2, help="GMRES iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run three plain device copies of MIB MiB (counter calibration) and exit")
     ap.add_argument("--relax-chunk", type=int, default=None,
                     help="rows per relax-6 chunk (0: one-thread sweeps); default: the package's large-workload value")
     return ap.parse_args()
@@ -64,6 +66,16 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        n = args.copy_probe * 1024 * 1024 // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {n * 8} bytes read and written")
+        return
 
     import jxpamg_all_amd as jx
 
