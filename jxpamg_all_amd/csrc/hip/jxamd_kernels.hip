@@ -32,9 +32,19 @@ __device__ __forceinline__ int ld_flag(const int *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Publishing a slice.  Every value another wave reads back in the same
+// launch is written with a device-scope (agent) atomic store, which is
+// written through to the device coherence point, and read with a device-scope
+// atomic load, which is served from it; so no L2 write-back or invalidation
+// is needed, only completion order: wait until this wave's stores are
+// acknowledged (vmcnt 0 covers stores on gfx9-family CUs), then raise the
+// flag.  The workgroup-scope fence keeps the compiler from moving memory
+// operations across this point and emits no cache maintenance.
 __device__ __forceinline__ void st_flag_release(int *p, int v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double ld_coherent(const double *p)
 {
@@ -188,7 +198,10 @@ __device__ __forceinline__ void wait_slices(const int *__restrict__ dep_ptr, con
             else __builtin_amdgcn_s_sleep(4);
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // The flag loads have returned (the loop waited on them); the dependent
+    // data is read afterwards with device-scope atomic loads (see
+    // st_flag_release), so only a compiler barrier is needed here.
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // One Gauss-Seidel sweep of hybrid SSOR (relax 6, weights 1).
