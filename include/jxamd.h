@@ -151,6 +151,13 @@ typedef struct {
      * accumulation; 2 FP16 level matrices with a power-of-two scale per row,
      * FP32 vectors and accumulation.  Stage 2 and GMRES stay FP64. */
     int precond_precision;
+    /* AMG smoother: 0 = relax 6 as in the reference (two calls per pre- and
+     * post-smoothing); d = 1..8 = a degree-d polynomial smoother instead, one
+     * application per pre- and post-smoothing, u <- u + w_k D^{-1}(f - A u)
+     * for k < d (no reference counterpart).  Step sizes default to Chebyshev
+     * roots on [0.3, 1.1] x an estimate of lambda_max(D^{-1} A) per level and
+     * can be replaced with jxamd_solver_set_poly (e.g. learned ones). */
+    int poly_degree;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread
@@ -198,6 +205,13 @@ int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x);
 int jxamd_solver_interface(int argc, char *argv[], int row, int *ia, int *ja, double *a, double *ser_x,
                            double *ser_b, double tol, int solver_id, int num_functions, int *iter);
 
+/* Polynomial smoother step sizes of one level (degree must equal
+ * poly_degree); get returns the ones in use and the level's lambda_max
+ * estimate of D^{-1} A. */
+int jxamd_solver_set_poly(jxamd_solver *s, int level, int degree, const double *omega);
+int jxamd_solver_get_poly(jxamd_solver *s, int level, int degree, double *omega, double *lambda_max);
+int jxamd_solver_num_levels(jxamd_solver *s, int *num_levels);
+
 /* Single operators on device vectors (parity tests, benchmarks). */
 /* y = alpha*A*x + beta*y, jxf_BSRMatrixMatvec (seq_bsr_matvec.c:21-134) */
 int jxamd_op_bsr_matvec(jxamd_solver *s, double alpha, const double *d_x, double beta, double *d_y);
@@ -222,7 +236,7 @@ int jxamd_cpr_precond_timed(jxamd_solver *s, const double *d_r, double *d_x, int
  * together with its algorithmic HBM bytes (the bytes the algorithm must move
  * once: matrix entries, indices, each vector read or written once). */
 enum {
-    JXAMD_PROF_GS_SWEEP = 0,   /* scalar hybrid-GS sweeps, all levels */
+    JXAMD_PROF_GS_SWEEP = 0,   /* AMG smoother (relax-6 sweeps or polynomial steps), all levels */
     JXAMD_PROF_BHGS_SWEEP = 1, /* stage-2 block sweeps */
     JXAMD_PROF_BSR_SPMV = 2,   /* y = A x, residuals */
     JXAMD_PROF_CSR_SPMV = 3,   /* level residual, restriction, prolongation */

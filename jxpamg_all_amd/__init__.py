@@ -56,7 +56,7 @@ class Params(ctypes.Structure):
                 ("print_level", ctypes.c_int),
                 ("threshold", ctypes.c_double), ("stage2_weight", ctypes.c_double),
                 ("stage2_omega", ctypes.c_double), ("relax_threads", ctypes.c_int), ("relax_chunk", ctypes.c_int),
-                ("precond_precision", ctypes.c_int)]
+                ("precond_precision", ctypes.c_int), ("poly_degree", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -113,6 +113,9 @@ def _declare(L):
         "jxamd_cpr_precond_timed": [vp, vp, vp, i32, vp],
         "jxamd_solver_interface": [i32, vp, i32, vp, vp, vp, vp, vp, f64, i32, i32, vp],
         "jxamd_op_bilu_solve": [vp, vp, vp],
+        "jxamd_solver_set_poly": [vp, i32, i32, vp],
+        "jxamd_solver_get_poly": [vp, i32, i32, vp, vp],
+        "jxamd_solver_num_levels": [vp, vp],
         "jxamd_bilu0_factor": [i32, i32, vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
@@ -403,6 +406,22 @@ class Solver:
 
     def vcycle(self, d_f, d_u):
         check(self._L.jxamd_op_vcycle(self._s, ctypes.c_void_p(d_f), ctypes.c_void_p(d_u)), "jxamd_op_vcycle")
+
+    @property
+    def num_levels(self):
+        n = ctypes.c_int()
+        check(self._L.jxamd_solver_num_levels(self._s, ctypes.byref(n)), "jxamd_solver_num_levels")
+        return n.value
+
+    def set_poly(self, level, omega):
+        w = np.ascontiguousarray(omega, dtype=np.float64)
+        check(self._L.jxamd_solver_set_poly(self._s, level, len(w), _p(w)), "jxamd_solver_set_poly")
+
+    def get_poly(self, level, degree):
+        w = np.zeros(degree)
+        lam = ctypes.c_double()
+        check(self._L.jxamd_solver_get_poly(self._s, level, degree, _p(w), ctypes.byref(lam)), "jxamd_solver_get_poly")
+        return w, lam.value
 
     def bilu_solve(self, d_r, d_z):
         check(self._L.jxamd_op_bilu_solve(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_z)), "jxamd_op_bilu_solve")

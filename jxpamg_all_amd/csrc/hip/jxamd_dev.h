@@ -35,6 +35,8 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
 #define JXAMD_T_F16 2
 int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
                      const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream);
+int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
+                      const void *dinv, const void *f, int fs, const void *uo, void *un, double omega, void *stream);
 int jxamd_sweep_waves(int nsl);
 int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                      const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,

@@ -160,6 +160,33 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
     else y[row] += acc;
 }
 
+// One step of the polynomial smoother on a level (SELL-64, full rows):
+//   un_i = uo_i + omega * dinv_i * (f_i - sum_j a_ij uo_j)
+// uo == nullptr stands for zeros.  dinv_i = 0 keeps a row at its old value.
+template <typename V, typename X>
+__global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                   const V *__restrict__ val, const float *__restrict__ rsc,
+                                                   const X *__restrict__ dinv, const X *__restrict__ f, int fs,
+                                                   const X *__restrict__ uo, X *__restrict__ un, X omega)
+{
+    const int s = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int nsl = (n + WAVE - 1) / WAVE;
+    if (s >= nsl) return;
+    const int row = s * WAVE + lane;
+    X acc = 0;
+    if (uo) {
+        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+            const size_t e = (size_t)k * WAVE + lane;
+            acc += (X)cvt(val[e]) * uo[col[e]];
+        }
+    }
+    if (row >= n) return;
+    if (row_scaled<V>::value) acc *= (X)rsc[row];
+    const X r = f[(size_t)row * fs] - acc;
+    un[row] = (uo ? uo[row] : (X)0) + omega * dinv[row] * r;
+}
+
 // ---------------------------------------------------------------------------
 // Sync-free sweeps over level-pure wavefront slices.
 //
@@ -835,6 +862,19 @@ int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, con
     DISPATCH_VX(vt, xt,
                 (k_csr_spmv<V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
                     n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y, mode)));
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
+                      const void *dinv, const void *f, int fs, const void *uo, void *un, double omega, void *stream)
+{
+    const int nsl = (n + WAVE - 1) / WAVE;
+    const int grid = (nsl + 3) / 4;
+    if (grid == 0) return 0;
+    DISPATCH_VX(vt, xt,
+                (k_poly_step<V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                    n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs, (const X_ *)uo,
+                    (X_ *)un, (X_)omega)));
     return chk(hipGetLastError());
 }
 

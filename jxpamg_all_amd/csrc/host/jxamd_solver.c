@@ -96,6 +96,9 @@ struct jxamd_solver_s {
     void *F[JXAMD_MAX_LEVELS], *U[JXAMD_MAX_LEVELS], *T[JXAMD_MAX_LEVELS], *Vt[JXAMD_MAX_LEVELS];
     double *up64;          /* FP64 copy of the stage-1 result when xt != F64 */
     dtri Lf, Ub;           /* stage-2 BILU(0) factors (stage2_solver_type 5) */
+    void *dinvl[JXAMD_MAX_LEVELS];          /* polynomial smoother: 1/a_ii (0 if a_ii = 0), type xt */
+    double poly_w[JXAMD_MAX_LEVELS][8];     /* its step sizes */
+    double lam[JXAMD_MAX_LEVELS];           /* lambda_max(D^{-1} A_l) estimate */
     double *bz;            /* BILU(0) intermediate vector, N */
     double *xtmp;          /* stage-2 ping-pong partner, N */
     double *V;             /* (k_dim+1) Krylov vectors, N each */
@@ -895,6 +898,109 @@ static int build_bilu(jxamd_solver *s, const jxamd_bsr *A)
     return e;
 }
 
+/* level l's transfer operators (P, R = P^T) or, on the coarsest level, the
+ * explicit inverse */
+static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
+{
+    const jxamd_csr *A = &H->A[l];
+    const int n = A->n;
+    hsell h;
+    int e;
+    if (l < s->L - 1) {
+        /* interpolation P (n x nc) and restriction R = P^T (nc x n) */
+        const jxamd_csr *Pm = &H->P[l];
+        e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
+        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL, s->vt, s->xt);
+        /* y += P x: x once, y read and written */
+        s->P[l].abytes = level_bytes(&s->P[l], Pm->ia[Pm->n], Pm->n, 2.0) + (double)type_size(s->xt) * Pm->ncols;
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        CK(e);
+        int *ri, *rj;
+        double *ra;
+        CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
+        e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
+        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL, s->vt, s->xt);
+        s->R[l].abytes = level_bytes(&s->R[l], Pm->ia[Pm->n], Pm->ncols, 1.0) + (double)type_size(s->xt) * Pm->n;
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        free(ri); free(rj); free(ra);
+        CK(e);
+    } else if (s->L > 1) {
+        /* coarsest: explicit inverse of the densified matrix */
+        double *M = (double *)calloc((size_t)n * n + 1, sizeof(double));
+        if (!M) return JXAMD_ERROR_MEMORY;
+        for (int i = 0; i < n; i++)
+            for (int k = A->ia[i]; k < A->ia[i + 1]; k++) M[(size_t)i * n + A->ja[k]] = A->a[k];
+        e = dense_inverse(M, n);
+        if (!e && s->xt != JXAMD_T_F64) {
+            float *Mf = (float *)M; /* in place: element k of the float array lies before element k of the double one */
+            for (size_t k = 0; k < (size_t)n * n; k++) Mf[k] = (float)M[k];
+        }
+        if (!e) e = jxamd_dev_malloc(&s->cinv, s->xsz * ((size_t)n * n + 1));
+        if (!e) e = jxamd_h2d(s->cinv, M, s->xsz * (size_t)n * n, s->stream);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        free(M);
+        s->nc = n;
+        CK(e);
+    }
+    return 0;
+}
+
+
+/* polynomial smoother data of level l: 1/a_ii, a lambda_max(D^{-1} A)
+ * estimate by 20 power iterations from a fixed start vector, and default
+ * Chebyshev step sizes on [0.3, 1.1] x lambda */
+static int build_poly(jxamd_solver *s, int l, const jxamd_csr *A)
+{
+    const int n = A->n;
+    double *di = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double *v = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double *w = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    if (!di || !v || !w) { free(di); free(v); free(w); return JXAMD_ERROR_MEMORY; }
+    for (int i = 0; i < n; i++) {
+        const double d = A->a[A->ia[i]];
+        di[i] = d != 0.0 ? 1.0 / d : 0.0;
+    }
+    uint64_t st = 0x9e3779b97f4a7c15ull + (uint64_t)l;
+    for (int i = 0; i < n; i++) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        v[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    double lam = 1.0;
+    for (int it = 0; it < 20; it++) {
+        double nv = 0.0, nw = 0.0;
+        for (int i = 0; i < n; i++) {
+            double acc = 0.0;
+            for (int k = A->ia[i]; k < A->ia[i + 1]; k++) acc += A->a[k] * v[A->ja[k]];
+            w[i] = di[i] * acc;
+            nv += v[i] * v[i];
+            nw += w[i] * w[i];
+        }
+        if (nw == 0.0 || nv == 0.0) break;
+        lam = sqrt(nw / nv);
+        const double sc = 1.0 / sqrt(nw);
+        for (int i = 0; i < n; i++) v[i] = w[i] * sc;
+    }
+    s->lam[l] = lam;
+    const int m = s->prm.poly_degree;
+    const double lo = 0.3 * lam, hi = 1.1 * lam, c = 0.5 * (hi + lo), d = 0.5 * (hi - lo);
+    for (int k = 0; k < m; k++) s->poly_w[l][k] = 1.0 / (c + d * cos(M_PI * (2.0 * k + 1.0) / (2.0 * m)));
+    int e;
+    if (s->xt == JXAMD_T_F64) {
+        e = jxamd_dev_malloc(&s->dinvl[l], sizeof(double) * (n > 0 ? n : 1));
+        if (!e) e = jxamd_h2d(s->dinvl[l], di, sizeof(double) * n, s->stream);
+    } else {
+        float *f = (float *)v; /* reuse: float view of the scratch vector */
+        for (int i = 0; i < n; i++) f[i] = (float)di[i];
+        e = jxamd_dev_malloc(&s->dinvl[l], sizeof(float) * (n > 0 ? n : 1));
+        if (!e) e = jxamd_h2d(s->dinvl[l], f, sizeof(float) * n, s->stream);
+    }
+    if (!e) e = jxamd_stream_sync(s->stream);
+    free(di); free(v); free(w);
+    return e;
+}
+
 static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
 {
     const jxamd_csr *A = &H->A[l];
@@ -909,6 +1015,10 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     CK(e);
     /* CSR SpMV bytes: nnz (value + index) + row pointer + x + rhs + y */
     s->Ares[l].abytes = level_bytes(&s->Ares[l], A->ia[n], n, 3.0);
+    if (s->prm.poly_degree > 0 && (l < s->L - 1 || s->L == 1)) {
+        CK(build_poly(s, l, A));
+        return build_transfer(s, l, H);
+    }
     int T = 0;
     if (s->prm.relax_threads > 1) T = s->prm.relax_threads;
     else if (s->prm.relax_chunk > 0) T = (n + s->prm.relax_chunk - 1) / s->prm.relax_chunk;
@@ -973,45 +1083,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
             (dir > 0 ? &s->Gf[l] : &s->Gb[l])->abytes = level_bytes(dir > 0 ? &s->Gf[l] : &s->Gb[l], A->ia[n], n, 3.0);
         }
     }
-    if (l < s->L - 1) {
-        /* interpolation P (n x nc) and restriction R = P^T (nc x n) */
-        const jxamd_csr *Pm = &H->P[l];
-        e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL, s->vt, s->xt);
-        /* y += P x: x once, y read and written */
-        s->P[l].abytes = level_bytes(&s->P[l], Pm->ia[Pm->n], Pm->n, 2.0) + (double)type_size(s->xt) * Pm->ncols;
-        if (!e) e = jxamd_stream_sync(s->stream);
-        hsell_free(&h);
-        CK(e);
-        int *ri, *rj;
-        double *ra;
-        CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
-        e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL, s->vt, s->xt);
-        s->R[l].abytes = level_bytes(&s->R[l], Pm->ia[Pm->n], Pm->ncols, 1.0) + (double)type_size(s->xt) * Pm->n;
-        if (!e) e = jxamd_stream_sync(s->stream);
-        hsell_free(&h);
-        free(ri); free(rj); free(ra);
-        CK(e);
-    } else if (s->L > 1) {
-        /* coarsest: explicit inverse of the densified matrix */
-        double *M = (double *)calloc((size_t)n * n + 1, sizeof(double));
-        if (!M) return JXAMD_ERROR_MEMORY;
-        for (int i = 0; i < n; i++)
-            for (int k = A->ia[i]; k < A->ia[i + 1]; k++) M[(size_t)i * n + A->ja[k]] = A->a[k];
-        e = dense_inverse(M, n);
-        if (!e && s->xt != JXAMD_T_F64) {
-            float *Mf = (float *)M; /* in place: element k of the float array lies before element k of the double one */
-            for (size_t k = 0; k < (size_t)n * n; k++) Mf[k] = (float)M[k];
-        }
-        if (!e) e = jxamd_dev_malloc(&s->cinv, s->xsz * ((size_t)n * n + 1));
-        if (!e) e = jxamd_h2d(s->cinv, M, s->xsz * (size_t)n * n, s->stream);
-        if (!e) e = jxamd_stream_sync(s->stream);
-        free(M);
-        s->nc = n;
-        CK(e);
-    }
-    return 0;
+    return build_transfer(s, l, H);
 }
 
 int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hierarchy *H, const jxamd_params *p)
@@ -1025,6 +1097,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->relax_threads < 0 || p->relax_chunk < 0) return JXAMD_ERROR_ARG;
     if (p->precond_precision < 0 || p->precond_precision > 2) return JXAMD_ERROR_ARG;
     if (p->stage2_solver_type != 2 && p->stage2_solver_type != 5) return JXAMD_ERROR_ARG;
+    if (p->poly_degree < 0 || p->poly_degree > 8) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -1116,6 +1189,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
         dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]); dsell_free(&s->Gc[l]);
         jxamd_dev_free(s->wstep[l]);
+        jxamd_dev_free(s->dinvl[l]);
         dsell_free(&s->R[l]); dsell_free(&s->P[l]);
         jxamd_dev_free(s->F[l]); jxamd_dev_free(s->U[l]); jxamd_dev_free(s->T[l]); jxamd_dev_free(s->Vt[l]);
     }
@@ -1181,10 +1255,31 @@ static int hsgs_chunk(jxamd_solver *s, int l, const void *f, int fs, const void 
     return 0;
 }
 
-/* the two relax-6 calls of one level visit (relax_order 1, par_relax_if.c:36-60);
- * the result stays in u */
+/* one application of the level's polynomial smoother; steps alternate
+ * between T and Vt and the last one lands in u */
+static int poly_apply(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
+{
+    const dsell *A = &s->Ares[l];
+    const int m = s->prm.poly_degree;
+    const void *src = uo_zero ? NULL : u;
+    for (int k = 0; k < m; k++) {
+        void *dst = (k == m - 1 && m > 1) ? u : ((k % 2 == 0) ? s->T[l] : s->Vt[l]);
+        int pq = prof_begin(s);
+        CK(jxamd_k_poly_step(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs, src, dst,
+                             s->poly_w[l][k], s->stream));
+        /* a step from zeros reads only f and dinv */
+        prof_end(s, pq, JXAMD_PROF_GS_SWEEP, src ? A->abytes : 3.0 * (double)s->xsz * A->n);
+        src = dst;
+    }
+    if (m == 1) return jxamd_d2d(u, s->T[l], s->xsz * s->nlev[l], s->stream);
+    return 0;
+}
+
+/* the two relax-6 calls of one level visit (relax_order 1, par_relax_if.c:36-60),
+ * or one polynomial smoother application; the result stays in u */
 static int relax_pair(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
+    if (s->prm.poly_degree > 0) return poly_apply(s, l, f, fs, u, uo_zero);
     if (s->nchunks[l] > 1) {
         CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
         return hsgs_chunk(s, l, f, fs, s->T[l], u);
@@ -1193,9 +1288,10 @@ static int relax_pair(jxamd_solver *s, int l, const void *f, int fs, void *u, in
     return hsgs(s, l, f, fs, u, 0);
 }
 
-/* a single relax-6 call; the result stays in u */
+/* a single relax-6 call (or polynomial smoother application); the result stays in u */
 static int relax_once(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
+    if (s->prm.poly_degree > 0) return poly_apply(s, l, f, fs, u, uo_zero);
     if (s->nchunks[l] > 1) {
         CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
         return jxamd_d2d(u, s->T[l], s->xsz * s->nlev[l], s->stream);
@@ -1403,6 +1499,37 @@ int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u)
     CK(vcycle(s, s->F[0], 1, 0));
     CK(from_level(s, n, s->U[0], d_u));
     return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_solver_num_levels(jxamd_solver *s, int *num_levels)
+{
+    if (!s || !num_levels) return JXAMD_ERROR_ARG;
+    *num_levels = s->L;
+    return 0;
+}
+
+static int poly_level_ok(const jxamd_solver *s, int level, int degree)
+{
+    return s && s->prm.poly_degree > 0 && degree == s->prm.poly_degree && level >= 0 && level < s->L &&
+           (level < s->L - 1 || s->L == 1);
+}
+
+int jxamd_solver_set_poly(jxamd_solver *s, int level, int degree, const double *omega)
+{
+    if (!poly_level_ok(s, level, degree) || !omega) return JXAMD_ERROR_ARG;
+    for (int k = 0; k < degree; k++)
+        if (!(omega[k] == omega[k])) return JXAMD_ERROR_ARG; /* NaN */
+    for (int k = 0; k < degree; k++) s->poly_w[level][k] = omega[k];
+    return 0;
+}
+
+int jxamd_solver_get_poly(jxamd_solver *s, int level, int degree, double *omega, double *lambda_max)
+{
+    if (!poly_level_ok(s, level, degree)) return JXAMD_ERROR_ARG;
+    if (omega)
+        for (int k = 0; k < degree; k++) omega[k] = s->poly_w[level][k];
+    if (lambda_max) *lambda_max = s->lam[level];
+    return 0;
 }
 
 int jxamd_op_bilu_solve(jxamd_solver *s, const double *d_r, double *d_z)

@@ -173,6 +173,27 @@ void cref_relax6_threads(const cref_csr *A, const double *f, double *u, int num_
     free(tmp);
 }
 
+/* Polynomial smoother (no reference counterpart): degree Richardson steps
+ * with the Jacobi preconditioner, u <- u + w_k D^{-1} (f - A u); the error
+ * propagates by prod_k (I - w_k D^{-1} A). */
+void cref_poly_smooth(const cref_csr *A, const double *f, double *u, int degree, const double *omega)
+{
+    const int n = A->n;
+    double *r = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int k = 0; k < degree; k++) {
+        for (int i = 0; i < n; i++) {
+            double acc = f[i];
+            for (int jj = A->ia[i]; jj < A->ia[i + 1]; jj++) acc -= A->a[jj] * u[A->ja[jj]];
+            r[i] = acc;
+        }
+        for (int i = 0; i < n; i++) {
+            const double d = A->a[A->ia[i]];
+            if (d != 0.0) u[i] += omega[k] * r[i] / d;
+        }
+    }
+    free(r);
+}
+
 /* Gaussian elimination without pivoting, J/csrc/operation/matops/par_gs.c:20-74 */
 int cref_gselim(double *A, double *x, int size)
 {
@@ -383,7 +404,9 @@ static int cref_pamg_cycle(const cref_hier *H, double **F, double **U, double *v
     int level = 0, cycle_param = 1, err = 0;
     for (;;) {
         if (L > 1) {
-            if (cycle_param < 3) {
+            if (cycle_param < 3 && H->poly_degree > 0) {
+                cref_poly_smooth(&H->A[level], F[level], U[level], H->poly_degree, H->poly_omega[level]);
+            } else if (cycle_param < 3) {
                 cref_relax6_threads(&H->A[level], F[level], U[level], H->relax_threads[level]);
                 cref_relax6_threads(&H->A[level], F[level], U[level], H->relax_threads[level]);
             } else {
@@ -392,7 +415,10 @@ static int cref_pamg_cycle(const cref_hier *H, double **F, double **U, double *v
             }
         } else {
             /* no coarsening: one user-type (6) sweep, par_cycle.c:236-255 */
-            cref_relax6_threads(&H->A[level], F[level], U[level], H->relax_threads[level]);
+            if (H->poly_degree > 0)
+                cref_poly_smooth(&H->A[level], F[level], U[level], H->poly_degree, H->poly_omega[level]);
+            else
+                cref_relax6_threads(&H->A[level], F[level], U[level], H->relax_threads[level]);
         }
         --lev_counter[level];
         if (lev_counter[level] >= 0 && level != L - 1) {

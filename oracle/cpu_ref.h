@@ -24,6 +24,11 @@ typedef struct {
     cref_csr A[CREF_MAX_LEVELS]; /* A[0] = pressure matrix */
     cref_csr P[CREF_MAX_LEVELS]; /* P[l]: n_l x n_{l+1}, l < num_levels-1 */
     int relax_threads[CREF_MAX_LEVELS]; /* relax-6 thread count per level; <= 1: sequential */
+    /* poly_degree > 0: every smoothing on level l (pre and post) is instead one
+     * polynomial smoother u <- u + w_k D^{-1} (f - A u), k = 0..degree-1,
+     * with w_k = poly_omega[l][k] (no reference counterpart) */
+    int poly_degree;
+    double poly_omega[CREF_MAX_LEVELS][8];
 } cref_hier;
 
 typedef struct {
@@ -54,6 +59,7 @@ void cref_csr_matvecT(const cref_csr *A, double alpha, const double *x, double b
 double cref_dot(int n, const double *x, const double *y);
 void cref_relax6(const cref_csr *A, const double *f, double *u);
 void cref_relax6_threads(const cref_csr *A, const double *f, double *u, int num_threads);
+void cref_poly_smooth(const cref_csr *A, const double *f, double *u, int degree, const double *omega);
 int cref_relax9(const cref_csr *A, const double *f, double *u);
 int cref_gselim(double *A, double *x, int n);
 void cref_bsr_block_matvec_stable(double alpha, const double *mat, const double *v, double beta, double *ov, int b);

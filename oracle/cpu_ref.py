@@ -20,7 +20,8 @@ class CSR(ctypes.Structure):
 
 class Hier(ctypes.Structure):
     _fields_ = [("num_levels", ctypes.c_int), ("A", CSR * MAX_LEVELS), ("P", CSR * MAX_LEVELS),
-                ("relax_threads", ctypes.c_int * MAX_LEVELS)]
+                ("relax_threads", ctypes.c_int * MAX_LEVELS), ("poly_degree", ctypes.c_int),
+                ("poly_omega", (ctypes.c_double * 8) * MAX_LEVELS)]
 
 
 class BSR(ctypes.Structure):
@@ -59,6 +60,7 @@ def lib():
             ("cref_relax6", [vp, vp, vp], None),
             ("cref_relax6_threads", [vp, vp, vp, i32], None),
             ("cref_relax9", [vp, vp, vp], i32),
+            ("cref_poly_smooth", [vp, vp, vp, i32, vp], None),
             ("cref_gselim", [vp, vp, i32], i32),
             ("cref_bsr_block_matvec_stable", [f64, vp, vp, f64, vp, i32], None),
             ("cref_bsr_block_invmatvec_stable", [vp, vp, vp, i32], None),
@@ -136,6 +138,21 @@ class Oracle:
     def set_relax_threads(self, per_level):
         for l, t in enumerate(per_level):
             self.hier.relax_threads[l] = int(t)
+
+    def set_poly(self, omegas):
+        """Polynomial smoother on every level: omegas[l] = step sizes w_k."""
+        deg = len(omegas[0])
+        self.hier.poly_degree = deg
+        for l, w in enumerate(omegas):
+            assert len(w) == deg
+            for k, v in enumerate(w):
+                self.hier.poly_omega[l][k] = float(v)
+
+    def poly_smooth(self, level, f, u, omegas):
+        u = np.array(u, dtype=np.float64, copy=True)
+        w = np.ascontiguousarray(omegas, dtype=np.float64)
+        self.L.cref_poly_smooth(ctypes.byref(self.hier.A[level]), _p(np.ascontiguousarray(f)), _p(u), len(w), _p(w))
+        return u
 
     def relax6_threads(self, level, f, u, threads):
         u = np.array(u, dtype=np.float64, copy=True)

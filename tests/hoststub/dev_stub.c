@@ -32,6 +32,9 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
 int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
                      const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream)
 { (void)vt; (void)xt; (void)n; (void)sptr; (void)col; (void)val; (void)rsc; (void)x; (void)xs; (void)rhs; (void)rs; (void)y; (void)mode; (void)stream; return 0; }
+int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
+                      const void *dinv, const void *f, int fs, const void *uo, void *un, double omega, void *stream)
+{ (void)vt; (void)xt; (void)n; (void)sptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)f; (void)fs; (void)uo; (void)un; (void)omega; (void)stream; return 0; }
 int jxamd_sweep_waves(int nsl) { return nsl < 2048 ? (nsl > 0 ? nsl : 1) : 2048; }
 int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                      const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,

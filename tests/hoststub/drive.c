@@ -32,9 +32,9 @@ int main(int argc, char **argv)
     jxamd_hierarchy H;
     if (jxamd_amg_setup(&A0, &ap, &H)) return 7;
     jxamd_bsr A = {nb, nb, nnzb, b, ia, ja, a};
-    const int chunks[] = {0, 1, 3, 16, 40};
+    const int chunks[] = {0, 1, 3, 16, 40, 0, 0};
     int fails = 0, runs = 0;
-    for (int ci = 0; ci < 5; ci++)
+    for (int ci = 0; ci < 7; ci++)
         for (int prec = 0; prec <= 2; prec++)
             for (int s2 = 0; s2 < 4; s2++) {
                 jxamd_params p;
@@ -44,9 +44,16 @@ int main(int argc, char **argv)
                 p.stage2_solver_type = s2 < 3 ? 2 : 5;
                 p.stage2_maxit = s2 < 3 ? s2 + 1 : 1;
                 if (ci == 4) { p.relax_chunk = 0; p.relax_threads = 4; }
+                if (ci == 5) p.poly_degree = 4;
+                if (ci == 6) p.poly_degree = 1;
                 jxamd_solver *s = NULL;
                 int rc = jxamd_solver_create(&s, &A, &H, &p);
                 jxamd_stats st;
+                if (!rc && p.poly_degree > 0) {
+                    double w[8], lam;
+                    rc = jxamd_solver_get_poly(s, 0, p.poly_degree, w, &lam);
+                    if (!rc) rc = jxamd_solver_set_poly(s, 0, p.poly_degree, w);
+                }
                 if (!rc) rc = jxamd_solver_solve(s, rhs, x, NULL, &st);
                 if (s) jxamd_solver_destroy(s);
                 runs++;

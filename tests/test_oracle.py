@@ -303,3 +303,36 @@ def test_cpr_with_bilu0_stage2_converges():
     x, st, hist, rc = o.gmres(bd, o.params(tol=1e-8, stage2_maxit=1, stage2_solver_type=5))
     true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
     assert rc == 0 and st.converged == 1 and true <= 1e-8
+
+
+def test_poly_smoother_error_propagation():
+    """u <- u + w_k D^{-1}(f - A u): the error e = u - A^{-1} f propagates by
+    prod_k (I - w_k D^{-1} A), checked with dense matrices."""
+    import jxpamg_all_amd as jx
+    A, csr = _small_pressure_system()
+    n = A.shape[0]
+    Ab = jx.BSRMatrix(1, 2, np.array([0, 1], np.int32), np.array([0], np.int32), np.eye(2).ravel())
+    o = Oracle(Ab, jx.Hierarchy([csr], []))
+    rng = np.random.default_rng(9)
+    f, u0 = rng.standard_normal(n), rng.standard_normal(n)
+    w = np.array([0.9, 0.35, 1.4, 0.6])
+    got = o.poly_smooth(0, f, u0, w)
+    Ad = A.toarray()
+    Dinv = np.diag(1.0 / np.diag(Ad))
+    xs = np.linalg.solve(Ad, f)
+    E = np.eye(n)
+    for wk in w:
+        E = (np.eye(n) - wk * Dinv @ Ad) @ E
+    assert relerr(got - xs, E @ (u0 - xs)) < 1e-10
+
+
+def test_vcycle_with_poly_smoother_is_linear_and_converges():
+    d, A, rhs, Ad, bd, H, meta = system("lvl3")
+    o = Oracle(Ad, H)
+    o.set_poly([[0.6, 1.2, 0.7, 1.0]] * H.num_levels)
+    rng = np.random.default_rng(6)
+    n = H.A[0][0].size - 1
+    f1, f2 = rng.standard_normal(n), rng.standard_normal(n)
+    assert relerr(o.vcycle(2.0 * f1 - 3.0 * f2), 2.0 * o.vcycle(f1) - 3.0 * o.vcycle(f2)) < 1e-10
+    x, st, hist, rc = o.gmres(bd, o.params(tol=1e-8))
+    assert st.converged == 1 and np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd) <= 1e-8
