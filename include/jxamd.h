@@ -212,6 +212,36 @@ int jxamd_solver_set_poly(jxamd_solver *s, int level, int degree, const double *
 int jxamd_solver_get_poly(jxamd_solver *s, int level, int degree, double *omega, double *lambda_max);
 int jxamd_solver_num_levels(jxamd_solver *s, int *num_levels);
 
+/* ------------------------------------------------------------------------
+ * Learned polynomial smoother (no reference counterpart).  A 3-layer MLP
+ * (JXAMD_MLP_NFEAT level features -> 64 -> 64 -> JXAMD_MLP_NOUT, tanh hidden
+ * units) predicts the degree-4 step sizes of each level as w_k * lambda_max.
+ * BF16 operands on the matrix cores, FP32 accumulation and master weights.
+ * Parameter layout (FP32): W1[64][16] b1[64] W2[64][64] b2[64] W3[4][64] b3[4].
+ * ---------------------------------------------------------------------- */
+#define JXAMD_MLP_NFEAT  16
+#define JXAMD_MLP_NOUT   4
+#define JXAMD_MLP_NPARAM 5508
+/* y[n][4] = MLP(x[n][16]), device pointers */
+int jxamd_mlp_forward(const float *d_params, const float *d_x, int n, float *d_y, void *stream);
+/* steps Adam steps (lr, betas 0.9/0.999, eps 1e-8; update = 0 only computes
+ * the gradient) on the mean squared error over mini-batches of 32 samples
+ * taken cyclically from (x[n][16], t[n][4]) starting at batch t0; d_grad
+ * receives the last step's gradient, d_loss[steps] (optional) the losses */
+int jxamd_mlp_train(float *d_params, float *d_m, float *d_v, float *d_grad, const float *d_x, const float *d_t, int n,
+                    int steps, int t0, float lr, int update, float *d_loss, void *stream);
+/* the features of one smoothing level (solver created with poly_degree 4) */
+int jxamd_solver_poly_features(jxamd_solver *s, int level, float *feat);
+/* V-cycle contraction factor: ||e_k|| / ||e_0|| to the power 1/ncycles for
+ * ncycles V-cycles on f = 0 from a fixed pseudo-random e_0 */
+int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor);
+/* Set every level's step sizes from the MLP (d_params on the device), then
+ * judge: keep them only if the V-cycle contraction factor is below the one
+ * of the default Chebyshev steps, else restore the defaults.  *used = 1 if
+ * the learned steps are kept; factors reported if non-NULL. */
+int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, double *factor_learned,
+                           double *factor_default);
+
 /* Single operators on device vectors (parity tests, benchmarks). */
 /* y = alpha*A*x + beta*y, jxf_BSRMatrixMatvec (seq_bsr_matvec.c:21-134) */
 int jxamd_op_bsr_matvec(jxamd_solver *s, double alpha, const double *d_x, double beta, double *d_y);

@@ -99,6 +99,7 @@ struct jxamd_solver_s {
     void *dinvl[JXAMD_MAX_LEVELS];          /* polynomial smoother: 1/a_ii (0 if a_ii = 0), type xt */
     double poly_w[JXAMD_MAX_LEVELS][8];     /* its step sizes */
     double lam[JXAMD_MAX_LEVELS];           /* lambda_max(D^{-1} A_l) estimate */
+    float feat[JXAMD_MAX_LEVELS][JXAMD_MLP_NFEAT]; /* level features of the learned step sizes */
     double *bz;            /* BILU(0) intermediate vector, N */
     double *xtmp;          /* stage-2 ping-pong partner, N */
     double *V;             /* (k_dim+1) Krylov vectors, N each */
@@ -951,8 +952,9 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
 /* polynomial smoother data of level l: 1/a_ii, a lambda_max(D^{-1} A)
  * estimate by 20 power iterations from a fixed start vector, and default
  * Chebyshev step sizes on [0.3, 1.1] x lambda */
-static int build_poly(jxamd_solver *s, int l, const jxamd_csr *A)
+static int build_poly(jxamd_solver *s, int l, const jxamd_hierarchy *H)
 {
+    const jxamd_csr *A = &H->A[l];
     const int n = A->n;
     double *di = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
     double *v = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
@@ -983,6 +985,54 @@ static int build_poly(jxamd_solver *s, int l, const jxamd_csr *A)
         for (int i = 0; i < n; i++) v[i] = w[i] * sc;
     }
     s->lam[l] = lam;
+    /* features of the level for the learned step sizes */
+    {
+        double gmax = 0.0, rsum = 0.0, rmin = 1e300, rmean = 0.0, pos = 0.0, noff = 0.0, lsum = 0.0, lsq = 0.0,
+               rsratio = 0.0;
+        int maxrow = 0;
+        for (int i = 0; i < n; i++) {
+            const double d = A->a[A->ia[i]];
+            double off = 0.0, sum = 0.0;
+            for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
+                sum += A->a[k];
+                if (k == A->ia[i]) continue;
+                off += fabs(A->a[k]);
+                pos += A->a[k] > 0.0;
+                noff += 1.0;
+            }
+            const double ad = fabs(d) > 0.0 ? fabs(d) : 1.0;
+            const double ratio = off / ad;
+            if (1.0 + ratio > gmax) gmax = 1.0 + ratio;
+            if (ratio < rmin) rmin = ratio;
+            rmean += ratio;
+            rsratio += sum / (d != 0.0 ? d : 1.0);
+            const double lg = log10(ad);
+            lsum += lg;
+            lsq += lg * lg;
+            if (A->ia[i + 1] - A->ia[i] > maxrow) maxrow = A->ia[i + 1] - A->ia[i];
+        }
+        (void)rsum;
+        const double nn = n > 0 ? (double)n : 1.0;
+        const double lmean = lsum / nn, lvar = lsq / nn - lmean * lmean;
+        const int last = s->L == 1 ? 0 : s->L - 2;
+        float *f = s->feat[l];
+        f[0] = (float)(l / 10.0);
+        f[1] = (float)(log10(nn) / 7.0);
+        f[2] = (float)(log10(nn / (double)H->A[0].n) / 3.0);
+        f[3] = (float)((double)A->ia[n] / nn / 30.0);
+        f[4] = (float)(maxrow / 100.0);
+        f[5] = (float)(lam / 2.0);
+        f[6] = (float)(gmax / 2.0);
+        f[7] = (float)(rmean / nn);
+        f[8] = (float)(n > 0 ? rmin : 0.0);
+        f[9] = (float)(noff > 0.0 ? pos / noff : 0.0);
+        f[10] = (float)(sqrt(lvar > 0.0 ? lvar : 0.0) / 5.0);
+        f[11] = (float)(l + 1 < s->L ? (double)H->A[l + 1].n / nn : 0.0);
+        f[12] = (float)(l == 0);
+        f[13] = (float)(l == last);
+        f[14] = (float)(rsratio / nn);
+        f[15] = 1.0f;
+    }
     const int m = s->prm.poly_degree;
     const double lo = 0.3 * lam, hi = 1.1 * lam, c = 0.5 * (hi + lo), d = 0.5 * (hi - lo);
     for (int k = 0; k < m; k++) s->poly_w[l][k] = 1.0 / (c + d * cos(M_PI * (2.0 * k + 1.0) / (2.0 * m)));
@@ -1016,7 +1066,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     /* CSR SpMV bytes: nnz (value + index) + row pointer + x + rhs + y */
     s->Ares[l].abytes = level_bytes(&s->Ares[l], A->ia[n], n, 3.0);
     if (s->prm.poly_degree > 0 && (l < s->L - 1 || s->L == 1)) {
-        CK(build_poly(s, l, A));
+        CK(build_poly(s, l, H));
         return build_transfer(s, l, H);
     }
     int T = 0;
@@ -1529,6 +1579,94 @@ int jxamd_solver_get_poly(jxamd_solver *s, int level, int degree, double *omega,
     if (omega)
         for (int k = 0; k < degree; k++) omega[k] = s->poly_w[level][k];
     if (lambda_max) *lambda_max = s->lam[level];
+    return 0;
+}
+
+int jxamd_solver_poly_features(jxamd_solver *s, int level, float *feat)
+{
+    if (!poly_level_ok(s, level, s ? s->prm.poly_degree : 0) || !feat) return JXAMD_ERROR_ARG;
+    memcpy(feat, s->feat[level], sizeof(float) * JXAMD_MLP_NFEAT);
+    return 0;
+}
+
+/* ||U[0]|| from the device (host copy; setup-time only) */
+static int level0_norm(jxamd_solver *s, double *nrm)
+{
+    const int n = s->nlev[0];
+    void *h = malloc(s->xsz * (n > 0 ? n : 1));
+    if (!h) return JXAMD_ERROR_MEMORY;
+    int e = jxamd_d2h(h, s->U[0], s->xsz * n, s->stream);
+    if (!e) e = jxamd_stream_sync(s->stream);
+    double acc = 0.0;
+    for (int i = 0; i < n && !e; i++) {
+        double v = s->xt == JXAMD_T_F64 ? ((double *)h)[i] : (double)((float *)h)[i];
+        acc += v * v;
+    }
+    free(h);
+    *nrm = sqrt(acc);
+    return e;
+}
+
+int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor)
+{
+    if (!s || ncycles < 1 || !factor) return JXAMD_ERROR_ARG;
+    const int n = s->nlev[0];
+    void *h = malloc(s->xsz * (n > 0 ? n : 1));
+    if (!h) return JXAMD_ERROR_MEMORY;
+    uint64_t st = 0x2545f4914f6cdd1dull;
+    for (int i = 0; i < n; i++) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        const double v = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        if (s->xt == JXAMD_T_F64) ((double *)h)[i] = v;
+        else ((float *)h)[i] = (float)v;
+    }
+    int e = jxamd_h2d(s->U[0], h, s->xsz * n, s->stream);
+    if (!e) e = s->xt == JXAMD_T_F64 ? jxamd_k_fill(n, (double *)s->F[0], 0.0, s->stream)
+                                     : jxamd_k_fill_f32(n, (float *)s->F[0], 0.0f, s->stream);
+    if (!e) e = jxamd_stream_sync(s->stream);
+    free(h);
+    CK(e);
+    double n0 = 0.0, n1 = 0.0;
+    CK(level0_norm(s, &n0));
+    for (int c = 0; c < ncycles; c++) CK(vcycle(s, s->F[0], 1, 0));
+    CK(level0_norm(s, &n1));
+    *factor = n0 > 0.0 ? pow(n1 / n0, 1.0 / ncycles) : 0.0;
+    return 0;
+}
+
+int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, double *factor_learned,
+                           double *factor_default)
+{
+    if (!s || !d_params || s->prm.poly_degree != JXAMD_MLP_NOUT) return JXAMD_ERROR_ARG;
+    const int Ls = s->L > 1 ? s->L - 1 : 1;
+    float hy[JXAMD_MAX_LEVELS * JXAMD_MLP_NOUT];
+    float *dx = NULL, *dy = NULL;
+    int e = jxamd_dev_malloc((void **)&dx, sizeof(float) * Ls * JXAMD_MLP_NFEAT);
+    if (!e) e = jxamd_dev_malloc((void **)&dy, sizeof(float) * Ls * JXAMD_MLP_NOUT);
+    if (!e) e = jxamd_h2d(dx, s->feat, sizeof(float) * Ls * JXAMD_MLP_NFEAT, s->stream);
+    if (!e) e = jxamd_mlp_forward(d_params, dx, Ls, dy, s->stream);
+    if (!e) e = jxamd_d2h(hy, dy, sizeof(float) * Ls * JXAMD_MLP_NOUT, s->stream);
+    if (!e) e = jxamd_stream_sync(s->stream);
+    jxamd_dev_free(dx);
+    jxamd_dev_free(dy);
+    CK(e);
+    double wdef[JXAMD_MAX_LEVELS][8];
+    memcpy(wdef, s->poly_w, sizeof(wdef));
+    double fdef = 0.0, flrn = 1e300;
+    CK(jxamd_solver_vcycle_factor(s, 4, &fdef));
+    int ok = 1;
+    for (int l = 0; l < Ls; l++)
+        for (int k = 0; k < JXAMD_MLP_NOUT; k++) {
+            const double w = (double)hy[l * JXAMD_MLP_NOUT + k] / s->lam[l];
+            if (!(w > 0.0) || !(w < 1e3)) ok = 0; /* also rejects NaN */
+            s->poly_w[l][k] = w;
+        }
+    if (ok) CK(jxamd_solver_vcycle_factor(s, 4, &flrn));
+    const int keep = ok && flrn < fdef;
+    if (!keep) memcpy(s->poly_w, wdef, sizeof(wdef));
+    if (used) *used = keep;
+    if (factor_learned) *factor_learned = ok ? flrn : -1.0;
+    if (factor_default) *factor_default = fdef;
     return 0;
 }
 

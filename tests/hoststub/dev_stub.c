@@ -94,3 +94,13 @@ int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, 
 { (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }
 int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream)
 { (void)n; (void)m; (void)V; (void)ld; (void)y; (void)w; (void)stream; return 0; }
+
+int jxamd_mlp_forward(const float *d_params, const float *d_x, int n, float *d_y, void *stream)
+{
+    (void)d_params; (void)d_x; (void)stream;
+    for (int i = 0; i < n * JXAMD_MLP_NOUT; i++) d_y[i] = 1.0f;
+    return 0;
+}
+int jxamd_mlp_train(float *d_params, float *d_m, float *d_v, float *d_grad, const float *d_x, const float *d_t, int n,
+                    int steps, int t0, float lr, int update, float *d_loss, void *stream)
+{ (void)d_params; (void)d_m; (void)d_v; (void)d_grad; (void)d_x; (void)d_t; (void)n; (void)steps; (void)t0; (void)lr; (void)update; (void)d_loss; (void)stream; return 0; }

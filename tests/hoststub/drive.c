@@ -46,6 +46,7 @@ int main(int argc, char **argv)
                 if (ci == 4) { p.relax_chunk = 0; p.relax_threads = 4; }
                 if (ci == 5) p.poly_degree = 4;
                 if (ci == 6) p.poly_degree = 1;
+                if (ci == 5 && s2 == 1) p.poly_degree = 4;
                 jxamd_solver *s = NULL;
                 int rc = jxamd_solver_create(&s, &A, &H, &p);
                 jxamd_stats st;
@@ -53,6 +54,13 @@ int main(int argc, char **argv)
                     double w[8], lam;
                     rc = jxamd_solver_get_poly(s, 0, p.poly_degree, w, &lam);
                     if (!rc) rc = jxamd_solver_set_poly(s, 0, p.poly_degree, w);
+                    if (!rc && p.poly_degree == 4) {
+                        float feat[JXAMD_MLP_NFEAT], params[JXAMD_MLP_NPARAM] = {0};
+                        int used;
+                        double fl, fd;
+                        rc = jxamd_solver_poly_features(s, 0, feat);
+                        if (!rc) rc = jxamd_solver_apply_mlp(s, params, &used, &fl, &fd);
+                    }
                 }
                 if (!rc) rc = jxamd_solver_solve(s, rhs, x, NULL, &st);
                 if (s) jxamd_solver_destroy(s);

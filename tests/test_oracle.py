@@ -336,3 +336,23 @@ def test_vcycle_with_poly_smoother_is_linear_and_converges():
     assert relerr(o.vcycle(2.0 * f1 - 3.0 * f2), 2.0 * o.vcycle(f1) - 3.0 * o.vcycle(f2)) < 1e-10
     x, st, hist, rc = o.gmres(bd, o.params(tol=1e-8))
     assert st.converged == 1 and np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd) <= 1e-8
+
+
+def test_mlp_checker_gradient_matches_finite_differences():
+    """The checker's backward pass (BF16-rounded operands) against central
+    differences of the unrounded loss: agreement to the BF16 rounding level
+    pins the transposes, signs and tanh derivatives."""
+    from oracle import mlp_ref as M
+    rng = np.random.default_rng(0)
+    P = rng.normal(0, 0.2, M.NPARAM)
+    X, T = rng.normal(0, 1, (20, 16)), rng.normal(0, 1, (20, 4))
+    _, g = M.loss_and_grad(P, X, T)
+    idx = rng.choice(M.NPARAM, 40, replace=False)
+    fd = []
+    for i in idx:
+        e = np.zeros(M.NPARAM)
+        e[i] = 1e-5
+        fd.append((M.loss_exact(P + e, X, T) - M.loss_exact(P - e, X, T)) / 2e-5)
+    fd = np.array(fd)
+    assert np.max(np.abs(fd - g[idx])) < 0.03 * np.max(np.abs(fd))
+    assert np.corrcoef(fd, g[idx])[0, 1] > 0.999
