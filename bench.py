@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=2, help="GMRES iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-learned", action="store_true", help="skip the mixed-precision + learned-smoother leg")
+    ap.add_argument("--train-seeds", type=int, default=8, help="config-1-style systems labelled for training")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three plain device copies of MIB MiB (counter calibration) and exit")
     ap.add_argument("--relax-chunk", type=int, default=None,
@@ -118,6 +120,51 @@ def main():
         times.append(time.perf_counter() - t0)
         log(f"step {k}: {times[-1] * 1e3:.1f} ms iters {st.iterations} true_rel {st.true_rel_res:.3e}")
     sec = float(np.mean(times))
+
+    def timed_solves(solver, n):
+        ts, stx = [], None
+        x_d.zero_()
+        solver.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
+        for _ in range(n):
+            x_d.zero_()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            stx = solver.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t1)
+        return float(np.mean(ts)), stx
+
+    mixed = None
+    if not args.no_learned:
+        # mixed precision + learned polynomial smoother: FP32 stage 1 with the
+        # network's step sizes (trained here, on the GPU, on data the plain
+        # solver labels itself), FP64 stage 2 and GMRES
+        from jxpamg_all_amd import learned as lrn
+        t0 = time.time()
+        Xtr, Ttr, _ = lrn.make_dataset(range(1001, 1001 + args.train_seeds), precision=1)
+        net = lrn.MLP(seed=0)
+        losses = net.fit(Xtr, Ttr, steps=3000, lr=3e-3)
+        t_train = time.time() - t0
+        log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
+        legs = {}
+        for name, kw in (("mixed_default_poly", dict(precond_precision=1, poly_degree=4)),
+                         ("mixed_learned", dict(precond_precision=1, poly_degree=4))):
+            pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, **kw)
+            sm = jx.Solver(Ad, H, pm)
+            used = None
+            if name == "mixed_learned":
+                used, f_l, f_d = sm.apply_mlp(net.P.data_ptr())
+                log(f"learned steps used={used}: V-cycle factor {f_l:.4f} vs default {f_d:.4f}")
+            t_m, st_m = timed_solves(sm, args.steps)
+            assert st_m.converged == 1 and st_m.true_rel_res <= 1e-6, f"{name} did not reach rtol 1e-6"
+            legs[name] = {"s": round(t_m, 4), "iterations": st_m.iterations, "true_rel_res": st_m.true_rel_res,
+                          "speedup_vs_plain_fp64": round(sec / t_m, 3)}
+            if used is not None:
+                legs[name]["learned_steps_used"] = used
+            log(f"{name}: {t_m * 1e3:.1f} ms iters {st_m.iterations} true_rel {st_m.true_rel_res:.3e}")
+            sm.close()
+        legs["training_s"] = round(t_train, 2)
+        mixed = legs
     iters = st.iterations
     assert st.converged == 1 and st.true_rel_res <= 1e-6, "solve did not reach rtol 1e-6"
 
@@ -181,7 +228,8 @@ def main():
                       "true_rel_res": st.true_rel_res, "amg_levels": [len(m[0]) - 1 for m in H.A],
                       "host_setup_s": {"generate_decouple": round(t_gen, 2), "amg": round(t_amg, 2),
                                        "upload": round(t_create, 2)}},
-           "roofline": roofline, "spmv": spmv, "kernel_classes": breakdown, "cpu_baseline": cpu}
+           "roofline": roofline, "spmv": spmv, "kernel_classes": breakdown, "cpu_baseline": cpu,
+           "mixed_precision_learned": mixed}
     s.close()
     print(json.dumps(out), flush=True)
 

@@ -116,6 +116,11 @@ def _declare(L):
         "jxamd_solver_set_poly": [vp, i32, i32, vp],
         "jxamd_solver_get_poly": [vp, i32, i32, vp, vp],
         "jxamd_solver_num_levels": [vp, vp],
+        "jxamd_mlp_forward": [vp, vp, i32, vp, vp],
+        "jxamd_mlp_train": [vp, vp, vp, vp, vp, vp, i32, i32, i32, ctypes.c_float, i32, vp, vp],
+        "jxamd_solver_poly_features": [vp, i32, vp],
+        "jxamd_solver_vcycle_factor": [vp, i32, vp],
+        "jxamd_solver_apply_mlp": [vp, vp, vp, vp, vp],
         "jxamd_bilu0_factor": [i32, i32, vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
@@ -422,6 +427,24 @@ class Solver:
         lam = ctypes.c_double()
         check(self._L.jxamd_solver_get_poly(self._s, level, degree, _p(w), ctypes.byref(lam)), "jxamd_solver_get_poly")
         return w, lam.value
+
+    def poly_features(self, level):
+        f = np.zeros(16, np.float32)
+        check(self._L.jxamd_solver_poly_features(self._s, level, _p(f)), "jxamd_solver_poly_features")
+        return f
+
+    def vcycle_factor(self, ncycles=4):
+        v = ctypes.c_double()
+        check(self._L.jxamd_solver_vcycle_factor(self._s, ncycles, ctypes.byref(v)), "jxamd_solver_vcycle_factor")
+        return v.value
+
+    def apply_mlp(self, d_params):
+        """Learned step sizes on every level, kept only if they contract the
+        V-cycle better than the defaults: (used, factor_learned, factor_default)."""
+        used, fl, fd = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(self._L.jxamd_solver_apply_mlp(self._s, ctypes.c_void_p(d_params), ctypes.byref(used), ctypes.byref(fl),
+                                             ctypes.byref(fd)), "jxamd_solver_apply_mlp")
+        return bool(used.value), fl.value, fd.value
 
     def bilu_solve(self, d_r, d_z):
         check(self._L.jxamd_op_bilu_solve(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_z)), "jxamd_op_bilu_solve")

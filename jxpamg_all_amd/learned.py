@@ -1,0 +1,136 @@
+"""Learned polynomial AMG smoother: data from the plain solver, GPU training.
+
+The network (csrc/hip/jxamd_mlp.hip) maps 16 features of an AMG level to the
+four step sizes of its degree-4 polynomial smoother, as w_k * lambda_max.
+Training data come from the solver itself: on seeded 16x16x8 systems the
+step sizes of each level are chosen, level by level, from Chebyshev
+candidates on [lo, hi] x lambda_max by the V-cycle contraction factor the
+solver measures on the GPU; the network learns features -> the best steps.
+No reference counterpart; the solver keeps learned steps only when they
+contract the V-cycle better than its default Chebyshev steps.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import CONFIGS, FIELD_IID, Solver, amg_setup, check, decouple_abf, default_params, extract_pressure, \
+    generate, lib
+
+NPARAM, NFEAT, NOUT = 5508, 16, 4
+LO = (0.03, 0.06, 0.1, 0.15, 0.2, 0.3, 0.45)
+HI = (0.9, 1.0, 1.1, 1.25, 1.4)
+
+
+def chebyshev_steps(lo, hi, m=NOUT):
+    """Step sizes (in units of 1/lambda_max) with roots of the degree-m
+    Chebyshev polynomial on [lo, hi]."""
+    c, d = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    return np.array([1.0 / (c + d * math.cos(math.pi * (2 * k + 1) / (2 * m))) for k in range(m)])
+
+
+def training_system(seed, nx=16, ny=16, nz=8):
+    """A config-1-style system (iid lognormal permeability, sigma 1.5)."""
+    kw = dict(CONFIGS["C1"], nx=nx, ny=ny, nz=nz, field=FIELD_IID, seed=seed)
+    A, rhs = generate(**kw)
+    Ad, bd = decouple_abf(A, rhs)
+    H = amg_setup(*extract_pressure(Ad))
+    return Ad, bd, H
+
+
+def level_targets(solver, ncycles=4):
+    """Greedy per-level search of the best Chebyshev steps by the measured
+    V-cycle contraction; returns (features [L][16], targets [L][4],
+    best factor, default factor)."""
+    Ls = max(solver.num_levels - 1, 1)
+    f_def = solver.vcycle_factor(ncycles)
+    feats, targets = [], []
+    for l in range(Ls):
+        w0, lam = solver.get_poly(l, NOUT)
+        best = (solver.vcycle_factor(ncycles), w0 * lam)
+        for lo in LO:
+            for hi in HI:
+                z = chebyshev_steps(lo, hi)
+                solver.set_poly(l, z / lam)
+                f = solver.vcycle_factor(ncycles)
+                if f < best[0]:
+                    best = (f, z)
+        solver.set_poly(l, best[1] / lam)
+        feats.append(solver.poly_features(l))
+        targets.append(best[1])
+    return np.array(feats, np.float32), np.array(targets, np.float32), best[0], f_def
+
+
+def make_dataset(seeds, precision=1, **grid):
+    X, T, info = [], [], []
+    for sd in seeds:
+        Ad, bd, H = training_system(sd, **grid)
+        s = Solver(Ad, H, default_params(poly_degree=NOUT, precond_precision=precision))
+        f, t, fb, fd = level_targets(s)
+        s.close()
+        X.append(f)
+        T.append(t)
+        info.append((sd, fb, fd))
+    return np.concatenate(X), np.concatenate(T), info
+
+
+class MLP:
+    """FP32 master weights, Adam state and gradient in HBM (torch tensors as
+    storage); every forward, backward and update runs in the HIP kernels."""
+
+    def __init__(self, seed=0, device="cuda"):
+        import torch
+        self.torch = torch
+        rng = np.random.default_rng(seed)
+        P = np.zeros(NPARAM, np.float32)
+        o = 0
+        for shp in ((64, 16), (64,), (64, 64), (64,), (4, 64), (4,)):
+            n = int(np.prod(shp))
+            if len(shp) == 2:
+                P[o:o + n] = rng.normal(0.0, 1.0 / math.sqrt(shp[1]), n)
+            o += n
+        self.P = torch.tensor(P, device=device)
+        self.M = torch.zeros_like(self.P)
+        self.V = torch.zeros_like(self.P)
+        self.G = torch.zeros_like(self.P)
+        self.t = 0
+
+    def _dev(self, a, cols):
+        a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1, cols)
+        return self.torch.tensor(a, device=self.P.device)
+
+    def forward(self, X):
+        x = self._dev(X, NFEAT)
+        y = self.torch.zeros((len(x), NOUT), dtype=self.torch.float32, device=self.P.device)
+        check(lib().jxamd_mlp_forward(ctypes.c_void_p(self.P.data_ptr()), ctypes.c_void_p(x.data_ptr()), len(x),
+                                      ctypes.c_void_p(y.data_ptr()), None), "jxamd_mlp_forward")
+        self.torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    def _train(self, X, T, steps, lr, update):
+        x, t = self._dev(X, NFEAT), self._dev(T, NOUT)
+        loss = self.torch.zeros(steps, dtype=self.torch.float32, device=self.P.device)
+        check(lib().jxamd_mlp_train(ctypes.c_void_p(self.P.data_ptr()), ctypes.c_void_p(self.M.data_ptr()),
+                                    ctypes.c_void_p(self.V.data_ptr()), ctypes.c_void_p(self.G.data_ptr()),
+                                    ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(t.data_ptr()), len(x), steps,
+                                    self.t, ctypes.c_float(lr), int(update), ctypes.c_void_p(loss.data_ptr()), None),
+              "jxamd_mlp_train")
+        self.torch.cuda.synchronize()
+        if update:
+            self.t += steps
+        return loss.cpu().numpy()
+
+    def gradient(self, X, T):
+        """(loss, gradient) of one batch (first 32 samples), no update."""
+        loss = self._train(X, T, 1, 0.0, False)
+        return float(loss[0]), self.G.cpu().numpy()
+
+    def fit(self, X, T, steps=3000, lr=3e-3):
+        return self._train(X, T, steps, lr, True)
+
+    @property
+    def params(self):
+        return self.P.cpu().numpy()
+
+    def load(self, P):
+        self.P.copy_(self.torch.tensor(np.asarray(P, np.float32)))
