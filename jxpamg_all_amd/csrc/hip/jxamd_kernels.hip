@@ -79,11 +79,10 @@ template <> struct row_scaled<__half> { static constexpr bool value = true; };
 __device__ __forceinline__ int xcd_tile(int bid, int nblocks)
 {
     const int nx = 8;
-    int per = (nblocks + nx - 1) / nx;
-    int xcd = bid % nx, k = bid / nx;
-    int t = xcd * per + k;
-    // the last XCD may have fewer tiles; fall back to identity for the tail
-    return (nblocks % nx == 0) ? t : bid;
+    const int base = nblocks / nx, rem = nblocks % nx;
+    const int xcd = bid % nx, k = bid / nx; // k-th workgroup dispatched to this XCD
+    // XCD x owns a contiguous run of base (+1 for the first rem XCDs) tiles
+    return xcd < rem ? xcd * (base + 1) + k : rem * (base + 1) + (xcd - rem) * base + k;
 }
 
 // ---------------------------------------------------------------------------
@@ -143,7 +142,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
                                                   const X *__restrict__ x, int xs, const X *__restrict__ rhs, int rs,
                                                   X *__restrict__ y, int mode)
 {
-    const int s = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     const int nsl = (n + WAVE - 1) / WAVE;
     if (s >= nsl) return;
@@ -169,7 +168,7 @@ __global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict_
                                                    const X *__restrict__ dinv, const X *__restrict__ f, int fs,
                                                    const X *__restrict__ uo, X *__restrict__ un, X omega)
 {
-    const int s = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     const int nsl = (n + WAVE - 1) / WAVE;
     if (s >= nsl) return;
@@ -549,7 +548,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
     X *sm = reinterpret_cast<X *>(smraw);
     X *g = sm, *ux = sm + 256, *dd = sm + 512, *B = sm + 768;
     const int size = n / nchunks, rest = n - size * nchunks;
-    const int c0 = blockIdx.x * cpb;
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * cpb;
     const int c1 = min(c0 + cpb, nchunks);
     const int row0 = c0 < rest ? c0 * (size + 1) : c0 * size + rest;
     const int row1 = c1 < rest ? c1 * (size + 1) : c1 * size + rest;
