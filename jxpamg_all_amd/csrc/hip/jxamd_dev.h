@@ -62,6 +62,16 @@ int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, co
 int jxamd_k_gs_chunk(int vt, int xt, int n, int nchunks, const int *wstep, const int *sptr, const int *col,
                      const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
                      void *un, void *stream);
+/* row-group variants over plain CSR (G = 4, 8, 16 or 32 lanes per row) */
+int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+                         const float *rsc, const void *x, int xs, const void *rhs, int rs, void *y, int mode,
+                         void *stream);
+int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+                          const float *rsc, const void *dinv, const void *f, int fs, const void *uo, void *un,
+                          double omega, void *stream);
+int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *ia, const int *ja, const void *val,
+                         const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                         void *stream);
 int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream);
 int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream);
 int jxamd_k_widen(int n, const float *x, double *y, void *stream);

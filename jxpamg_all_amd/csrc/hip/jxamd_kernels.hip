@@ -613,6 +613,138 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
     if (i < row1) un[i] = ux[r];
 }
 
+// ---------------------------------------------------------------------------
+// Row-group variants for levels with long rows (coarse AMG levels hold rows of
+// tens to hundreds of entries): G lanes share one row of a plain CSR matrix
+// (ia, ja, val in natural order, the diagonal first), read its entries
+// G-strided (contiguous across the group) and reduce with cross-lane adds.
+// rsc[i] scales row i for FP16 storage.
+// ---------------------------------------------------------------------------
+template <int G, typename X> __device__ __forceinline__ X group_sum(X v)
+{
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    return v;
+}
+
+//   mode 0: y = A x;  1: y = rhs - A x;  2: y += A x
+template <int G, typename V, typename X>
+__global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const V *__restrict__ val, const float *__restrict__ rsc,
+                                                      const X *__restrict__ x, int xs, const X *__restrict__ rhs,
+                                                      int rs, X *__restrict__ y, int mode)
+{
+    const int gl = threadIdx.x % G;
+    const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
+    X acc = 0;
+    if (row < n)
+        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * x[(size_t)ja[k] * xs];
+    acc = group_sum<G>(acc);
+    if (row >= n || gl != 0) return;
+    if (row_scaled<V>::value) acc *= (X)rsc[row];
+    if (mode == 0) y[row] = acc;
+    else if (mode == 1) y[row] = rhs[(size_t)row * rs] - acc;
+    else y[row] += acc;
+}
+
+template <int G, typename V, typename X>
+__global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                       const V *__restrict__ val, const float *__restrict__ rsc,
+                                                       const X *__restrict__ dinv, const X *__restrict__ f, int fs,
+                                                       const X *__restrict__ uo, X *__restrict__ un, X omega)
+{
+    const int gl = threadIdx.x % G;
+    const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
+    X acc = 0;
+    if (row < n && uo)
+        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * uo[ja[k]];
+    acc = group_sum<G>(acc);
+    if (row >= n || gl != 0) return;
+    if (row_scaled<V>::value) acc *= (X)rsc[row];
+    const X r = f[(size_t)row * fs] - acc;
+    un[row] = (uo ? uo[row] : (X)0) + omega * dinv[row] * r;
+}
+
+// chunked hybrid SSOR (as k_gs_chunk_lds) with phase A done by row groups
+template <int LMAX, int G, typename V, typename X>
+__global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cpb, const int *__restrict__ ia,
+                                                      const int *__restrict__ ja, const V *__restrict__ val,
+                                                      const float *__restrict__ rsc, const X *__restrict__ diag,
+                                                      const X *__restrict__ f, int fs, const X *__restrict__ uo,
+                                                      X *__restrict__ un)
+{
+    constexpr int BST = LMAX * LMAX + 1;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    X *sm = reinterpret_cast<X *>(smraw);
+    X *g = sm, *ux = sm + 256, *dd = sm + 512, *B = sm + 768;
+    const int size = n / nchunks, rest = n - size * nchunks;
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * cpb;
+    const int c1 = min(c0 + cpb, nchunks);
+    const int row0 = c0 < rest ? c0 * (size + 1) : c0 * size + rest;
+    const int row1 = c1 < rest ? c1 * (size + 1) : c1 * size + rest;
+    for (int e = threadIdx.x; e < cpb * BST; e += blockDim.x) B[e] = 0;
+    __syncthreads();
+    const int gl = threadIdx.x % G;
+    const int big = rest * (size + 1);
+    for (int r = threadIdx.x / G; r < row1 - row0; r += blockDim.x / G) {
+        const int i = row0 + r;
+        const int c = i < big ? i / (size + 1) : rest + (i - big) / size;
+        const int ns = c < rest ? c * (size + 1) : c * size + rest;
+        const int len = c < rest ? size + 1 : size;
+        X *Brow = B + (size_t)(c - c0) * BST + (i - ns) * LMAX;
+        const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
+        X acc = 0;
+        for (int k = ia[i] + 1 + gl; k < ia[i + 1]; k += G) {
+            const int j = ja[k];
+            const X v = sc * (X)cvt(val[k]);
+            if (j >= ns && j < ns + len) {
+                if (j != i) atomicAdd(&Brow[j - ns], v);
+            } else if (uo) {
+                acc -= v * uo[j];
+            }
+        }
+        acc = group_sum<G>(acc);
+        if (gl == 0) {
+            dd[r] = diag[i];
+            g[r] = f[(size_t)i * fs] + acc;
+            ux[r] = uo ? uo[i] : (X)0;
+        }
+    }
+    __syncthreads();
+    const int c = c0 + (int)threadIdx.x;
+    if (c < c1) {
+        const int ns = c < rest ? c * (size + 1) : c * size + rest;
+        const int len = c < rest ? size + 1 : size;
+        const int rl = ns - row0;
+        const X *Bq = B + (size_t)(c - c0) * BST;
+        X u[LMAX];
+#pragma unroll
+        for (int t = 0; t < LMAX; t++) u[t] = t < len ? ux[rl + t] : (X)0;
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+            for (int tt = 0; tt < LMAX; tt++) {
+                const int t = pass == 0 ? tt : LMAX - 1 - tt;
+                if (t < len) {
+                    const X d = dd[rl + t];
+                    if (d != (X)0) {
+                        X x = g[rl + t];
+#pragma unroll
+                        for (int q = 0; q < LMAX; q++)
+                            if (q != t && q < len) x -= Bq[t * LMAX + q] * u[q];
+                        u[t] = x / d;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < LMAX; t++)
+            if (t < len) ux[rl + t] = u[t];
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < row1 - row0; r += blockDim.x) un[row0 + r] = ux[r];
+}
+
 // dense y = M x (row-major n x n), one wavefront per row
 template <typename X>
 __global__ void __launch_bounds__(256) k_dense_gemv(int n, const X *__restrict__ M, const X *__restrict__ x,
@@ -1001,6 +1133,64 @@ int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, co
     else if (maxlen <= 8) LAUNCH_LDS(8);
     else LAUNCH_LDS(16);
 #undef LAUNCH_LDS
+    return chk(hipGetLastError());
+}
+
+#define DISPATCH_G(G, CALL)                                                                                            \
+    do {                                                                                                               \
+        if ((G) == 4) { constexpr int G_ = 4; CALL; }                                                                  \
+        else if ((G) == 8) { constexpr int G_ = 8; CALL; }                                                             \
+        else if ((G) == 16) { constexpr int G_ = 16; CALL; }                                                           \
+        else if ((G) == 32) { constexpr int G_ = 32; CALL; }                                                           \
+        else return JXAMD_ERROR_ARG;                                                                                   \
+    } while (0)
+
+int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+                         const float *rsc, const void *x, int xs, const void *rhs, int rs, void *y, int mode,
+                         void *stream)
+{
+    if (n == 0) return 0;
+    const int grid = (int)(((long)n * G + 255) / 256);
+    DISPATCH_G(G, DISPATCH_VX(vt, xt,
+                              (k_csr_spmv_grp<G_, V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                                  n, ia, ja, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y,
+                                  mode))));
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+                          const float *rsc, const void *dinv, const void *f, int fs, const void *uo, void *un,
+                          double omega, void *stream)
+{
+    if (n == 0) return 0;
+    const int grid = (int)(((long)n * G + 255) / 256);
+    DISPATCH_G(G, DISPATCH_VX(vt, xt,
+                              (k_poly_step_grp<G_, V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                                  n, ia, ja, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs,
+                                  (const X_ *)uo, (X_ *)un, (X_)omega))));
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *ia, const int *ja, const void *val,
+                         const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                         void *stream)
+{
+    if (n == 0) return 0;
+    const int maxlen = jxamd_gs_chunk_lds_rows(n, nchunks);
+    if (maxlen < 1 || maxlen > JXAMD_CHUNK_LDS_MAX) return JXAMD_ERROR_ARG;
+    const int cpb = 256 / maxlen;
+    const int grid = (nchunks + cpb - 1) / cpb;
+    hipStream_t st = (hipStream_t)stream;
+#define LAUNCH_GRP(LM)                                                                                                 \
+    DISPATCH_G(G, DISPATCH_VX(vt, xt,                                                                                  \
+                              (k_gs_chunk_grp<LM, G_, V_, X_>                                                         \
+                               <<<grid, 256, sizeof(X_) * (768 + (size_t)cpb * (LM * LM + 1)), st>>>(                 \
+                                   n, nchunks, cpb, ia, ja, (const V_ *)val, rsc, (const X_ *)diag, (const X_ *)f,    \
+                                   fs, (const X_ *)uo, (X_ *)un))))
+    if (maxlen <= 4) LAUNCH_GRP(4);
+    else if (maxlen <= 8) LAUNCH_GRP(8);
+    else LAUNCH_GRP(16);
+#undef LAUNCH_GRP
     return chk(hipGetLastError());
 }
 

@@ -40,6 +40,7 @@ typedef struct {
     void *val, *diag;        /* val: storage type vt; diag: vector type xt */
     float *rsc;              /* vt == F16: power-of-two scale per position */
     int vt, xt;
+    int G;                   /* > 1: plain CSR (sptr = ia, col = ja), G lanes per row */
     double abytes;           /* algorithmic HBM bytes of one launch over it */
 } dsell;
 
@@ -573,6 +574,90 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsc
     return jxamd_stream_sync(st);
 }
 
+/* lanes per row for a matrix of mean row length nnz/n: one (SELL-64) for
+ * short rows, else a power of two near a quarter of the row length */
+static int choose_group(long nnz, int n)
+{
+    const double m = n > 0 ? (double)nnz / n : 0.0;
+    if (m < 12.0) return 1;
+    if (m < 24.0) return 4;
+    if (m < 48.0) return 8;
+    if (m < 96.0) return 16;
+    return 32;
+}
+
+/* Upload a CSR matrix for the row-group kernels; F16 rows are scaled by the
+ * smallest power of two at or above their largest off-diagonal magnitude
+ * (the whole row if it has no off-diagonal); want_diag: also the diagonal
+ * (first entry of each row) as a vector of type xt. */
+static int dcsr_upload(dsell *d, int n, int ncols, const int *ia, const int *ja, const double *a, int vt, int xt,
+                       int G, int want_diag, void *st)
+{
+    memset(d, 0, sizeof(*d));
+    d->n = n;
+    d->ncols = ncols;
+    d->vt = vt;
+    d->xt = xt;
+    d->G = G;
+    const size_t nnz = (size_t)ia[n];
+    CK(upload_ints(&d->sptr, ia, (size_t)n + 1, st));
+    CK(upload_ints(&d->col, ja, nnz, st));
+    CK(jxamd_dev_malloc(&d->val, type_size(vt) * (nnz ? nnz : 1)));
+    if (vt == JXAMD_T_F64) {
+        CK(jxamd_h2d(d->val, a, sizeof(double) * nnz, st));
+    } else {
+        void *tmp = malloc(type_size(vt) * (nnz ? nnz : 1));
+        float *sc = NULL;
+        if (!tmp) return JXAMD_ERROR_MEMORY;
+        if (vt == JXAMD_T_F32) {
+            for (size_t k = 0; k < nnz; k++) ((float *)tmp)[k] = (float)a[k];
+        } else {
+            sc = (float *)malloc(sizeof(float) * (n > 0 ? n : 1));
+            if (!sc) { free(tmp); return JXAMD_ERROR_MEMORY; }
+            for (int i = 0; i < n; i++) {
+                double mx = 0.0;
+                const int k0 = ia[i + 1] - ia[i] > 1 ? ia[i] + 1 : ia[i];
+                for (int k = k0; k < ia[i + 1]; k++)
+                    if (fabs(a[k]) > mx) mx = fabs(a[k]);
+                int ex = 0;
+                if (mx > 0.0) {
+                    double m = frexp(mx, &ex);
+                    if (m == 0.5) ex -= 1;
+                }
+                const float scale = mx > 0.0 ? (float)ldexp(1.0, ex) : 1.0f;
+                sc[i] = scale;
+                for (int k = ia[i]; k < ia[i + 1]; k++) ((uint16_t *)tmp)[k] = half_bits((float)(a[k] / scale));
+            }
+            int rc = jxamd_dev_malloc((void **)&d->rsc, sizeof(float) * (n > 0 ? n : 1));
+            if (!rc) rc = jxamd_h2d(d->rsc, sc, sizeof(float) * n, st);
+            if (rc) { free(tmp); free(sc); return rc; }
+        }
+        int rc = jxamd_h2d(d->val, tmp, type_size(vt) * nnz, st);
+        if (!rc) rc = jxamd_stream_sync(st);
+        free(tmp);
+        free(sc);
+        CK(rc);
+    }
+    if (want_diag) {
+        double *dg = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+        if (!dg) return JXAMD_ERROR_MEMORY;
+        for (int i = 0; i < n; i++) dg[i] = a[ia[i]];
+        int rc = jxamd_dev_malloc(&d->diag, type_size(xt) * (n > 0 ? n : 1));
+        if (!rc) {
+            if (xt == JXAMD_T_F64) rc = jxamd_h2d(d->diag, dg, sizeof(double) * n, st);
+            else {
+                float *f = (float *)dg; /* in place: float k lies before double k */
+                for (int i = 0; i < n; i++) f[i] = (float)dg[i];
+                rc = jxamd_h2d(d->diag, f, sizeof(float) * n, st);
+            }
+        }
+        if (!rc) rc = jxamd_stream_sync(st);
+        free(dg);
+        CK(rc);
+    }
+    return jxamd_stream_sync(st);
+}
+
 static void dsell_free(dsell *d)
 {
     jxamd_dev_free(d->sptr); jxamd_dev_free(d->col); jxamd_dev_free(d->val);
@@ -910,8 +995,14 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     if (l < s->L - 1) {
         /* interpolation P (n x nc) and restriction R = P^T (nc x n) */
         const jxamd_csr *Pm = &H->P[l];
-        e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL, s->vt, s->xt);
+        const int GP = choose_group(Pm->ia[Pm->n], Pm->n);
+        if (GP > 1) {
+            e = dcsr_upload(&s->P[l], Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, s->vt, s->xt, GP, 0, s->stream);
+            memset(&h, 0, sizeof(h));
+        } else {
+            e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
+            if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL, s->vt, s->xt);
+        }
         /* y += P x: x once, y read and written */
         s->P[l].abytes = level_bytes(&s->P[l], Pm->ia[Pm->n], Pm->n, 2.0) + (double)type_size(s->xt) * Pm->ncols;
         if (!e) e = jxamd_stream_sync(s->stream);
@@ -920,8 +1011,14 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         int *ri, *rj;
         double *ra;
         CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
-        e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL, s->vt, s->xt);
+        const int GR = choose_group(ri[Pm->ncols], Pm->ncols);
+        if (GR > 1) {
+            e = dcsr_upload(&s->R[l], Pm->ncols, Pm->n, ri, rj, ra, s->vt, s->xt, GR, 0, s->stream);
+            memset(&h, 0, sizeof(h));
+        } else {
+            e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
+            if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL, s->vt, s->xt);
+        }
         s->R[l].abytes = level_bytes(&s->R[l], Pm->ia[Pm->n], Pm->ncols, 1.0) + (double)type_size(s->xt) * Pm->n;
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
@@ -1057,12 +1154,17 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     const int n = A->n;
     hsell h;
     int e;
-    /* natural order, full rows: residual F - A U */
-    e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 0, 0);
-    if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL, s->vt, s->xt);
-    if (!e) e = jxamd_stream_sync(s->stream);
-    hsell_free(&h);
-    CK(e);
+    /* natural order, full rows: residual F - A U (and the polynomial smoother) */
+    const int GA = choose_group(A->ia[n], n);
+    if (GA > 1) {
+        CK(dcsr_upload(&s->Ares[l], n, n, A->ia, A->ja, A->a, s->vt, s->xt, GA, 0, s->stream));
+    } else {
+        e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 0, 0);
+        if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL, s->vt, s->xt);
+        if (!e) e = jxamd_stream_sync(s->stream);
+        hsell_free(&h);
+        CK(e);
+    }
     /* CSR SpMV bytes: nnz (value + index) + row pointer + x + rhs + y */
     s->Ares[l].abytes = level_bytes(&s->Ares[l], A->ia[n], n, 3.0);
     if (s->prm.poly_degree > 0 && (l < s->L - 1 || s->L == 1)) {
@@ -1074,12 +1176,17 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     else if (s->prm.relax_chunk > 0) T = (n + s->prm.relax_chunk - 1) / s->prm.relax_chunk;
     if (T > n) T = n;
     if (T > 1 && (l < s->L - 1 || s->L == 1) && jxamd_gs_chunk_lds_rows(n, T) <= JXAMD_CHUNK_LDS_MAX) {
-        /* short chunks: one pass, off-diagonals in natural row order */
-        e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 1, 1);
-        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt);
-        if (!e) e = jxamd_stream_sync(s->stream);
-        hsell_free(&h);
-        CK(e);
+        /* short chunks: one pass, off-diagonals in natural row order (SELL),
+         * or full CSR rows read by row groups */
+        if (GA > 1) {
+            CK(dcsr_upload(&s->Gc[l], n, n, A->ia, A->ja, A->a, s->vt, s->xt, GA, 1, s->stream));
+        } else {
+            e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 1, 1);
+            if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt);
+            if (!e) e = jxamd_stream_sync(s->stream);
+            hsell_free(&h);
+            CK(e);
+        }
         s->nchunks[l] = T;
         s->wstep[l] = NULL;
         /* matrix once, f + entry values + results */
@@ -1298,6 +1405,9 @@ static int hsgs_chunk(jxamd_solver *s, int l, const void *f, int fs, const void 
     if (s->wstep[l])
         CK(jxamd_k_gs_chunk(g->vt, g->xt, s->nlev[l], s->nchunks[l], s->wstep[l], g->sptr, g->col, g->val, g->rsc,
                             g->diag, f, fs, uo, un, s->stream));
+    else if (g->G > 1)
+        CK(jxamd_k_gs_chunk_grp(g->G, g->vt, g->xt, s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->rsc,
+                                g->diag, f, fs, uo, un, s->stream));
     else
         CK(jxamd_k_gs_chunk_lds(g->vt, g->xt, s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->rsc, g->diag, f,
                                 fs, uo, un, s->stream));
@@ -1315,8 +1425,12 @@ static int poly_apply(jxamd_solver *s, int l, const void *f, int fs, void *u, in
     for (int k = 0; k < m; k++) {
         void *dst = (k == m - 1 && m > 1) ? u : ((k % 2 == 0) ? s->T[l] : s->Vt[l]);
         int pq = prof_begin(s);
-        CK(jxamd_k_poly_step(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs, src, dst,
-                             s->poly_w[l][k], s->stream));
+        if (A->G > 1)
+            CK(jxamd_k_poly_step_grp(A->G, A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs,
+                                     src, dst, s->poly_w[l][k], s->stream));
+        else
+            CK(jxamd_k_poly_step(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs, src, dst,
+                                 s->poly_w[l][k], s->stream));
         /* a step from zeros reads only f and dinv */
         prof_end(s, pq, JXAMD_PROF_GS_SWEEP, src ? A->abytes : 3.0 * (double)s->xsz * A->n);
         src = dst;
@@ -1352,7 +1466,12 @@ static int relax_once(jxamd_solver *s, int l, const void *f, int fs, void *u, in
 static int csr_launch(jxamd_solver *s, const dsell *A, const void *x, const void *rhs, int rs, void *y, int mode)
 {
     int pq = prof_begin(s);
-    CK(jxamd_k_csr_spmv(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode, s->stream));
+    if (A->G > 1)
+        CK(jxamd_k_csr_spmv_grp(A->G, A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode,
+                                s->stream));
+    else
+        CK(jxamd_k_csr_spmv(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode,
+                            s->stream));
     prof_end(s, pq, JXAMD_PROF_CSR_SPMV, A->abytes);
     return 0;
 }

@@ -104,3 +104,16 @@ int jxamd_mlp_forward(const float *d_params, const float *d_x, int n, float *d_y
 int jxamd_mlp_train(float *d_params, float *d_m, float *d_v, float *d_grad, const float *d_x, const float *d_t, int n,
                     int steps, int t0, float lr, int update, float *d_loss, void *stream)
 { (void)d_params; (void)d_m; (void)d_v; (void)d_grad; (void)d_x; (void)d_t; (void)n; (void)steps; (void)t0; (void)lr; (void)update; (void)d_loss; (void)stream; return 0; }
+
+int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+                         const float *rsc, const void *x, int xs, const void *rhs, int rs, void *y, int mode,
+                         void *stream)
+{ (void)G; (void)vt; (void)xt; (void)n; (void)ia; (void)ja; (void)val; (void)rsc; (void)x; (void)xs; (void)rhs; (void)rs; (void)y; (void)mode; (void)stream; return 0; }
+int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+                          const float *rsc, const void *dinv, const void *f, int fs, const void *uo, void *un,
+                          double omega, void *stream)
+{ (void)G; (void)vt; (void)xt; (void)n; (void)ia; (void)ja; (void)val; (void)rsc; (void)dinv; (void)f; (void)fs; (void)uo; (void)un; (void)omega; (void)stream; return 0; }
+int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *ia, const int *ja, const void *val,
+                         const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
+                         void *stream)
+{ (void)G; (void)vt; (void)xt; (void)n; (void)nchunks; (void)ia; (void)ja; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
