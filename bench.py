@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-learned", action="store_true", help="skip the mixed-precision + learned-smoother leg")
-    ap.add_argument("--train-seeds", type=int, default=8, help="config-1-style systems labelled for training")
+    ap.add_argument("--train-seeds", type=int, default=20, help="config-1-style systems labelled for training")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three plain device copies of MIB MiB (counter calibration) and exit")
     ap.add_argument("--relax-chunk", type=int, default=None,
@@ -154,13 +154,13 @@ def main():
             used = None
             if name == "mixed_learned":
                 used, f_l, f_d = sm.apply_mlp(net.P.data_ptr())
-                log(f"learned steps used={used}: V-cycle factor {f_l:.4f} vs default {f_d:.4f}")
+                log(f"learned steps on {used} level(s): V-cycle factor {f_l:.4f} vs default {f_d:.4f}")
             t_m, st_m = timed_solves(sm, args.steps)
             assert st_m.converged == 1 and st_m.true_rel_res <= 1e-6, f"{name} did not reach rtol 1e-6"
             legs[name] = {"s": round(t_m, 4), "iterations": st_m.iterations, "true_rel_res": st_m.true_rel_res,
                           "speedup_vs_plain_fp64": round(sec / t_m, 3)}
             if used is not None:
-                legs[name]["learned_steps_used"] = used
+                legs[name]["levels_on_learned_steps"] = used
             log(f"{name}: {t_m * 1e3:.1f} ms iters {st_m.iterations} true_rel {st_m.true_rel_res:.3e}")
             sm.close()
         legs["training_s"] = round(t_train, 2)

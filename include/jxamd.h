@@ -215,7 +215,8 @@ int jxamd_solver_num_levels(jxamd_solver *s, int *num_levels);
 /* ------------------------------------------------------------------------
  * Learned polynomial smoother (no reference counterpart).  A 3-layer MLP
  * (JXAMD_MLP_NFEAT level features -> 64 -> 64 -> JXAMD_MLP_NOUT, tanh hidden
- * units) predicts the degree-4 step sizes of each level as w_k * lambda_max.
+ * units) predicts the four roots of each level's degree-4 polynomial smoother
+ * in units of lambda_max(D^{-1} A): step w_k = 1 / (tau_k lambda_max).
  * BF16 operands on the matrix cores, FP32 accumulation and master weights.
  * Parameter layout (FP32): W1[64][16] b1[64] W2[64][64] b2[64] W3[4][64] b3[4].
  * ---------------------------------------------------------------------- */
@@ -235,10 +236,11 @@ int jxamd_solver_poly_features(jxamd_solver *s, int level, float *feat);
 /* V-cycle contraction factor: ||e_k|| / ||e_0|| to the power 1/ncycles for
  * ncycles V-cycles on f = 0 from a fixed pseudo-random e_0 */
 int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor);
-/* Set every level's step sizes from the MLP (d_params on the device), then
- * judge: keep them only if the V-cycle contraction factor is below the one
- * of the default Chebyshev steps, else restore the defaults.  *used = 1 if
- * the learned steps are kept; factors reported if non-NULL. */
+/* Predict every level's polynomial roots tau_k (w_k = 1 / (tau_k lambda_max))
+ * with the MLP (d_params on the device) and, level by level, keep them only
+ * where they lower the measured V-cycle contraction factor; elsewhere the
+ * default Chebyshev steps stay (the fallback).  *used = number of levels on
+ * learned steps; final and default factors reported if non-NULL. */
 int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, double *factor_learned,
                            double *factor_default);
 

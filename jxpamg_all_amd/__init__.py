@@ -439,12 +439,12 @@ class Solver:
         return v.value
 
     def apply_mlp(self, d_params):
-        """Learned step sizes on every level, kept only if they contract the
-        V-cycle better than the defaults: (used, factor_learned, factor_default)."""
+        """Learned polynomial roots, kept on each level only where they lower the
+        V-cycle contraction: (levels on learned steps, final factor, default factor)."""
         used, fl, fd = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         check(self._L.jxamd_solver_apply_mlp(self._s, ctypes.c_void_p(d_params), ctypes.byref(used), ctypes.byref(fl),
                                              ctypes.byref(fd)), "jxamd_solver_apply_mlp")
-        return bool(used.value), fl.value, fd.value
+        return used.value, fl.value, fd.value
 
     def bilu_solve(self, d_r, d_z):
         check(self._L.jxamd_op_bilu_solve(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_z)), "jxamd_op_bilu_solve")

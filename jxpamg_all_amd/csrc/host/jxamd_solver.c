@@ -1769,22 +1769,35 @@ int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, do
     jxamd_dev_free(dx);
     jxamd_dev_free(dy);
     CK(e);
-    double wdef[JXAMD_MAX_LEVELS][8];
-    memcpy(wdef, s->poly_w, sizeof(wdef));
-    double fdef = 0.0, flrn = 1e300;
-    CK(jxamd_solver_vcycle_factor(s, 4, &fdef));
-    int ok = 1;
-    for (int l = 0; l < Ls; l++)
+    /* the network predicts the polynomial's roots in units of lambda_max:
+     * w_k = 1 / (tau_k lambda_max); level by level, learned steps replace the
+     * defaults only where they lower the measured V-cycle contraction */
+    double best = 0.0;
+    CK(jxamd_solver_vcycle_factor(s, 4, &best));
+    const double fdef = best;
+    int kept = 0;
+    for (int l = 0; l < Ls; l++) {
+        double keep[8], w[8];
+        int ok = 1;
         for (int k = 0; k < JXAMD_MLP_NOUT; k++) {
-            const double w = (double)hy[l * JXAMD_MLP_NOUT + k] / s->lam[l];
-            if (!(w > 0.0) || !(w < 1e3)) ok = 0; /* also rejects NaN */
-            s->poly_w[l][k] = w;
+            const double tau = (double)hy[l * JXAMD_MLP_NOUT + k];
+            if (!(tau > 5e-3) || !(tau < 4.0)) ok = 0; /* also rejects NaN */
+            w[k] = 1.0 / (tau * s->lam[l]);
+            keep[k] = s->poly_w[l][k];
         }
-    if (ok) CK(jxamd_solver_vcycle_factor(s, 4, &flrn));
-    const int keep = ok && flrn < fdef;
-    if (!keep) memcpy(s->poly_w, wdef, sizeof(wdef));
-    if (used) *used = keep;
-    if (factor_learned) *factor_learned = ok ? flrn : -1.0;
+        if (!ok) continue;
+        for (int k = 0; k < JXAMD_MLP_NOUT; k++) s->poly_w[l][k] = w[k];
+        double f = 0.0;
+        CK(jxamd_solver_vcycle_factor(s, 4, &f));
+        if (f < best) {
+            best = f;
+            kept++;
+        } else {
+            for (int k = 0; k < JXAMD_MLP_NOUT; k++) s->poly_w[l][k] = keep[k];
+        }
+    }
+    if (used) *used = kept;
+    if (factor_learned) *factor_learned = best;
     if (factor_default) *factor_default = fdef;
     return 0;
 }

@@ -1,13 +1,15 @@
 """Learned polynomial AMG smoother: data from the plain solver, GPU training.
 
 The network (csrc/hip/jxamd_mlp.hip) maps 16 features of an AMG level to the
-four step sizes of its degree-4 polynomial smoother, as w_k * lambda_max.
-Training data come from the solver itself: on seeded 16x16x8 systems the
-step sizes of each level are chosen, level by level, from Chebyshev
-candidates on [lo, hi] x lambda_max by the V-cycle contraction factor the
-solver measures on the GPU; the network learns features -> the best steps.
-No reference counterpart; the solver keeps learned steps only when they
-contract the V-cycle better than its default Chebyshev steps.
+four roots of its degree-4 polynomial smoother in units of lambda_max
+(steps w_k = 1 / (tau_k lambda_max)).  Training data come from the solver
+itself: on seeded config-1-style systems (iid lognormal permeability, a few
+grid sizes so that coarse levels appear too) the roots of each level are
+chosen, level by level, from Chebyshev candidates on [lo, hi] by the V-cycle
+contraction factor the solver measures on the GPU; the network learns
+features -> the best roots.  No reference counterpart; the solver keeps
+learned steps on a level only when they contract the V-cycle better than
+its default Chebyshev steps.
 """
 import ctypes
 import math
@@ -39,8 +41,8 @@ def training_system(seed, nx=16, ny=16, nz=8):
 
 
 def level_targets(solver, ncycles=4):
-    """Greedy per-level search of the best Chebyshev steps by the measured
-    V-cycle contraction; returns (features [L][16], targets [L][4],
+    """Greedy per-level search of the best Chebyshev roots by the measured
+    V-cycle contraction; returns (features [L][16], roots [L][4],
     best factor, default factor)."""
     Ls = max(solver.num_levels - 1, 1)
     f_def = solver.vcycle_factor(ncycles)
@@ -57,14 +59,19 @@ def level_targets(solver, ncycles=4):
                     best = (f, z)
         solver.set_poly(l, best[1] / lam)
         feats.append(solver.poly_features(l))
-        targets.append(best[1])
+        targets.append(1.0 / best[1])
     return np.array(feats, np.float32), np.array(targets, np.float32), best[0], f_def
 
 
-def make_dataset(seeds, precision=1, **grid):
+GRIDS = ((16, 16, 8), (24, 24, 12), (20, 20, 10), (32, 16, 8))
+
+
+def make_dataset(seeds, precision=1, grids=GRIDS):
+    """Labelled levels of config-1-style systems, grid sizes cycling over grids."""
     X, T, info = [], [], []
-    for sd in seeds:
-        Ad, bd, H = training_system(sd, **grid)
+    for q, sd in enumerate(seeds):
+        nx, ny, nz = grids[q % len(grids)]
+        Ad, bd, H = training_system(sd, nx=nx, ny=ny, nz=nz)
         s = Solver(Ad, H, default_params(poly_degree=NOUT, precond_precision=precision))
         f, t, fb, fd = level_targets(s)
         s.close()

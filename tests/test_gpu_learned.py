@@ -57,16 +57,16 @@ def test_learned_smoother_gain_on_unseen_systems():
     V-cycle contraction and the GMRES iterations of learned and default
     steps on unseen seeds; learned steps are kept only where they win."""
     print("[gpu] learned smoother: dataset", flush=True)
-    X, T, info = L.make_dataset(range(1001, 1009), precision=1)
+    X, T, info = L.make_dataset(range(1001, 1021), precision=1)
     print(f"[gpu] dataset {X.shape} best/default factors {[(round(b, 3), round(d, 3)) for _, b, d in info]}",
           flush=True)
     net = L.MLP(seed=0)
     losses = net.fit(X, T, steps=3000, lr=3e-3)
     print(f"[gpu] training loss {losses[0]:.4f} -> {losses[-1]:.5f}", flush=True)
-    assert losses[-1] < losses[0]
+    assert losses[-1] < 0.2 * losses[0]
     gains = []
-    for seed in (2001, 2002, 2003):
-        Ad, bd, H = L.training_system(seed, nx=20, ny=20, nz=10)
+    for seed, grid in ((2001, (20, 20, 10)), (2002, (24, 24, 12)), (2003, (28, 20, 10))):
+        Ad, bd, H = L.training_system(seed, *grid)
         p = jx.default_params(poly_degree=4, precond_precision=1, tol=1e-6)
         s_def = jx.Solver(Ad, H, p)
         _, st_def, _ = s_def.solve(bd)
@@ -80,13 +80,14 @@ def test_learned_smoother_gain_on_unseen_systems():
               f"{st.iterations} vs {st_def.iterations}", flush=True)
         assert abs(f_d - f_def) < 1e-6
         assert st.converged == 1 and true <= 1e-6
+        assert abs(s.vcycle_factor() - f_l) < 1e-6  # the steps in place give the reported factor
         if used:
             assert f_l < f_d
         else:
             # fallback: the default steps are back in place
-            assert abs(s.vcycle_factor() - f_d) < 1e-6
+            assert abs(f_l - f_d) < 1e-6
             assert st.iterations == st_def.iterations
-        gains.append(f_d - (f_l if used else f_d))
+        gains.append(f_d - f_l)
         s.close()
     # the network must win somewhere (otherwise it adds nothing over the defaults)
     assert max(gains) > 0.0
