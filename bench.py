@@ -138,7 +138,9 @@ def main():
     if not args.no_learned:
         # mixed precision + learned polynomial smoother: FP32 stage 1 with the
         # network's step sizes (trained here, on the GPU, on data the plain
-        # solver labels itself), FP64 stage 2 and GMRES
+        # solver labels itself), FP64 stage 2 and GMRES; stage 2 as in the
+        # reference's mixed driver (two block sweeps, w 1.0, omega 0.8:
+        # src/test_bsr_cprgmres_hybrid.c:373-375, 648)
         from jxpamg_all_amd import learned as lrn
         t0 = time.time()
         Xtr, Ttr, _ = lrn.make_dataset(range(1001, 1001 + args.train_seeds), precision=1)
@@ -147,8 +149,8 @@ def main():
         t_train = time.time() - t0
         log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
         legs = {}
-        for name, kw in (("mixed_default_poly", dict(precond_precision=1, poly_degree=4)),
-                         ("mixed_learned", dict(precond_precision=1, poly_degree=4))):
+        mixed_kw = dict(precond_precision=1, poly_degree=4, stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8)
+        for name, kw in (("mixed_default_poly", mixed_kw), ("mixed_learned", mixed_kw)):
             pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, **kw)
             sm = jx.Solver(Ad, H, pm)
             used = None

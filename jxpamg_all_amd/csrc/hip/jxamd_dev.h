@@ -42,12 +42,16 @@ int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, 
                      const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
                      const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
                      unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
+/* stage 2: x^(-1) into slice order, then K fused sweeps (columns of the
+ * layout are positions, ~pos for lower neighbours; xs holds K slice-ordered
+ * vectors of xstride doubles; the last sweep also writes xout in row order) */
+int jxamd_k_bhgs_gather(int npos, int b, const int *perm, const double *x0, const double *xp, int pidx, double *out,
+                        void *stream);
 int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,
-                       const double *val, const double *dinv, const double *rhs, const double *x0, const double *xp,
-                       int pidx, double *xs, long long xstride, double *xout, const int *dlo_ptr,
-                       const int *dlo_list, const int *dall_ptr, const int *dall_list, int *sflag, int epoch,
-                       double w, double omega, unsigned long long *ticket, unsigned long long *base, int *err,
-                       void *stream);
+                       const double *val, const double *dinv, const double *rhs, const double *xsl0, double *xs,
+                       long long xstride, double *xout, const int *dlo_ptr, const int *dlo_list, const int *dall_ptr,
+                       const int *dall_list, int *sflag, int epoch, double w, double omega,
+                       unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
 /* BILU(0) block triangular solve (vt: F64 or F32 factor storage) */
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                  const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,

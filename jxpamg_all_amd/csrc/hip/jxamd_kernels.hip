@@ -283,9 +283,14 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
 //   tmp = omega*b_i - sum_{j<i} A_ij x^(k)_j - sum_{j>i} A_ij x^(k-1)_j
 //         + (1-omega) sum_{j!=i} A_ij x^(k-1)_j
 //   x^(k)_i = (1 - w*omega) x^(k-1)_i + w * Dinv_i tmp
-// x^(-1) is x0, or, when x0 == nullptr, the prolongated pressure vector xp
-// (pressure slot pidx, zeros elsewhere).  x^(k) for k < K-1 goes to
-// xs + k*xstride, x^(K-1) to xout.
+// Every x^(k) lives in slice order: the B values of the row at position p
+// (perm[p] = i) sit at xs + k*xstride + p*B; x^(-1) is xsl0 in the same
+// order.  A slice's 64 rows fill exactly 12 (B = 3) or 8/16 (B = 2/4) whole
+// 128-byte lines, so a line is read only after the flag of the one slice that
+// writes it: ordinary cached loads then see final values, and only the
+// stores (written through at device scope) and the flags need coherence.
+// Columns of the layout are positions, bit-inverted (~pos) for lower
+// neighbours (j < i); the final sweep also stores x in row order into xout.
 // Work items are virtual slices v = k*nsl + s taken in the order vorder (the
 // identity if nullptr), a topological order of
 //   (k, s) after (k, s') for s' in dlo(s)   (slices of lower neighbours)
@@ -297,11 +302,11 @@ template <int B>
 __global__ void __launch_bounds__(256)
 k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__restrict__ perm,
              const int *__restrict__ sptr, const int *__restrict__ col, const double *__restrict__ val,
-             const double *__restrict__ dinv, const double *__restrict__ rhs, const double *__restrict__ x0,
-             const double *__restrict__ xp, int pidx, double *xs, long long xstride, double *xout,
-             const int *__restrict__ dlo_ptr, const int *__restrict__ dlo_list, const int *__restrict__ dall_ptr,
-             const int *__restrict__ dall_list, int *sflag, int epoch, double w, double omega,
-             unsigned long long *ticket, unsigned long long base, int *err)
+             const double *__restrict__ dinv, const double *__restrict__ rhs, const double *__restrict__ xsl0,
+             double *xs, long long xstride, double *__restrict__ xout, const int *__restrict__ dlo_ptr,
+             const int *__restrict__ dlo_list, const int *__restrict__ dall_ptr, const int *__restrict__ dall_list,
+             int *sflag, int epoch, double w, double omega, unsigned long long *ticket, unsigned long long base,
+             int *err)
 {
     const int lane = threadIdx.x % WAVE;
     const int K = nvs / nsl;
@@ -310,10 +315,9 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
         if (t < 0 || t >= nvs) break;
         const int v = vorder ? vorder[t] : t;
         const int k = v / nsl, s = v - k * nsl;
-        double *xn = k == K - 1 ? xout : xs + (size_t)k * xstride;
-        const double *xo = k == 0 ? x0 : xs + (size_t)(k - 1) * xstride;
-        const bool prev = k > 0; // old values written in this launch
-        if (prev) wait_slices(dall_ptr, dall_list, sflag + (size_t)(k - 1) * nsl, s, epoch, lane, err);
+        double *xn = xs + (size_t)k * xstride;
+        const double *xo = k == 0 ? xsl0 : xs + (size_t)(k - 1) * xstride;
+        if (k > 0) wait_slices(dall_ptr, dall_list, sflag + (size_t)(k - 1) * nsl, s, epoch, lane, err);
         const int p = s * WAVE + lane;
         const int i = perm[p];
         const int k0 = sptr[s], k1 = sptr[s + 1];
@@ -323,13 +327,13 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
         if (i >= 0) {
             // all old-value terms, and the current-value terms of upper neighbours
             for (int kk = k0; kk < k1; kk++) {
-                const int j = col[(size_t)kk * WAVE + lane];
-                if (j == i) continue; // padding slot
+                const int c = col[(size_t)kk * WAVE + lane];
+                const bool lower = c < 0;
+                const int jp = lower ? ~c : c;
+                if (jp == p) continue; // padding slot
                 double xold[B];
 #pragma unroll
-                for (int q = 0; q < B; q++)
-                    xold[q] = prev ? ld_coherent(xo + (size_t)j * B + q)
-                                   : (xo ? xo[(size_t)j * B + q] : (q == pidx ? xp[j] : 0.0));
+                for (int q = 0; q < B; q++) xold[q] = xo[(size_t)jp * B + q];
                 const double *vv = val + (size_t)kk * B * B * WAVE + lane;
 #pragma unroll
                 for (int r = 0; r < B; r++) {
@@ -337,7 +341,7 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
                     for (int q = 0; q < B; q++) {
                         const double a = vv[(r * B + q) * WAVE];
                         res2[r] += a * xold[q];
-                        if (j > i) res0[r] -= a * xold[q];
+                        if (!lower) res0[r] -= a * xold[q];
                     }
                 }
             }
@@ -346,8 +350,7 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
         if (i >= 0) {
 #pragma unroll
             for (int r = 0; r < B; r++) {
-                xi[r] = prev ? ld_coherent(xo + (size_t)i * B + r)
-                             : (xo ? xo[(size_t)i * B + r] : (r == pidx ? xp[i] : 0.0));
+                xi[r] = xo[(size_t)p * B + r];
                 rb[r] = rhs[(size_t)i * B + r];
             }
 #pragma unroll
@@ -356,11 +359,12 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
         wait_slices(dlo_ptr, dlo_list, sflag + (size_t)k * nsl, s, epoch, lane, err);
         if (i >= 0) {
             for (int kk = k0; kk < k1; kk++) {
-                const int j = col[(size_t)kk * WAVE + lane];
-                if (j >= i) continue;
+                const int c = col[(size_t)kk * WAVE + lane];
+                if (c >= 0) continue;
+                const int jp = ~c;
                 double xc[B];
 #pragma unroll
-                for (int q = 0; q < B; q++) xc[q] = ld_coherent(xn + (size_t)j * B + q);
+                for (int q = 0; q < B; q++) xc[q] = xn[(size_t)jp * B + q];
                 const double *vv = val + (size_t)kk * B * B * WAVE + lane;
 #pragma unroll
                 for (int r = 0; r < B; r++) {
@@ -377,11 +381,28 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
                 double o = 0.0;
 #pragma unroll
                 for (int q = 0; q < B; q++) o += D[r * B + q] * tmp[q];
-                st_coherent(xn + (size_t)i * B + r, xi[r] * prod + w * o);
+                const double xv = xi[r] * prod + w * o;
+                st_coherent(xn + (size_t)p * B + r, xv);
+                if (k == K - 1) xout[(size_t)i * B + r] = xv;
             }
         }
         st_flag_release(sflag + (size_t)k * nsl + s, epoch);
     }
+}
+
+// x^(-1) of the stage-2 sweeps in slice order: out[p*B + q] = x0[i*B + q],
+// or, if x0 == nullptr, the prolongated pressure (q == pidx ? xp[i] : 0),
+// i = perm[p]; padding positions get zeros
+__global__ void k_bhgs_gather(int npos, int B, const int *__restrict__ perm, const double *__restrict__ x0,
+                              const double *__restrict__ xp, int pidx, double *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npos * B) return;
+    const int p = e / B, q = e - p * B;
+    const int i = perm[p];
+    double v = 0.0;
+    if (i >= 0) v = x0 ? x0[(size_t)i * B + q] : (q == pidx ? xp[i] : 0.0);
+    out[e] = v;
 }
 
 // One block triangular solve of BILU(0), sync-free over level-pure slices:
@@ -1031,15 +1052,23 @@ int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, 
     return chk(hipGetLastError());
 }
 
+int jxamd_k_bhgs_gather(int npos, int b, const int *perm, const double *x0, const double *xp, int pidx, double *out,
+                        void *stream)
+{
+    if (npos == 0) return 0;
+    const long n = (long)npos * b;
+    k_bhgs_gather<<<(int)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(npos, b, perm, x0, xp, pidx, out);
+    return chk(hipGetLastError());
+}
+
 int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,
-                       const double *val, const double *dinv, const double *rhs, const double *x0, const double *xp,
-                       int pidx, double *xs, long long xstride, double *xout, const int *dlo_ptr,
-                       const int *dlo_list, const int *dall_ptr, const int *dall_list, int *sflag, int epoch,
-                       double w, double omega, unsigned long long *ticket, unsigned long long *base, int *err,
-                       void *stream)
+                       const double *val, const double *dinv, const double *rhs, const double *xsl0, double *xs,
+                       long long xstride, double *xout, const int *dlo_ptr, const int *dlo_list, const int *dall_ptr,
+                       const int *dall_list, int *sflag, int epoch, double w, double omega,
+                       unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
 {
     if (nsl == 0 || K < 1) return 0;
-    if (K > 1 && (!vorder || !xs || !dall_ptr || !dall_list)) return JXAMD_ERROR_ARG;
+    if (!xsl0 || !xs || !xout || (K > 1 && (!vorder || !dall_ptr || !dall_list))) return JXAMD_ERROR_ARG;
     if ((long long)nsl * K > 0x7fffffffLL) return JXAMD_ERROR_ARG;
     const int nvs = nsl * K;
     const int waves = jxamd_sweep_waves(nvs);
@@ -1047,9 +1076,9 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
     hipStream_t st = (hipStream_t)stream;
     const unsigned long long b0 = *base;
 #define LAUNCH_BHGS(BS)                                                                                                \
-    k_bhgs_multi<BS><<<grid, 256, 0, st>>>(nsl, nvs, vorder, perm, sptr, col, val, dinv, rhs, x0, xp, pidx, xs,       \
-                                           xstride, xout, dlo_ptr, dlo_list, dall_ptr, dall_list, sflag, epoch, w,     \
-                                           omega, ticket, b0, err)
+    k_bhgs_multi<BS><<<grid, 256, 0, st>>>(nsl, nvs, vorder, perm, sptr, col, val, dinv, rhs, xsl0, xs, xstride,      \
+                                           xout, dlo_ptr, dlo_list, dall_ptr, dall_list, sflag, epoch, w, omega,       \
+                                           ticket, b0, err)
     switch (b) {
     case 2: LAUNCH_BHGS(2); break;
     case 3: LAUNCH_BHGS(3); break;

@@ -49,10 +49,10 @@ typedef struct {
     int *sptr, *col;
     double *val;
     /* forward block-HGS order */
-    int gnsl, K;             /* K: stage-2 sweeps fused into one launch */
+    int gnsl, gnpos, K;      /* K: stage-2 sweeps fused into one launch */
     int *gperm, *gsptr, *gcol, *flag, *gdep_ptr, *gdep_list;
     int *gall_ptr, *gall_list, *gvorder; /* neighbourhood slices, sweep-overlapped order */
-    double *gval, *dinv, *xs;  /* xs: x^(k) of the first K-1 sweeps */
+    double *gval, *dinv, *xs;  /* xs: x^(k) of the K sweeps in slice order, then x^(-1) */
     double spmv_bytes, sweep_bytes; /* algorithmic bytes: y = A x, one block-HGS sweep */
 } dbsr;
 
@@ -842,7 +842,24 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             }
             memcpy(dinv + (size_t)i * bb, M, sizeof(double) * bb);
         }
+        /* columns as positions: ~pos for lower neighbours, the own position for padding */
+        {
+            int *inv = (int *)malloc(sizeof(int) * (size_t)nb);
+            if (!inv) { free(v); free(dinv); hsell_free(&h); hsched_free(&sc); return JXAMD_ERROR_MEMORY; }
+            for (int q = 0; q < npos; q++)
+                if (perm[q] >= 0) inv[perm[q]] = q;
+            for (int sl = 0; sl < h.nsl; sl++)
+                for (int k = h.sptr[sl]; k < h.sptr[sl + 1]; k++)
+                    for (int l = 0; l < SELLW; l++) {
+                        const int q = sl * SELLW + l, i = perm[q];
+                        int *c = &h.col[(size_t)k * SELLW + l];
+                        if (i < 0 || *c == i) *c = q;
+                        else *c = *c < i ? ~inv[*c] : inv[*c];
+                    }
+            free(inv);
+        }
         d->gnsl = h.nsl;
+        d->gnpos = npos;
         int rc = jxamd_dev_malloc((void **)&d->gsptr, sizeof(int) * (h.nsl + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->gcol, sizeof(int) * (slots * SELLW + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->gval, sizeof(double) * (slots * bb * SELLW + 1));
@@ -855,6 +872,7 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         if (!rc) rc = upload_ints(&d->gdep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
         if (!rc) rc = upload_ints(&d->gdep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
         d->K = s->prm.stage2_maxit > 1 ? s->prm.stage2_maxit : 1;
+        if (!rc) rc = jxamd_dev_malloc((void **)&d->xs, sizeof(double) * (size_t)(d->K + 1) * npos * b);
         if (!rc) rc = zero_ints(&d->flag, (size_t)sc.nsl * d->K, s->stream);
         if (!rc && d->K > 1) {
             int *nptr = NULL, *nlist = NULL, *ord = NULL;
@@ -863,7 +881,6 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             if (!rc) rc = upload_ints(&d->gall_ptr, nptr, (size_t)sc.nsl + 1, s->stream);
             if (!rc) rc = upload_ints(&d->gall_list, nlist, (size_t)nptr[sc.nsl], s->stream);
             if (!rc) rc = upload_ints(&d->gvorder, ord, (size_t)sc.nsl * d->K, s->stream);
-            if (!rc) rc = jxamd_dev_malloc((void **)&d->xs, sizeof(double) * (size_t)(d->K - 1) * nb * b);
             if (!rc) rc = jxamd_stream_sync(s->stream);
             free(nptr);
             free(nlist);
@@ -1512,12 +1529,15 @@ static int bhgs_run(jxamd_solver *s, int K, const double *rhs, const double *x0,
                     double w, double omega)
 {
     dbsr *d = &s->A;
+    const long long xstride = (long long)d->gnpos * d->b;
+    double *xsl0 = d->xs + (size_t)d->K * xstride;
     s->epoch++;
     int pq = prof_begin(s);
-    int e = jxamd_k_bhgs_multi(d->b, d->gnsl, K, K > 1 ? d->gvorder : NULL, d->gperm, d->gsptr, d->gcol, d->gval,
-                               d->dinv, rhs, x0, xp, s->prm.pressure_index, d->xs, (long long)s->N, xn, d->gdep_ptr,
-                               d->gdep_list, d->gall_ptr, d->gall_list, d->flag, s->epoch, w, omega, s->ticket,
-                               &s->tbase, s->err, s->stream);
+    int e = jxamd_k_bhgs_gather(d->gnpos, d->b, d->gperm, x0, xp, s->prm.pressure_index, xsl0, s->stream);
+    if (!e)
+        e = jxamd_k_bhgs_multi(d->b, d->gnsl, K, K > 1 ? d->gvorder : NULL, d->gperm, d->gsptr, d->gcol, d->gval,
+                               d->dinv, rhs, xsl0, d->xs, xstride, xn, d->gdep_ptr, d->gdep_list, d->gall_ptr,
+                               d->gall_list, d->flag, s->epoch, w, omega, s->ticket, &s->tbase, s->err, s->stream);
     prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, K * d->sweep_bytes);
     return e;
 }

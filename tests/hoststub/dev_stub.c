@@ -41,18 +41,19 @@ int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, 
                      const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
                      unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
 { (void)vt; (void)xt; (void)nsl; (void)perm; (void)sptr; (void)col; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)dep_ptr; (void)dep_list; (void)sflag; (void)epoch; (void)dir; (void)ticket; (void)base; (void)err; (void)stream; return 0; }
+int jxamd_k_bhgs_gather(int npos, int b, const int *perm, const double *x0, const double *xp, int pidx, double *out,
+                        void *stream)
+{ (void)npos; (void)b; (void)perm; (void)x0; (void)xp; (void)pidx; (void)out; (void)stream; return 0; }
 int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,
-                       const double *val, const double *dinv, const double *rhs, const double *x0, const double *xp,
-                       int pidx, double *xs, long long xstride, double *xout, const int *dlo_ptr,
-                       const int *dlo_list, const int *dall_ptr, const int *dall_list, int *sflag, int epoch,
-                       double w, double omega, unsigned long long *ticket, unsigned long long *base, int *err,
-                       void *stream)
+                       const double *val, const double *dinv, const double *rhs, const double *xsl0, double *xs,
+                       long long xstride, double *xout, const int *dlo_ptr, const int *dlo_list, const int *dall_ptr,
+                       const int *dall_list, int *sflag, int epoch, double w, double omega,
+                       unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
 {
-    (void)b; (void)perm; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)x0; (void)xp; (void)pidx;
-    (void)xstride; (void)xout; (void)dlo_ptr; (void)dlo_list; (void)sflag; (void)epoch; (void)w; (void)omega;
-    (void)ticket; (void)base; (void)err; (void)stream;
-    if (K > 1 && (!vorder || !xs || !dall_ptr || !dall_list)) return JXAMD_ERROR_ARG;
-    (void)nsl;
+    (void)b; (void)nsl; (void)perm; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)xstride;
+    (void)dlo_ptr; (void)dlo_list; (void)sflag; (void)epoch; (void)w; (void)omega; (void)ticket; (void)base;
+    (void)err; (void)stream;
+    if (!xsl0 || !xs || !xout || (K > 1 && (!vorder || !dall_ptr || !dall_list))) return JXAMD_ERROR_ARG;
     return 0;
 }
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
