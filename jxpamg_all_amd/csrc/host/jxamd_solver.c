@@ -575,14 +575,14 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsc
 }
 
 /* lanes per row for a matrix of mean row length nnz/n: one (SELL-64) for
- * short rows, else a power of two near a quarter of the row length */
+ * short rows, else a power of two near a third of the row length (at most 32) */
 static int choose_group(long nnz, int n)
 {
     const double m = n > 0 ? (double)nnz / n : 0.0;
     if (m < 12.0) return 1;
-    if (m < 24.0) return 4;
-    if (m < 48.0) return 8;
-    if (m < 96.0) return 16;
+    if (m < 20.0) return 4;
+    if (m < 40.0) return 8;
+    if (m < 80.0) return 16;
     return 32;
 }
 
