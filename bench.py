@@ -149,8 +149,10 @@ def main():
         t_train = time.time() - t0
         log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
         legs = {}
-        mixed_kw = dict(precond_precision=1, poly_degree=4, stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8)
-        for name, kw in (("mixed_default_poly", mixed_kw), ("mixed_learned", mixed_kw)):
+        stage2_kw = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8)
+        mixed_kw = dict(precond_precision=1, poly_degree=4, **stage2_kw)
+        for name, kw in (("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_kw)),
+                         ("mixed_default_poly", mixed_kw), ("mixed_learned", mixed_kw)):
             pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, **kw)
             sm = jx.Solver(Ad, H, pm)
             used = None
