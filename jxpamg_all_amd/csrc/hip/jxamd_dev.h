@@ -45,7 +45,6 @@ int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, 
 /* stage 2: x^(-1) into slice order, then K fused sweeps (columns of the
  * layout are positions, ~pos for lower neighbours; xs holds K slice-ordered
  * vectors of xstride doubles; the last sweep also writes xout in row order) */
-int jxamd_k_bhgs_unset(size_t n, double *x, void *stream); /* the K sweep vectors before a launch */
 int jxamd_k_bhgs_gather(int npos, int b, const int *perm, const double *x0, const double *xp, int pidx, double *out,
                         void *stream);
 int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm, const int *sptr, const int *col,

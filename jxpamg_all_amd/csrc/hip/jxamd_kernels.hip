@@ -55,11 +55,6 @@ __device__ __forceinline__ void st_coherent(double *p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ bool flag_ready(int f, int epoch) { return (int)(f - epoch) >= 0; }
-// Stage-2 values not yet written hold this signalling-NaN pattern; arithmetic
-// never produces it (a NaN result is quiet, bit 51 set).
-#define UNSET_BITS 0x7ff0dead5e7b17adULL
-__device__ __forceinline__ bool is_unset(double v) { return __double_as_longlong(v) == (long long)UNSET_BITS; }
-
 __device__ __forceinline__ float ld_coherent(const float *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -67,26 +62,6 @@ __device__ __forceinline__ float ld_coherent(const float *p)
 __device__ __forceinline__ void st_coherent(float *p, float v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// A stage-2 value whose slice flag has been seen: a cached load, unless the
-// line was cached while still UNSET (a copy fetched in this launch holds
-// either UNSET or the final value, as every value is written once), in which
-// case the device-scope copy is read until it is set (bounded).
-__device__ __forceinline__ double ld_final(const double *p, int *err)
-{
-    double v = *p;
-    int spins = 0;
-    while (is_unset(v)) {
-        v = ld_coherent(p);
-        if (!is_unset(v)) break;
-        if (++spins > SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return v;
 }
 
 // Level matrices of the V-cycle are stored as V (double, float, or __half
@@ -316,11 +291,6 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
 // stores (written through at device scope) and the flags need coherence.
 // Columns of the layout are positions, bit-inverted (~pos) for lower
 // neighbours (j < i); the final sweep also stores x in row order into xout.
-// The K slice-ordered vectors start at the UNSET pattern (k_fill_bits): a
-// lower neighbour of the same sweep is waited for by polling its values with
-// device-scope loads (one fewer store-acknowledge and flag hop per
-// dependency level); slice flags still gate the previous sweep's values,
-// which are read with cached loads.
 // Work items are virtual slices v = k*nsl + s taken in the order vorder (the
 // identity if nullptr), a topological order of
 //   (k, s) after (k, s') for s' in dlo(s)   (slices of lower neighbours)
@@ -363,7 +333,7 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
                 if (jp == p) continue; // padding slot
                 double xold[B];
 #pragma unroll
-                for (int q = 0; q < B; q++) xold[q] = k > 0 ? ld_final(xo + (size_t)jp * B + q, err) : xo[(size_t)jp * B + q];
+                for (int q = 0; q < B; q++) xold[q] = xo[(size_t)jp * B + q];
                 const double *vv = val + (size_t)kk * B * B * WAVE + lane;
 #pragma unroll
                 for (int r = 0; r < B; r++) {
@@ -380,34 +350,21 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
         if (i >= 0) {
 #pragma unroll
             for (int r = 0; r < B; r++) {
-                xi[r] = k > 0 ? ld_final(xo + (size_t)p * B + r, err) : xo[(size_t)p * B + r];
+                xi[r] = xo[(size_t)p * B + r];
                 rb[r] = rhs[(size_t)i * B + r];
             }
 #pragma unroll
             for (int q = 0; q < B * B; q++) D[q] = dinv[(size_t)i * B * B + q];
         }
+        wait_slices(dlo_ptr, dlo_list, sflag + (size_t)k * nsl, s, epoch, lane, err);
         if (i >= 0) {
-            // lower neighbours of this sweep: wait on the values themselves
-            // (device-scope loads until they leave the UNSET pattern)
             for (int kk = k0; kk < k1; kk++) {
                 const int c = col[(size_t)kk * WAVE + lane];
                 if (c >= 0) continue;
                 const int jp = ~c;
                 double xc[B];
 #pragma unroll
-                for (int q = 0; q < B; q++) {
-                    double x = ld_coherent(xn + (size_t)jp * B + q);
-                    int spins = 0;
-                    while (is_unset(x)) {
-                        if (++spins > SPIN_LIMIT) {
-                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        x = ld_coherent(xn + (size_t)jp * B + q);
-                    }
-                    xc[q] = x;
-                }
+                for (int q = 0; q < B; q++) xc[q] = xn[(size_t)jp * B + q];
                 const double *vv = val + (size_t)kk * B * B * WAVE + lane;
 #pragma unroll
                 for (int r = 0; r < B; r++) {
@@ -431,12 +388,6 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
         }
         st_flag_release(sflag + (size_t)k * nsl + s, epoch);
     }
-}
-
-__global__ void k_fill_bits(size_t n, unsigned long long *__restrict__ y, unsigned long long bits)
-{
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = bits;
 }
 
 // x^(-1) of the stage-2 sweeps in slice order: out[p*B + q] = x0[i*B + q],
@@ -1098,15 +1049,6 @@ int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, 
                     nsl, perm, sptr, col, (const V_ *)val, rsc, (const X_ *)diag, (const X_ *)f, fs, (const X_ *)uo,
                     (X_ *)un, dep_ptr, dep_list, sflag, epoch, dir, ticket, *base, err)));
     *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
-    return chk(hipGetLastError());
-}
-
-int jxamd_k_bhgs_unset(size_t n, double *x, void *stream)
-{
-    if (n == 0) return 0;
-    size_t blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    k_fill_bits<<<(int)blocks, 256, 0, (hipStream_t)stream>>>(n, (unsigned long long *)x, UNSET_BITS);
     return chk(hipGetLastError());
 }
 

@@ -1533,8 +1533,7 @@ static int bhgs_run(jxamd_solver *s, int K, const double *rhs, const double *x0,
     double *xsl0 = d->xs + (size_t)d->K * xstride;
     s->epoch++;
     int pq = prof_begin(s);
-    int e = jxamd_k_bhgs_unset((size_t)K * xstride, d->xs, s->stream);
-    if (!e) e = jxamd_k_bhgs_gather(d->gnpos, d->b, d->gperm, x0, xp, s->prm.pressure_index, xsl0, s->stream);
+    int e = jxamd_k_bhgs_gather(d->gnpos, d->b, d->gperm, x0, xp, s->prm.pressure_index, xsl0, s->stream);
     if (!e)
         e = jxamd_k_bhgs_multi(d->b, d->gnsl, K, K > 1 ? d->gvorder : NULL, d->gperm, d->gsptr, d->gcol, d->gval,
                                d->dinv, rhs, xsl0, d->xs, xstride, xn, d->gdep_ptr, d->gdep_list, d->gall_ptr,
