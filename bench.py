@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--train-seeds", type=int, default=20, help="config-1-style systems labelled for training")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three plain device copies of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--stage2-chunk", type=int, default=None,
+                    help="block rows per stage-2 partition (0: one-rank sweeps); default: the package's value")
     ap.add_argument("--relax-chunk", type=int, default=None,
                     help="rows per relax-6 chunk (0: one-thread sweeps); default: the package's large-workload value")
     return ap.parse_args()
@@ -96,7 +98,8 @@ def main():
     log(f"{args.config}: nb={Ad.nb} nnzb={Ad.nnzb} levels={H.num_levels} sizes={[len(m[0]) - 1 for m in H.A]} "
         f"gen+decouple {t_gen:.1f}s amg {t_amg:.1f}s")
     chunk = jx.BENCH_RELAX_CHUNK if args.relax_chunk is None else args.relax_chunk
-    params = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk)
+    s2chunk = jx.BENCH_STAGE2_CHUNK if args.stage2_chunk is None else args.stage2_chunk
+    params = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_chunk=s2chunk)
     t0 = time.time()
     s = jx.Solver(Ad, H, params)
     t_create = time.time() - t0
@@ -149,7 +152,7 @@ def main():
         t_train = time.time() - t0
         log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
         legs = {}
-        stage2_kw = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8)
+        stage2_kw = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk)
         mixed_kw = dict(precond_precision=1, poly_degree=4, **stage2_kw)
         for name, kw in (("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_kw)),
                          ("mixed_default_poly", mixed_kw), ("mixed_learned", mixed_kw)):
@@ -169,6 +172,13 @@ def main():
             sm.close()
         legs["training_s"] = round(t_train, 2)
         mixed = legs
+    # the plain path with the one-rank (sequential-order) stage 2, for reference
+    pe = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_chunk=0)
+    se = jx.Solver(Ad, H, pe)
+    t_e, st_e = timed_solves(se, args.steps)
+    se.close()
+    exact_stage2 = {"s": round(t_e, 4), "iterations": st_e.iterations, "true_rel_res": st_e.true_rel_res}
+    log(f"plain FP64, one-rank stage 2: {t_e * 1e3:.1f} ms iters {st_e.iterations}")
     iters = st.iterations
     assert st.converged == 1 and st.true_rel_res <= 1e-6, "solve did not reach rtol 1e-6"
 
@@ -227,13 +237,13 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"{args.config} {cfg['nx']}x{cfg['ny']}x{cfg['nz']} 3x3 BSR FP64, ABF, "
                                   f"CPR(AMG V-cycle hSGS + 3 block HGS)-GMRES(30) to rtol 1e-6",
-                      "relax6_rows_per_chunk": chunk,
+                      "relax6_rows_per_chunk": chunk, "stage2_rows_per_partition": s2chunk,
                       "cells": Ad.nb, "nnzb": Ad.nnzb, "iterations": iters,
                       "true_rel_res": st.true_rel_res, "amg_levels": [len(m[0]) - 1 for m in H.A],
                       "host_setup_s": {"generate_decouple": round(t_gen, 2), "amg": round(t_amg, 2),
                                        "upload": round(t_create, 2)}},
            "roofline": roofline, "spmv": spmv, "kernel_classes": breakdown, "cpu_baseline": cpu,
-           "mixed_precision_learned": mixed}
+           "mixed_precision_learned": mixed, "plain_fp64_one_rank_stage2": exact_stage2}
     s.close()
     print(json.dumps(out), flush=True)
 

@@ -158,6 +158,12 @@ typedef struct {
      * roots on [0.3, 1.1] x an estimate of lambda_max(D^{-1} A) per level and
      * can be replaced with jxamd_solver_set_poly (e.g. learned ones). */
     int poly_degree;
+    /* Stage-2 block hybrid SOR (type 2) partitions: 0 = the one-rank sweep
+     * (exact sequential order); c = 1..8 = the sweep as run on ceil(nb / c)
+     * ranks, i.e. contiguous partitions of about c block rows, Gauss-Seidel
+     * inside a partition and the other partitions' values at entry
+     * (jxfpamg-0220/csrc/amg/par_relax_bsr.c:296-330, 523-557). */
+    int stage2_chunk;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread
@@ -256,7 +262,8 @@ int jxamd_op_vcycle(jxamd_solver *s, const double *d_f, double *d_u);
 /* z = (LU)^{-1} r with the exact BILU(0) factors of A (stage2_solver_type 5;
  * the reference's iterative BILU: jxfpamg-0220/csrc/ilu/par_bilu_solve.c:360-366) */
 int jxamd_op_bilu_solve(jxamd_solver *s, const double *d_r, double *d_z);
-/* one block hybrid-SOR sweep, jxf_ParBSRHGSRelax forward (par_relax_bsr.c:193-573) */
+/* one block hybrid-SOR sweep, jxf_ParBSRHGSRelax forward (par_relax_bsr.c:193-573),
+ * one-rank or partitioned as set by stage2_chunk */
 int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, double omega);
 /* y = A*x launched reps times back to back on the solver's stream; the mean
  * kernel time per launch from HIP events around the whole batch */

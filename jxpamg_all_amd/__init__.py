@@ -56,7 +56,7 @@ class Params(ctypes.Structure):
                 ("print_level", ctypes.c_int),
                 ("threshold", ctypes.c_double), ("stage2_weight", ctypes.c_double),
                 ("stage2_omega", ctypes.c_double), ("relax_threads", ctypes.c_int), ("relax_chunk", ctypes.c_int),
-                ("precond_precision", ctypes.c_int), ("poly_degree", ctypes.c_int)]
+                ("precond_precision", ctypes.c_int), ("poly_degree", ctypes.c_int), ("stage2_chunk", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -164,6 +164,10 @@ CONFIGS = {
 # (on the 60x220x85 system: 9 iterations against 8 for one thread, and the
 # fastest of chunk lengths 1..16)
 BENCH_RELAX_CHUNK = 4
+# block rows per stage-2 partition on the GPU for the large workloads: the
+# stage-2 sweeps as run on ceil(nb / 4) ranks (par_relax_bsr.c), 9 iterations
+# on the 60x220x85 system for partitions of 4..64 rows
+BENCH_STAGE2_CHUNK = 4
 
 
 def gen_params(**kw):

@@ -390,6 +390,133 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
     }
 }
 
+// One forward block hybrid SOR sweep as run on nparts ranks (contiguous
+// block-row partitions of at most LMAX rows, first nb % nparts one row
+// longer; par_relax_bsr.c:296-330 and :523-557 with the other ranks'
+// neighbours at entry values): per row i of partition P
+//   res  = b_i - sum_{j not in P} A_ij x_old_j
+//   res0 = -sum_{j in P, j<i} A_ij x_new_j - sum_{j in P, j>i} A_ij x_old_j
+//   res2 = sum_{j in P, j!=i} A_ij x_old_j
+//   x_i  = (1 - w omega) x_old_i + w Dinv_i (omega res + res0 + (1-omega) res2)
+// Everything but the x_new terms is row-parallel (phase A, one lane per row
+// over the SpMV layout: SELL-64, all blocks, natural order); the in-partition
+// lower blocks go to LDS and one lane per partition runs the short forward
+// recurrence (phase B).  x_old: xo (B values per row) or, when xo ==
+// nullptr, the prolongated pressure xp (slot pidx, zeros elsewhere).
+template <int B, int LMAX>
+__global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, const int *__restrict__ sptr,
+                                                   const int *__restrict__ col, const double *__restrict__ val,
+                                                   const double *__restrict__ dinv, const double *__restrict__ rhs,
+                                                   const double *__restrict__ xo, const double *__restrict__ xp,
+                                                   int pidx, double *__restrict__ xn, double w, double omega)
+{
+    constexpr int NL = LMAX * (LMAX - 1) / 2; // strictly lower pairs of a partition
+    constexpr int LST = NL * B * B + 1;       // odd stride between partitions
+    extern __shared__ __align__(16) unsigned char smraw[];
+    double *sm = reinterpret_cast<double *>(smraw);
+    double *h = sm, *xold = sm + 256 * B, *Lb = sm + 2 * 256 * B;
+    const int size = nb / nparts, rest = nb - size * nparts;
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * cpb;
+    const int c1 = min(c0 + cpb, nparts);
+    const int row0 = c0 < rest ? c0 * (size + 1) : c0 * size + rest;
+    const int row1 = c1 < rest ? c1 * (size + 1) : c1 * size + rest;
+    const int r = threadIdx.x;
+    const int i = row0 + r;
+    if (i < row1) {
+        const int big = rest * (size + 1);
+        const int c = i < big ? i / (size + 1) : rest + (i - big) / size;
+        const int ns = c < rest ? c * (size + 1) : c * size + rest;
+        const int len = c < rest ? size + 1 : size;
+        const int ti = i - ns;
+        double res[B], res0[B], res2[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            res[q] = rhs[(size_t)i * B + q];
+            res0[q] = 0.0;
+            res2[q] = 0.0;
+        }
+        const int s = i / WAVE, lane = i % WAVE;
+        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+            const int j = col[(size_t)k * WAVE + lane];
+            if (j == i) continue;
+            const double *vv = val + (size_t)k * B * B * WAVE + lane;
+            double xj[B];
+#pragma unroll
+            for (int q = 0; q < B; q++) xj[q] = xo ? xo[(size_t)j * B + q] : (q == pidx ? xp[j] : 0.0);
+            const bool in = j >= ns && j < ns + len;
+            if (in && j < i) {
+                double *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
+#pragma unroll
+                for (int e = 0; e < B * B; e++) L[e] = vv[e * WAVE];
+            }
+#pragma unroll
+            for (int rr = 0; rr < B; rr++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < B; q++) acc += vv[(rr * B + q) * WAVE] * xj[q];
+                if (!in) res[rr] -= acc;
+                else {
+                    res2[rr] += acc;
+                    if (j > i) res0[rr] -= acc;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            h[r * B + q] = omega * res[q] + res0[q] + (1.0 - omega) * res2[q];
+            xold[r * B + q] = xo ? xo[(size_t)i * B + q] : (q == pidx ? xp[i] : 0.0);
+        }
+    }
+    __syncthreads();
+    const int c = c0 + (int)threadIdx.x;
+    if (c < c1) {
+        const int ns = c < rest ? c * (size + 1) : c * size + rest;
+        const int len = c < rest ? size + 1 : size;
+        const int rl = ns - row0;
+        const double *Lc = Lb + (size_t)(c - c0) * LST;
+        const double prod = 1.0 - w * omega;
+        double xnew[LMAX][B];
+#pragma unroll
+        for (int t = 0; t < LMAX; t++) {
+            if (t < len) {
+                double tmp[B];
+#pragma unroll
+                for (int q = 0; q < B; q++) tmp[q] = h[(rl + t) * B + q];
+#pragma unroll
+                for (int u = 0; u < LMAX - 1; u++) {
+                    if (u < t) {
+                        const double *L = Lc + ((t * (t - 1)) / 2 + u) * B * B;
+#pragma unroll
+                        for (int rr = 0; rr < B; rr++)
+#pragma unroll
+                            for (int q = 0; q < B; q++) tmp[rr] -= L[rr * B + q] * xnew[u][q];
+                    }
+                }
+                const double *D = dinv + (size_t)(ns + t) * B * B;
+#pragma unroll
+                for (int rr = 0; rr < B; rr++) {
+                    double o = 0.0;
+#pragma unroll
+                    for (int q = 0; q < B; q++) o += D[rr * B + q] * tmp[q];
+                    xnew[t][rr] = prod * xold[(rl + t) * B + rr] + w * o;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < B; q++) xnew[t][q] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < LMAX; t++)
+            if (t < len)
+#pragma unroll
+                for (int q = 0; q < B; q++) h[(rl + t) * B + q] = xnew[t][q];
+    }
+    __syncthreads();
+    if (i < row1)
+#pragma unroll
+        for (int q = 0; q < B; q++) xn[(size_t)i * B + q] = h[r * B + q];
+}
+
 // x^(-1) of the stage-2 sweeps in slice order: out[p*B + q] = x0[i*B + q],
 // or, if x0 == nullptr, the prolongated pressure (q == pidx ? xp[i] : 0),
 // i = perm[p]; padding positions get zeros
@@ -1087,6 +1214,39 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
     }
 #undef LAUNCH_BHGS
     *base += (unsigned long long)nvs + (unsigned long long)grid * 4;
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_bhgs_part(int b, int nb, int nparts, const int *sptr, const int *col, const double *val,
+                      const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
+                      double w, double omega, void *stream)
+{
+    if (nb == 0) return 0;
+    const int maxlen = jxamd_gs_chunk_lds_rows(nb, nparts);
+    if (maxlen < 1 || maxlen > 8 || (!xo && !xp)) return JXAMD_ERROR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+#define LAUNCH_PART(BS, LM)                                                                                            \
+    do {                                                                                                               \
+        const int cpb = 256 / LM;                                                                                      \
+        const int grid = (nparts + cpb - 1) / cpb;                                                                     \
+        const size_t lds = sizeof(double) * (2 * 256 * BS + (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1));       \
+        k_bhgs_part<BS, LM><<<grid, 256, lds, st>>>(nb, nparts, cpb, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, w,  \
+                                                     omega);                                                           \
+    } while (0)
+#define PART_B(BS)                                                                                                     \
+    do {                                                                                                               \
+        if (maxlen <= 2) LAUNCH_PART(BS, 2);                                                                           \
+        else if (maxlen <= 4) LAUNCH_PART(BS, 4);                                                                      \
+        else LAUNCH_PART(BS, 8);                                                                                       \
+    } while (0)
+    switch (b) {
+    case 2: PART_B(2); break;
+    case 3: PART_B(3); break;
+    case 4: PART_B(4); break;
+    default: return JXAMD_ERROR_ARG;
+    }
+#undef PART_B
+#undef LAUNCH_PART
     return chk(hipGetLastError());
 }
 

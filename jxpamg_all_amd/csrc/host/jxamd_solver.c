@@ -1272,6 +1272,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->precond_precision < 0 || p->precond_precision > 2) return JXAMD_ERROR_ARG;
     if (p->stage2_solver_type != 2 && p->stage2_solver_type != 5) return JXAMD_ERROR_ARG;
     if (p->poly_degree < 0 || p->poly_degree > 8) return JXAMD_ERROR_ARG;
+    if (p->stage2_chunk < 0 || p->stage2_chunk > 8) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -1590,6 +1591,23 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
     const int m = s->prm.stage2_maxit;
     if (m == 0 || s->prm.stage2_solver_type == 5) CK(jxamd_k_prolong(s->nb, b, pi, xp, x, s->stream));
     if (m == 0) return 0;
+    if (s->prm.stage2_chunk > 0 && s->prm.stage2_solver_type == 2) {
+        /* sweeps as on ceil(nb / c) ranks: row-parallel launches, ping-pong
+         * between xtmp and x so that the last one lands in x */
+        dbsr *d = &s->A;
+        const int parts = (s->nb + s->prm.stage2_chunk - 1) / s->prm.stage2_chunk;
+        double *dst = (m % 2 == 1) ? x : s->xtmp;
+        const double *src = NULL;
+        for (int it = 0; it < m; it++) {
+            int pq = prof_begin(s);
+            CK(jxamd_k_bhgs_part(b, s->nb, parts, d->sptr, d->col, d->val, d->dinv, r, src, xp, pi, dst,
+                                 s->prm.stage2_weight, s->prm.stage2_omega, s->stream));
+            prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, d->sweep_bytes);
+            src = dst;
+            dst = (dst == x) ? s->xtmp : x;
+        }
+        return 0;
+    }
     if (s->prm.stage2_solver_type == 5) {
         /* BILU(0) corrections: x += (LU)^{-1} (r - A x) */
         for (int it = 0; it < m; it++) {
@@ -1832,7 +1850,13 @@ int jxamd_op_bilu_solve(jxamd_solver *s, const double *d_r, double *d_z)
 int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, double omega)
 {
     if (!s) return JXAMD_ERROR_ARG;
-    CK(bhgs_run(s, 1, d_b, d_x, NULL, s->xtmp, w, omega));
+    if (s->prm.stage2_chunk > 0) {
+        const int parts = (s->nb + s->prm.stage2_chunk - 1) / s->prm.stage2_chunk;
+        CK(jxamd_k_bhgs_part(s->b, s->nb, parts, s->A.sptr, s->A.col, s->A.val, s->A.dinv, d_b, d_x, NULL, 0, s->xtmp,
+                             w, omega, s->stream));
+    } else {
+        CK(bhgs_run(s, 1, d_b, d_x, NULL, s->xtmp, w, omega));
+    }
     CK(jxamd_k_copy(s->N, s->xtmp, d_x, s->stream));
     return jxamd_stream_sync(s->stream);
 }

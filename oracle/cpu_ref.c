@@ -389,6 +389,53 @@ void cref_bsr_hgs(const cref_bsr *A, double *x, const double *rhs, double *vtemp
     }
 }
 
+/* The same forward sweep as run on nparts MPI ranks, each owning a
+ * contiguous range of block rows (first nb % nparts ranks one row more,
+ * the row_part split of src/jxpamg_interface.c:177-199): Gauss-Seidel on the
+ * rank's rows, neighbours on other ranks at their entry values in the
+ * omega-weighted right-hand side, J/csrc/amg/par_relax_bsr.c:296-330
+ * (Vext_data) and :523-557 (A_offd terms into res). */
+void cref_bsr_hgs_parts(const cref_bsr *A, double *x, const double *rhs, double *vtemp, double w, double omega,
+                        int nparts)
+{
+    const int nb = A->nb, b = A->b, bb = b * b;
+    const long n = (long)nb * b;
+    if (nparts <= 1) {
+        cref_bsr_hgs(A, x, rhs, vtemp, w, omega, 1);
+        return;
+    }
+    double res[8], res0[8], res2[8], tmp[8], out[8];
+    for (long i = 0; i < n; i++) vtemp[i] = x[i];
+    const double prod = 1.0 - w * omega, one_minus_omega = 1.0 - omega;
+    const int size = nb / nparts, rest = nb - size * nparts;
+    for (int t = 0; t < nparts; t++) {
+        const int ns = t < rest ? t * (size + 1) : t * size + rest;
+        const int ne = ns + size + (t < rest ? 1 : 0);
+        for (int i = ns; i < ne; i++) {
+            for (int k = 0; k < b; k++) {
+                res[k] = rhs[(long)i * b + k];
+                res0[k] = 0.0;
+                res2[k] = 0.0;
+            }
+            for (int jj = A->ia[i] + 1; jj < A->ia[i + 1]; jj++) {
+                const int ii = A->ja[jj];
+                if (ii >= ns && ii < ne) {
+                    cref_bsr_block_matvec_stable(-1.0, A->a + (long)jj * bb, x + (long)ii * b, 1.0, res0, b);
+                    cref_bsr_block_matvec_stable(1.0, A->a + (long)jj * bb, vtemp + (long)ii * b, 1.0, res2, b);
+                } else {
+                    cref_bsr_block_matvec_stable(-1.0, A->a + (long)jj * bb, vtemp + (long)ii * b, 1.0, res, b);
+                }
+            }
+            for (int k = 0; k < b; k++) tmp[k] = omega * res[k] + res0[k] + one_minus_omega * res2[k];
+            cref_bsr_block_invmatvec_stable(A->a + (long)A->ia[i] * bb, tmp, out, b);
+            for (int k = 0; k < b; k++) {
+                x[(long)i * b + k] *= prod;
+                x[(long)i * b + k] += w * out[k];
+            }
+        }
+    }
+}
+
 /* One V-cycle, J/csrc/amg/par_cycle.c:20-969 with the CPR settings
  * (J/csrc/cpr/jxf_cpr.c:319-348): cycle_type 1, one sweep per visit, relax
  * type 6 down/up and 9 on the coarsest level, relax_order 1 so every visit
@@ -629,7 +676,7 @@ int cref_cpr_precond(const cref_bsr *A, const cref_hier *H, const cref_params *p
         free(z);
     } else {
         for (int it = 0; it < p->stage2_maxit; it++)
-            cref_bsr_hgs(A, x, r, work, p->stage2_weight, p->stage2_omega, 1);
+            cref_bsr_hgs_parts(A, x, r, work, p->stage2_weight, p->stage2_omega, p->stage2_parts);
     }
     free(rp);
     free(xp);

@@ -46,6 +46,9 @@ typedef struct {
      * corrections x += (LU)^{-1}(r - A x) with the factors below */
     int stage2_solver_type;
     const double *bilu_lu, *bilu_dinv;
+    /* stage2_solver_type 2 with stage2_parts > 1: the sweeps of a run on
+     * stage2_parts ranks (contiguous block-row partitions) */
+    int stage2_parts;
 } cref_params;
 
 typedef struct {
@@ -65,6 +68,8 @@ int cref_gselim(double *A, double *x, int n);
 void cref_bsr_block_matvec_stable(double alpha, const double *mat, const double *v, double beta, double *ov, int b);
 void cref_bsr_block_invmatvec_stable(const double *mat, const double *v, double *ov, int b);
 void cref_bsr_hgs(const cref_bsr *A, double *x, const double *rhs, double *vtemp, double w, double omega, int fwd);
+void cref_bsr_hgs_parts(const cref_bsr *A, double *x, const double *rhs, double *vtemp, double w, double omega,
+                        int nparts);
 int cref_pamg_solve(const cref_hier *H, const double *f, double *u);
 int cref_bilu0_factor(const cref_bsr *A, double *lu, double *dinv);
 void cref_bilu0_solve(const cref_bsr *A, const double *lu, const double *dinv, const double *r, double *z);

@@ -34,7 +34,7 @@ class Params(ctypes.Structure):
                 ("tol", ctypes.c_double), ("a_tol", ctypes.c_double), ("pressure_index", ctypes.c_int),
                 ("stage1_maxit", ctypes.c_int), ("stage2_maxit", ctypes.c_int), ("stage2_weight", ctypes.c_double),
                 ("stage2_omega", ctypes.c_double), ("stage2_solver_type", ctypes.c_int), ("bilu_lu", ctypes.c_void_p),
-                ("bilu_dinv", ctypes.c_void_p)]
+                ("bilu_dinv", ctypes.c_void_p), ("stage2_parts", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -65,6 +65,7 @@ def lib():
             ("cref_bsr_block_matvec_stable", [f64, vp, vp, f64, vp, i32], None),
             ("cref_bsr_block_invmatvec_stable", [vp, vp, vp, i32], None),
             ("cref_bsr_hgs", [vp, vp, vp, vp, f64, f64, i32], None),
+            ("cref_bsr_hgs_parts", [vp, vp, vp, vp, f64, f64, i32], None),
             ("cref_pamg_solve", [vp, vp, vp], i32),
             ("cref_bilu0_factor", [vp, vp, vp], i32),
             ("cref_bilu0_solve", [vp, vp, vp, vp, vp], None),
@@ -100,13 +101,14 @@ class Oracle:
                 self.keep += [ia, ja, a]
 
     def params(self, k_dim=30, max_iter=100, is_check_restarted=1, tol=1e-6, a_tol=0.0, pressure_index=0,
-               stage1_maxit=1, stage2_maxit=3, stage2_weight=0.7, stage2_omega=0.9, stage2_solver_type=2):
+               stage1_maxit=1, stage2_maxit=3, stage2_weight=0.7, stage2_omega=0.9, stage2_solver_type=2,
+               stage2_parts=0):
         lu = dinv = None
         if stage2_solver_type == 5:
             lu, dinv = self.bilu0()
             lu, dinv = _p(lu).value, _p(dinv).value
         return Params(k_dim, max_iter, is_check_restarted, tol, a_tol, pressure_index, stage1_maxit, stage2_maxit,
-                      stage2_weight, stage2_omega, stage2_solver_type, lu, dinv)
+                      stage2_weight, stage2_omega, stage2_solver_type, lu, dinv, stage2_parts)
 
     def bilu0(self):
         """Exact block ILU(0) factors (lu on A's pattern, dinv = U_ii^{-1}), cached."""
@@ -174,6 +176,13 @@ class Oracle:
         x = np.array(x, dtype=np.float64, copy=True)
         work = np.zeros_like(x)
         self.L.cref_bsr_hgs(ctypes.byref(self.bsr), _p(x), _p(np.ascontiguousarray(rhs)), _p(work), w, omega, fwd)
+        return x
+
+    def bhgs_parts(self, x, rhs, nparts, w=0.7, omega=0.9):
+        x = np.array(x, dtype=np.float64, copy=True)
+        work = np.zeros_like(x)
+        self.L.cref_bsr_hgs_parts(ctypes.byref(self.bsr), _p(x), _p(np.ascontiguousarray(rhs)), _p(work), w, omega,
+                                  int(nparts))
         return x
 
     def cpr(self, r, params=None):

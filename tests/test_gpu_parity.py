@@ -280,3 +280,36 @@ def test_bilu0_stage2_vs_oracle(name, prec):
         assert relerr(x, xo) < X_TOL
     assert s.error_flag() == 0
     s.close()
+
+
+@pytest.mark.parametrize("name,chunk", [("c1", 4), ("b4", 3), ("b2", 1), ("lvl3", 8), ("layered", 4)])
+def test_stage2_rank_partitions_vs_oracle(name, chunk):
+    """stage2_chunk = c: stage 2 as run on ceil(nb / c) ranks, against the
+    oracle's restatement; the solve stays within two iterations of the
+    one-rank reference."""
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    p = jx.default_params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]), tol=meta["rtol"],
+                          stage2_maxit=int(meta["stage2_maxit"]), stage2_chunk=chunk, relax_chunk=4)
+    print(f"[gpu] case={name} stage2 chunk {chunk}", flush=True)
+    s = jx.Solver(Ad, H, p)
+    o = Oracle(Ad, H)
+    parts = -(-Ad.nb // chunk)
+    rng = np.random.default_rng(chunk)
+    x0, r = rng.standard_normal(Ad.n), rng.standard_normal(Ad.n)
+    xb = dev(x0)
+    s.bhgs(xb.data_ptr(), dev(r).data_ptr(), 0.7, 0.9)
+    assert relerr(host(xb), o.bhgs_parts(x0, r, parts)) < OP_TOL
+    o.set_relax_threads([min(len(m[0]) - 1, -(-(len(m[0]) - 1) // 4)) for m in H.A])
+    op = o.params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]), tol=meta["rtol"],
+                  stage2_maxit=int(meta["stage2_maxit"]), stage2_parts=parts)
+    xc = dev(np.zeros(Ad.n))
+    s.precond(dev(r).data_ptr(), xc.data_ptr())
+    assert relerr(host(xc), o.cpr(r, op)) < 1e-9
+    x, st, hist = s.solve(bd)
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    print(f"[gpu] {name} stage2 chunk {chunk}: iters {st.iterations} (reference one-rank {int(d['iterations'][0])}) "
+          f"true_rel {true:.3e}", flush=True)
+    assert abs(st.iterations - int(d["iterations"][0])) <= 2
+    assert st.converged == 1 and true <= meta["rtol"]
+    assert s.error_flag() == 0
+    s.close()

@@ -356,3 +356,38 @@ def test_mlp_checker_gradient_matches_finite_differences():
     fd = np.array(fd)
     assert np.max(np.abs(fd - g[idx])) < 0.03 * np.max(np.abs(fd))
     assert np.corrcoef(fd, g[idx])[0, 1] > 0.999
+
+
+@pytest.mark.parametrize("name", ["tiny", "b4", "b2"])
+def test_stage2_rank_partitions(name):
+    """Stage 2 as run on several ranks (GS inside a rank's contiguous rows,
+    other ranks' neighbours at entry values): one partition is the one-rank
+    sweep; one partition per row is omega-damped block Jacobi; with
+    w = omega = 1 each partition is a block lower-triangular solve with the
+    couplings to other partitions on the right-hand side."""
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    o = Oracle(Ad)
+    rng = np.random.default_rng(2)
+    x0, r = rng.standard_normal(Ad.n), rng.standard_normal(Ad.n)
+    assert np.array_equal(o.bhgs_parts(x0, r, 1), o.bhgs(x0, r))
+    S = Ad.to_scipy().tocsr()
+    b, nb = Ad.b, Ad.nb
+    Dblk = sp.block_diag([S[i * b:(i + 1) * b, i * b:(i + 1) * b].toarray() for i in range(nb)]).tocsr()
+    w, om = 0.7, 0.9
+    jac = (1 - w * om) * x0 + w * om * spla.spsolve(Dblk.tocsc(), r - (S - Dblk) @ x0)
+    assert relerr(o.bhgs_parts(x0, r, nb), jac) < 1e-12
+    nparts = 3
+    got = o.bhgs_parts(x0, r, nparts, w=1.0, omega=1.0)
+    size, rest = divmod(nb, nparts)
+    blk_row = np.repeat(np.arange(nb), b)
+    for t in range(nparts):
+        ns = t * size + min(t, rest)
+        ne = ns + size + (1 if t < rest else 0)
+        sl = slice(ns * b, ne * b)
+        Spp = S[sl, sl].tocoo()
+        low = blk_row[ns * b + Spp.col] <= blk_row[ns * b + Spp.row]
+        DL = sp.csr_matrix((Spp.data[low], (Spp.row[low], Spp.col[low])), shape=Spp.shape)
+        Up = sp.csr_matrix((Spp.data[~low], (Spp.row[~low], Spp.col[~low])), shape=Spp.shape)
+        outside = r[sl] - S[sl, :] @ x0 + S[sl, sl] @ x0[sl]
+        expect = spla.spsolve_triangular(DL, outside - Up @ x0[sl], lower=True)
+        assert relerr(got[sl], expect) < 1e-11
