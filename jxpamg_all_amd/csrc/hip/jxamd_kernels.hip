@@ -420,6 +420,9 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
     const int c1 = min(c0 + cpb, nparts);
     const int row0 = c0 < rest ? c0 * (size + 1) : c0 * size + rest;
     const int row1 = c1 < rest ? c1 * (size + 1) : c1 * size + rest;
+    // pairs without a coupling stay zero
+    for (int e = threadIdx.x; e < cpb * LST; e += blockDim.x) Lb[e] = 0.0;
+    __syncthreads();
     const int r = threadIdx.x;
     const int i = row0 + r;
     if (i < row1) {

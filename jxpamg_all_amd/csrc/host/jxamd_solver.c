@@ -767,7 +767,7 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         /* reuse the scalar builder for the pattern, then place block values */
         double *dummy = (double *)calloc((size_t)(A->nnzb > 0 ? A->nnzb : 1), sizeof(double));
         if (!dummy) return JXAMD_ERROR_MEMORY;
-        int e = hsell_build(&h, nb, pat.ia, pat.ja, dummy, NULL, 0, 0);
+        int e = hsell_build(&h, nb, pat.ia, pat.ja, dummy, NULL, 0, 1); /* padding: own row, zero block */
         free(dummy);
         if (e) { hsell_free(&h); return e; }
         size_t slots = (size_t)(h.nsl ? h.sptr[h.nsl] : 0);
