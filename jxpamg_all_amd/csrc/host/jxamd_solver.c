@@ -1328,7 +1328,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_dev_malloc((void **)&s->Bd, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->Xd, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->dsc, sizeof(double) * (2 * (size_t)p->k_dim + 16));
-    if (!e) e = jxamd_dev_malloc((void **)&s->part, sizeof(double) * 1024);
+    if (!e) e = jxamd_dev_malloc((void **)&s->part, sizeof(double) * 4096); /* >= jxamd_dot_grid() blocks */
     if (!e) e = jxamd_dev_malloc((void **)&s->count, sizeof(unsigned int) * 4);
     if (!e) e = jxamd_dev_malloc((void **)&s->ticket, sizeof(unsigned long long));
     if (!e) e = jxamd_dev_malloc((void **)&s->err, sizeof(int) * 4);
