@@ -106,27 +106,7 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
     double acc[B];
 #pragma unroll
     for (int r = 0; r < B; r++) acc[r] = 0.0;
-    // two slots per step: both column loads go out before the gathers that
-    // depend on them, halving the dependent round trips per row
-    int k = k0;
-    for (; k + 1 < k1; k += 2) {
-        const int ca = col[(size_t)k * WAVE + lane];
-        const int cb = col[(size_t)(k + 1) * WAVE + lane];
-        double xa[B], xb[B];
-#pragma unroll
-        for (int t = 0; t < B; t++) {
-            xa[t] = x[(size_t)ca * B + t];
-            xb[t] = x[(size_t)cb * B + t];
-        }
-        const double *va = val + (size_t)k * B * B * WAVE + lane;
-        const double *vb = va + (size_t)B * B * WAVE;
-#pragma unroll
-        for (int r = 0; r < B; r++) {
-#pragma unroll
-            for (int t = 0; t < B; t++) acc[r] += va[(r * B + t) * WAVE] * xa[t] + vb[(r * B + t) * WAVE] * xb[t];
-        }
-    }
-    if (k < k1) {
+    for (int k = k0; k < k1; k++) {
         const int c = col[(size_t)k * WAVE + lane];
         double xv[B];
 #pragma unroll
@@ -1014,35 +994,14 @@ __global__ void __launch_bounds__(256) k_dot_mgs(size_t n, const double *u, doub
     double acc = 0.0;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    // four independent elements per step (loads issued together)
     if (v_prev) {
-        double acc1 = 0.0;
-        for (; i + 3 * stride < n; i += 4 * stride) {
-            const size_t i1 = i + stride, i2 = i + 2 * stride, i3 = i + 3 * stride;
-            const double y0 = y[i] - h * v_prev[i], y1 = y[i1] - h * v_prev[i1];
-            const double y2 = y[i2] - h * v_prev[i2], y3 = y[i3] - h * v_prev[i3];
-            y[i] = y0;
-            y[i1] = y1;
-            y[i2] = y2;
-            y[i3] = y3;
-            acc += u[i] * y0 + u[i2] * y2;
-            acc1 += u[i1] * y1 + u[i3] * y3;
-        }
         for (; i < n; i += stride) {
             double yi = y[i] - h * v_prev[i];
             y[i] = yi;
             acc += u[i] * yi;
         }
-        acc += acc1;
     } else {
-        double acc1 = 0.0;
-        for (; i + 3 * stride < n; i += 4 * stride) {
-            const size_t i1 = i + stride, i2 = i + 2 * stride, i3 = i + 3 * stride;
-            acc += u[i] * y[i] + u[i2] * y[i2];
-            acc1 += u[i1] * y[i1] + u[i3] * y[i3];
-        }
         for (; i < n; i += stride) acc += u[i] * y[i];
-        acc += acc1;
     }
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
@@ -1509,7 +1468,7 @@ int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *str
 int jxamd_dot_grid(size_t n)
 {
     size_t b = (n + 255) / 256;
-    if (b > 2048) b = 2048;
+    if (b > 1024) b = 1024;
     return b < 1 ? 1 : (int)b;
 }
 
