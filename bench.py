@@ -143,7 +143,7 @@ def main():
         # network's step sizes (trained here, on the GPU, on data the plain
         # solver labels itself), FP64 stage 2 and GMRES; stage 2 as in the
         # reference's mixed driver (two block sweeps, w 1.0, omega 0.8:
-        # src/test_bsr_cprgmres_hybrid.c:373-375, 648)
+        # src/test_bsr_cprgmres_hybrid.c:373-375, 648), its blocks stored in FP32
         from jxpamg_all_amd import learned as lrn
         t0 = time.time()
         Xtr, Ttr, _ = lrn.make_dataset(range(1001, 1001 + args.train_seeds), precision=1)
@@ -152,7 +152,7 @@ def main():
         t_train = time.time() - t0
         log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
         legs = {}
-        stage2_kw = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk)
+        stage2_kw = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
         mixed_kw = dict(precond_precision=1, poly_degree=4, **stage2_kw)
         for name, kw in (("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_kw)),
                          ("mixed_default_poly", mixed_kw), ("mixed_learned", mixed_kw)):

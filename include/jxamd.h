@@ -164,6 +164,11 @@ typedef struct {
      * inside a partition and the other partitions' values at entry
      * (jxfpamg-0220/csrc/amg/par_relax_bsr.c:296-330, 523-557). */
     int stage2_chunk;
+    /* Storage of the partitioned stage-2 sweeps' matrix blocks and inverted
+     * diagonal blocks: 0 FP64, 1 FP32 (FP64 vectors and arithmetic); used
+     * with stage2_chunk > 0.  No reference counterpart (its mixed driver
+     * keeps stage 2 in FP64). */
+    int stage2_precision;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread

@@ -53,8 +53,8 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
                        const int *dall_list, int *sflag, int epoch, double w, double omega,
                        unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
 /* one stage-2 sweep as on nparts ranks (partitions of at most 8 rows) */
-int jxamd_k_bhgs_part(int b, int nb, int nparts, const int *sptr, const int *col, const double *val,
-                      const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
+int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
+                      const void *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
                       double w, double omega, void *stream);
 /* BILU(0) block triangular solve (vt: F64 or F32 factor storage) */
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,

@@ -403,10 +403,10 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
 // lower blocks go to LDS and one lane per partition runs the short forward
 // recurrence (phase B).  x_old: xo (B values per row) or, when xo ==
 // nullptr, the prolongated pressure xp (slot pidx, zeros elsewhere).
-template <int B, int LMAX>
+template <int B, int LMAX, typename V>
 __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, const int *__restrict__ sptr,
-                                                   const int *__restrict__ col, const double *__restrict__ val,
-                                                   const double *__restrict__ dinv, const double *__restrict__ rhs,
+                                                   const int *__restrict__ col, const V *__restrict__ val,
+                                                   const V *__restrict__ dinv, const double *__restrict__ rhs,
                                                    const double *__restrict__ xo, const double *__restrict__ xp,
                                                    int pidx, double *__restrict__ xn, double w, double omega)
 {
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const int j = col[(size_t)k * WAVE + lane];
             if (j == i) continue;
-            const double *vv = val + (size_t)k * B * B * WAVE + lane;
+            const V *vv = val + (size_t)k * B * B * WAVE + lane;
             double xj[B];
 #pragma unroll
             for (int q = 0; q < B; q++) xj[q] = xo ? xo[(size_t)j * B + q] : (q == pidx ? xp[j] : 0.0);
@@ -450,13 +450,13 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
             if (in && j < i) {
                 double *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
 #pragma unroll
-                for (int e = 0; e < B * B; e++) L[e] = vv[e * WAVE];
+                for (int e = 0; e < B * B; e++) L[e] = (double)vv[e * WAVE];
             }
 #pragma unroll
             for (int rr = 0; rr < B; rr++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int q = 0; q < B; q++) acc += vv[(rr * B + q) * WAVE] * xj[q];
+                for (int q = 0; q < B; q++) acc += (double)vv[(rr * B + q) * WAVE] * xj[q];
                 if (!in) res[rr] -= acc;
                 else {
                     res2[rr] += acc;
@@ -495,12 +495,12 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
                             for (int q = 0; q < B; q++) tmp[rr] -= L[rr * B + q] * xnew[u][q];
                     }
                 }
-                const double *D = dinv + (size_t)(ns + t) * B * B;
+                const V *D = dinv + (size_t)(ns + t) * B * B;
 #pragma unroll
                 for (int rr = 0; rr < B; rr++) {
                     double o = 0.0;
 #pragma unroll
-                    for (int q = 0; q < B; q++) o += D[rr * B + q] * tmp[q];
+                    for (int q = 0; q < B; q++) o += (double)D[rr * B + q] * tmp[q];
                     xnew[t][rr] = prod * xold[(rl + t) * B + rr] + w * o;
                 }
             } else {
@@ -1220,10 +1220,11 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
     return chk(hipGetLastError());
 }
 
-int jxamd_k_bhgs_part(int b, int nb, int nparts, const int *sptr, const int *col, const double *val,
-                      const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
+int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
+                      const void *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
                       double w, double omega, void *stream)
 {
+    if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32) return JXAMD_ERROR_ARG;
     if (nb == 0) return 0;
     const int maxlen = jxamd_gs_chunk_lds_rows(nb, nparts);
     if (maxlen < 1 || maxlen > 8 || (!xo && !xp)) return JXAMD_ERROR_ARG;
@@ -1233,8 +1234,12 @@ int jxamd_k_bhgs_part(int b, int nb, int nparts, const int *sptr, const int *col
         const int cpb = 256 / LM;                                                                                      \
         const int grid = (nparts + cpb - 1) / cpb;                                                                     \
         const size_t lds = sizeof(double) * (2 * 256 * BS + (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1));       \
-        k_bhgs_part<BS, LM><<<grid, 256, lds, st>>>(nb, nparts, cpb, sptr, col, val, dinv, rhs, xo, xp, pidx, xn, w,  \
-                                                     omega);                                                           \
+        if (vt == JXAMD_T_F64)                                                                                        \
+            k_bhgs_part<BS, LM, double><<<grid, 256, lds, st>>>(nb, nparts, cpb, sptr, col, (const double *)val,      \
+                                                                (const double *)dinv, rhs, xo, xp, pidx, xn, w, omega); \
+        else                                                                                                           \
+            k_bhgs_part<BS, LM, float><<<grid, 256, lds, st>>>(nb, nparts, cpb, sptr, col, (const float *)val,        \
+                                                               (const float *)dinv, rhs, xo, xp, pidx, xn, w, omega);  \
     } while (0)
 #define PART_B(BS)                                                                                                     \
     do {                                                                                                               \

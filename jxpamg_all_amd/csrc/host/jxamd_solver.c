@@ -53,7 +53,8 @@ typedef struct {
     int *gperm, *gsptr, *gcol, *flag, *gdep_ptr, *gdep_list;
     int *gall_ptr, *gall_list, *gvorder; /* neighbourhood slices, sweep-overlapped order */
     double *gval, *dinv, *xs;  /* xs: x^(k) of the K sweeps in slice order, then x^(-1) */
-    double spmv_bytes, sweep_bytes; /* algorithmic bytes: y = A x, one block-HGS sweep */
+    void *val32, *dinv32;      /* FP32 copies for the partitioned stage 2 (stage2_precision 1) */
+    double spmv_bytes, sweep_bytes, sweep_bytes32; /* algorithmic bytes: y = A x, one block-HGS sweep (FP64, FP32 blocks) */
 } dbsr;
 
 /* one BILU(0) triangle on the device: level-ordered block SELL of the
@@ -751,6 +752,20 @@ static int check_csr(const jxamd_csr *c, int square)
     return 0;
 }
 
+/* upload n doubles as type vt (F64 or F32) */
+static int upload_real(void **dst, const double *src, size_t n, int vt, void *st)
+{
+    CK(jxamd_dev_malloc(dst, type_size(vt) * (n ? n : 1)));
+    if (vt == JXAMD_T_F64) return jxamd_h2d(*dst, src, sizeof(double) * n, st);
+    float *t = (float *)malloc(sizeof(float) * (n ? n : 1));
+    if (!t) return JXAMD_ERROR_MEMORY;
+    for (size_t k = 0; k < n; k++) t[k] = (float)src[k];
+    int rc = jxamd_h2d(*dst, t, sizeof(float) * n, st);
+    if (!rc) rc = jxamd_stream_sync(st);
+    free(t);
+    return rc;
+}
+
 /* ---------------------------------------------------------------------- */
 /* create / destroy                                                        */
 /* ---------------------------------------------------------------------- */
@@ -788,12 +803,15 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         d->nsl = h.nsl;
         d->spmv_bytes = (double)A->nnzb * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 16.0 * nb * b;
         d->sweep_bytes = d->spmv_bytes + 16.0 * nb * b + 8.0 * nb * bb; /* + old x, rhs, Dinv */
+        d->sweep_bytes32 = (double)A->nnzb * (4.0 * bb + 4.0) + 4.0 * (nb + 1) + 32.0 * nb * b + 4.0 * nb * bb;
         int rc = jxamd_dev_malloc((void **)&d->sptr, sizeof(int) * (h.nsl + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->col, sizeof(int) * (slots * SELLW + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->val, sizeof(double) * (slots * bb * SELLW + 1));
         if (!rc) rc = jxamd_h2d(d->sptr, h.sptr, sizeof(int) * (h.nsl + 1), s->stream);
         if (!rc) rc = jxamd_h2d(d->col, h.col, sizeof(int) * slots * SELLW, s->stream);
         if (!rc) rc = jxamd_h2d(d->val, v, sizeof(double) * slots * bb * SELLW, s->stream);
+        if (!rc && s->prm.stage2_chunk > 0 && s->prm.stage2_precision == 1)
+            rc = upload_real(&d->val32, v, slots * bb * SELLW, JXAMD_T_F32, s->stream);
         if (!rc) rc = jxamd_stream_sync(s->stream);
         free(v);
         hsell_free(&h);
@@ -869,6 +887,8 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         if (!rc) rc = jxamd_h2d(d->gval, v, sizeof(double) * slots * bb * SELLW, s->stream);
         if (!rc) rc = upload_ints(&d->gperm, perm, (size_t)npos, s->stream);
         if (!rc) rc = jxamd_h2d(d->dinv, dinv, sizeof(double) * (size_t)nb * bb, s->stream);
+        if (!rc && s->prm.stage2_chunk > 0 && s->prm.stage2_precision == 1)
+            rc = upload_real(&d->dinv32, dinv, (size_t)nb * bb, JXAMD_T_F32, s->stream);
         if (!rc) rc = upload_ints(&d->gdep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
         if (!rc) rc = upload_ints(&d->gdep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
         d->K = s->prm.stage2_maxit > 1 ? s->prm.stage2_maxit : 1;
@@ -901,20 +921,6 @@ static void dtri_free(dtri *t)
     jxamd_dev_free(t->perm); jxamd_dev_free(t->sptr); jxamd_dev_free(t->col); jxamd_dev_free(t->dep_ptr);
     jxamd_dev_free(t->dep_list); jxamd_dev_free(t->flag); jxamd_dev_free(t->val); jxamd_dev_free(t->dinv);
     memset(t, 0, sizeof(*t));
-}
-
-/* upload n doubles as type vt (F64 or F32) */
-static int upload_real(void **dst, const double *src, size_t n, int vt, void *st)
-{
-    CK(jxamd_dev_malloc(dst, type_size(vt) * (n ? n : 1)));
-    if (vt == JXAMD_T_F64) return jxamd_h2d(*dst, src, sizeof(double) * n, st);
-    float *t = (float *)malloc(sizeof(float) * (n ? n : 1));
-    if (!t) return JXAMD_ERROR_MEMORY;
-    for (size_t k = 0; k < n; k++) t[k] = (float)src[k];
-    int rc = jxamd_h2d(*dst, t, sizeof(float) * n, st);
-    if (!rc) rc = jxamd_stream_sync(st);
-    free(t);
-    return rc;
 }
 
 /* one triangle of the BILU(0) factors: dir > 0 the unit lower L (forward
@@ -1273,6 +1279,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->stage2_solver_type != 2 && p->stage2_solver_type != 5) return JXAMD_ERROR_ARG;
     if (p->poly_degree < 0 || p->poly_degree > 8) return JXAMD_ERROR_ARG;
     if (p->stage2_chunk < 0 || p->stage2_chunk > 8) return JXAMD_ERROR_ARG;
+    if (p->stage2_precision < 0 || p->stage2_precision > 1) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -1361,6 +1368,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(d->gval); jxamd_dev_free(d->dinv); jxamd_dev_free(d->flag);
     jxamd_dev_free(d->gdep_ptr); jxamd_dev_free(d->gdep_list);
     jxamd_dev_free(d->gall_ptr); jxamd_dev_free(d->gall_list); jxamd_dev_free(d->gvorder); jxamd_dev_free(d->xs);
+    jxamd_dev_free(d->val32); jxamd_dev_free(d->dinv32);
     for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
         dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]); dsell_free(&s->Gc[l]);
         jxamd_dev_free(s->wstep[l]);
@@ -1600,9 +1608,11 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
         const double *src = NULL;
         for (int it = 0; it < m; it++) {
             int pq = prof_begin(s);
-            CK(jxamd_k_bhgs_part(b, s->nb, parts, d->sptr, d->col, d->val, d->dinv, r, src, xp, pi, dst,
-                                 s->prm.stage2_weight, s->prm.stage2_omega, s->stream));
-            prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, d->sweep_bytes);
+            const int f32 = s->prm.stage2_precision == 1;
+            CK(jxamd_k_bhgs_part(b, f32 ? JXAMD_T_F32 : JXAMD_T_F64, s->nb, parts, d->sptr, d->col,
+                                 f32 ? d->val32 : (const void *)d->val, f32 ? d->dinv32 : (const void *)d->dinv, r, src,
+                                 xp, pi, dst, s->prm.stage2_weight, s->prm.stage2_omega, s->stream));
+            prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, f32 ? d->sweep_bytes32 : d->sweep_bytes);
             src = dst;
             dst = (dst == x) ? s->xtmp : x;
         }
@@ -1852,8 +1862,10 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
     if (!s) return JXAMD_ERROR_ARG;
     if (s->prm.stage2_chunk > 0) {
         const int parts = (s->nb + s->prm.stage2_chunk - 1) / s->prm.stage2_chunk;
-        CK(jxamd_k_bhgs_part(s->b, s->nb, parts, s->A.sptr, s->A.col, s->A.val, s->A.dinv, d_b, d_x, NULL, 0, s->xtmp,
-                             w, omega, s->stream));
+        const int f32 = s->prm.stage2_precision == 1;
+        CK(jxamd_k_bhgs_part(s->b, f32 ? JXAMD_T_F32 : JXAMD_T_F64, s->nb, parts, s->A.sptr, s->A.col,
+                             f32 ? s->A.val32 : (const void *)s->A.val, f32 ? s->A.dinv32 : (const void *)s->A.dinv,
+                             d_b, d_x, NULL, 0, s->xtmp, w, omega, s->stream));
     } else {
         CK(bhgs_run(s, 1, d_b, d_x, NULL, s->xtmp, w, omega));
     }

@@ -119,7 +119,7 @@ int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *i
                          void *stream)
 { (void)G; (void)vt; (void)xt; (void)n; (void)nchunks; (void)ia; (void)ja; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
 
-int jxamd_k_bhgs_part(int b, int nb, int nparts, const int *sptr, const int *col, const double *val,
-                      const double *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
+int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
+                      const void *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
                       double w, double omega, void *stream)
-{ (void)b; (void)nb; (void)nparts; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }
+{ (void)b; (void)vt; (void)nb; (void)nparts; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }

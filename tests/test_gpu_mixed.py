@@ -126,3 +126,22 @@ def test_fp16_stage1_quantisation_and_solve(name, chunk):
     assert st.iterations <= 1.5 * st64.iterations + 2
     assert s.error_flag() == 0
     s.close()
+
+
+@pytest.mark.parametrize("name", ["c1", "b4", "lvl3"])
+def test_fp32_stage2_storage(name):
+    """stage2_precision = 1: the partitioned stage-2 sweep with FP32 blocks and
+    inverted diagonals (FP64 vectors and arithmetic) against the FP64 oracle."""
+    d, Ad, bd, H, meta, s = _solver(name, 1, relax_chunk=4, stage2_chunk=4, stage2_precision=1)
+    o = Oracle(Ad, H)
+    parts = -(-Ad.nb // 4)
+    rng = np.random.default_rng(3)
+    x0, r = rng.standard_normal(Ad.n), rng.standard_normal(Ad.n)
+    xb = dev(x0)
+    s.bhgs(xb.data_ptr(), dev(r).data_ptr(), 0.7, 0.9)
+    assert relerr(host(xb), o.bhgs_parts(x0, r, parts)) < 1e-5
+    x, st, _ = s.solve(bd)
+    true = np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd)
+    assert st.converged == 1 and true <= meta["rtol"]
+    assert s.error_flag() == 0
+    s.close()
