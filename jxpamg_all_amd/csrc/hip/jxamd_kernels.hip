@@ -106,8 +106,10 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
     double acc[B];
 #pragma unroll
     for (int r = 0; r < B; r++) acc[r] = 0.0;
+    // matrix entries are streamed once: non-temporal loads keep them from
+    // displacing the reused x entries in the caches
     for (int k = k0; k < k1; k++) {
-        const int c = col[(size_t)k * WAVE + lane];
+        const int c = __builtin_nontemporal_load(col + (size_t)k * WAVE + lane);
         double xv[B];
 #pragma unroll
         for (int t = 0; t < B; t++) xv[t] = x[(size_t)c * B + t];
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
 #pragma unroll
         for (int r = 0; r < B; r++) {
 #pragma unroll
-            for (int t = 0; t < B; t++) acc[r] += v[(r * B + t) * WAVE] * xv[t];
+            for (int t = 0; t < B; t++) acc[r] += __builtin_nontemporal_load(v + (r * B + t) * WAVE) * xv[t];
         }
     }
     if (row >= nb) return;
@@ -1010,12 +1012,17 @@ __global__ void __launch_bounds__(256) k_dot_mgs(size_t n, const double *u, doub
     if (threadIdx.x == 0) {
         double b = 0.0;
         for (int w = 0; w < (int)(blockDim.x / WAVE); w++) b += sm[w];
+        // the partial is a device-scope store: wait for it to complete, then
+        // count (no L2 write-back/invalidate, as in st_flag_release)
         __hip_atomic_store(part + blockIdx.x, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned int prev = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        unsigned int prev = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = (prev == gridDim.x - 1);
     }
     __syncthreads();
     if (!last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     if (threadIdx.x < WAVE) {
         double a2 = 0.0;
         for (unsigned int b = threadIdx.x; b < gridDim.x; b += WAVE)
