@@ -71,6 +71,10 @@ __device__ __forceinline__ double cvt(double v) { return v; }
 __device__ __forceinline__ float cvt(float v) { return v; }
 __device__ __forceinline__ float cvt(__half v) { return __half2float(v); }
 template <typename V> struct row_scaled { static constexpr bool value = false; };
+// matrix entries and column indices are streamed once per pass: non-temporal
+// loads keep them from displacing reused vector entries in the caches
+template <typename T> __device__ __forceinline__ T ldnt(const T *p) { return __builtin_nontemporal_load(p); }
+template <> __device__ __forceinline__ __half ldnt<__half>(const __half *p) { return *p; }
 template <> struct row_scaled<__half> { static constexpr bool value = true; };
 
 // XCD-aware tile order for streaming kernels: consecutive tiles land on the
@@ -152,7 +156,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
     X acc = 0;
     for (int k = sptr[s]; k < sptr[s + 1]; k++) {
         const size_t e = (size_t)k * WAVE + lane;
-        acc += (X)cvt(val[e]) * x[(size_t)col[e] * xs];
+        acc += (X)cvt(ldnt(val + e)) * x[(size_t)ldnt(col + e) * xs];
     }
     if (row >= n) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -179,7 +183,7 @@ __global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict_
     if (uo) {
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const size_t e = (size_t)k * WAVE + lane;
-            acc += (X)cvt(val[e]) * uo[col[e]];
+            acc += (X)cvt(ldnt(val + e)) * uo[ldnt(col + e)];
         }
     }
     if (row >= n) return;
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
                 const size_t e = (size_t)k * WAVE + lane;
                 const int j = col[e];
                 const bool dep = dir > 0 ? (j < i) : (j > i);
-                if (!dep && j != i && uo) res -= sc * (X)cvt(val[e]) * uo[j];
+                if (!dep && j != i && uo) res -= sc * (X)cvt(ldnt(val + e)) * uo[j];
             }
         }
         wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
                 const size_t e = (size_t)k * WAVE + lane;
                 const int j = col[e];
                 const bool dep = dir > 0 ? (j < i) : (j > i);
-                if (dep) res -= sc * (X)cvt(val[e]) * ld_coherent(un + j);
+                if (dep) res -= sc * (X)cvt(ldnt(val + e)) * ld_coherent(un + j);
             }
             st_coherent(un + i, d != (X)0 ? res / d : uold_i);
         }
@@ -452,13 +456,13 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
             if (in && j < i) {
                 double *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
 #pragma unroll
-                for (int e = 0; e < B * B; e++) L[e] = (double)vv[e * WAVE];
+                for (int e = 0; e < B * B; e++) L[e] = (double)ldnt(vv + e * WAVE);
             }
 #pragma unroll
             for (int rr = 0; rr < B; rr++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int q = 0; q < B; q++) acc += (double)vv[(rr * B + q) * WAVE] * xj[q];
+                for (int q = 0; q < B; q++) acc += (double)ldnt(vv + (rr * B + q) * WAVE) * xj[q];
                 if (!in) res[rr] -= acc;
                 else {
                     res2[rr] += acc;
@@ -646,7 +650,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
             X xj;
             if (j >= ns && j < i) xj = un[j];
             else xj = uo ? uo[j] : (X)0;
-            res -= sc * (X)cvt(val[e]) * xj;
+            res -= sc * (X)cvt(ldnt(val + e)) * xj;
         }
         un[i] = d != (X)0 ? res / d : (uo ? uo[i] : (X)0);
     }
@@ -667,7 +671,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
             X xj;
             if (j >= ns && j < ne) xj = un[j];
             else xj = uo ? uo[j] : (X)0;
-            res -= sc * (X)cvt(val[e]) * xj;
+            res -= sc * (X)cvt(ldnt(val + e)) * xj;
         }
         un[i] = res / d;
     }
@@ -720,7 +724,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const size_t e = (size_t)k * WAVE + lane;
             const int j = col[e];
-            const X v = sc * (X)cvt(val[e]);
+            const X v = sc * (X)cvt(ldnt(val + e));
             if (j >= ns && j < ns + len) {
                 if (j != i) Brow[j - ns] += v;
             } else if (uo) {
@@ -791,7 +795,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restri
     const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
     X acc = 0;
     if (row < n)
-        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * x[(size_t)ja[k] * xs];
+        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(ldnt(val + k)) * x[(size_t)ldnt(ja + k) * xs];
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -810,7 +814,7 @@ __global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restr
     const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
     X acc = 0;
     if (row < n && uo)
-        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * uo[ja[k]];
+        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(ldnt(val + k)) * uo[ldnt(ja + k)];
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -849,7 +853,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cp
         X acc = 0;
         for (int k = ia[i] + 1 + gl; k < ia[i + 1]; k += G) {
             const int j = ja[k];
-            const X v = sc * (X)cvt(val[k]);
+            const X v = sc * (X)cvt(ldnt(val + k));
             if (j >= ns && j < ns + len) {
                 if (j != i) atomicAdd(&Brow[j - ns], v);
             } else if (uo) {
