@@ -71,8 +71,9 @@ __device__ __forceinline__ double cvt(double v) { return v; }
 __device__ __forceinline__ float cvt(float v) { return v; }
 __device__ __forceinline__ float cvt(__half v) { return __half2float(v); }
 template <typename V> struct row_scaled { static constexpr bool value = false; };
-// matrix entries and column indices are streamed once per pass: non-temporal
-// loads keep them from displacing reused vector entries in the caches
+// non-temporal load for data streamed once per launch (the fine BSR matrix;
+// the AMG level matrices are re-read several times per V-cycle and gain from
+// staying cached, so their kernels use ordinary loads)
 template <typename T> __device__ __forceinline__ T ldnt(const T *p) { return __builtin_nontemporal_load(p); }
 template <> __device__ __forceinline__ __half ldnt<__half>(const __half *p) { return *p; }
 template <> struct row_scaled<__half> { static constexpr bool value = true; };
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
     X acc = 0;
     for (int k = sptr[s]; k < sptr[s + 1]; k++) {
         const size_t e = (size_t)k * WAVE + lane;
-        acc += (X)cvt(ldnt(val + e)) * x[(size_t)ldnt(col + e) * xs];
+        acc += (X)cvt(val[e]) * x[(size_t)col[e] * xs];
     }
     if (row >= n) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict_
     if (uo) {
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const size_t e = (size_t)k * WAVE + lane;
-            acc += (X)cvt(ldnt(val + e)) * uo[ldnt(col + e)];
+            acc += (X)cvt(val[e]) * uo[col[e]];
         }
     }
     if (row >= n) return;
@@ -267,7 +268,7 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
                 const size_t e = (size_t)k * WAVE + lane;
                 const int j = col[e];
                 const bool dep = dir > 0 ? (j < i) : (j > i);
-                if (!dep && j != i && uo) res -= sc * (X)cvt(ldnt(val + e)) * uo[j];
+                if (!dep && j != i && uo) res -= sc * (X)cvt(val[e]) * uo[j];
             }
         }
         wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(256) k_gs_sweep(int nsl, const int *__restrict
                 const size_t e = (size_t)k * WAVE + lane;
                 const int j = col[e];
                 const bool dep = dir > 0 ? (j < i) : (j > i);
-                if (dep) res -= sc * (X)cvt(ldnt(val + e)) * ld_coherent(un + j);
+                if (dep) res -= sc * (X)cvt(val[e]) * ld_coherent(un + j);
             }
             st_coherent(un + i, d != (X)0 ? res / d : uold_i);
         }
@@ -650,7 +651,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
             X xj;
             if (j >= ns && j < i) xj = un[j];
             else xj = uo ? uo[j] : (X)0;
-            res -= sc * (X)cvt(ldnt(val + e)) * xj;
+            res -= sc * (X)cvt(val[e]) * xj;
         }
         un[i] = d != (X)0 ? res / d : (uo ? uo[i] : (X)0);
     }
@@ -671,7 +672,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
             X xj;
             if (j >= ns && j < ne) xj = un[j];
             else xj = uo ? uo[j] : (X)0;
-            res -= sc * (X)cvt(ldnt(val + e)) * xj;
+            res -= sc * (X)cvt(val[e]) * xj;
         }
         un[i] = res / d;
     }
@@ -724,7 +725,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         for (int k = sptr[s]; k < sptr[s + 1]; k++) {
             const size_t e = (size_t)k * WAVE + lane;
             const int j = col[e];
-            const X v = sc * (X)cvt(ldnt(val + e));
+            const X v = sc * (X)cvt(val[e]);
             if (j >= ns && j < ns + len) {
                 if (j != i) Brow[j - ns] += v;
             } else if (uo) {
@@ -795,7 +796,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restri
     const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
     X acc = 0;
     if (row < n)
-        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(ldnt(val + k)) * x[(size_t)ldnt(ja + k) * xs];
+        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * x[(size_t)ja[k] * xs];
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -814,7 +815,7 @@ __global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restr
     const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
     X acc = 0;
     if (row < n && uo)
-        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(ldnt(val + k)) * uo[ldnt(ja + k)];
+        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * uo[ja[k]];
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -853,7 +854,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cp
         X acc = 0;
         for (int k = ia[i] + 1 + gl; k < ia[i + 1]; k += G) {
             const int j = ja[k];
-            const X v = sc * (X)cvt(ldnt(val + k));
+            const X v = sc * (X)cvt(val[k]);
             if (j >= ns && j < ns + len) {
                 if (j != i) atomicAdd(&Brow[j - ns], v);
             } else if (uo) {
