@@ -52,12 +52,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch from the committed rocprof counter summary, if any."""
+def read_traffic(key):
+    """HBM bytes per launch (kernel class or kernel) from the committed rocprof
+    counter summary, if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel)
+            return json.load(f).get(key)
     except (OSError, ValueError):
         return None
 
@@ -192,10 +193,11 @@ def main():
     n_l, ms_l, by_l = prof[dom]
     per_launch_ms = ms_l / n_l
     achieved = (by_l / n_l) / (per_launch_ms * 1e-3) / 1e9
-    kernel_names = {"gs_sweep": "k_gs_chunk_lds" if chunk else "k_gs_sweep", "bhgs_sweep": "k_bhgs_multi", "bsr_spmv": "k_bsr_spmv",
-                    "csr_spmv": "k_csr_spmv", "coarse_solve": "k_dense_gemv", "mgs": "k_dot_mgs",
-                    "vector_ops": "k_scale/k_axpy/k_lincomb", "bilu_solve": "k_btri"}
-    traffic = read_traffic(kernel_names[dom])
+    kernel_names = {"gs_sweep": "k_gs_chunk_lds/k_gs_chunk_grp (relax-6 chunks, all levels)" if chunk else "k_gs_sweep",
+                    "bhgs_sweep": "k_bhgs_part" if s2chunk else "k_bhgs_multi", "bsr_spmv": "k_bsr_spmv",
+                    "csr_spmv": "k_csr_spmv/k_csr_spmv_grp (all levels)", "coarse_solve": "k_dense_gemv",
+                    "mgs": "k_dot_mgs", "vector_ops": "k_scale/k_axpy/k_lincomb", "bilu_solve": "k_btri"}
+    traffic = read_traffic(dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "kernel": kernel_names[dom], "launches_per_solve": n_l,
@@ -212,7 +214,7 @@ def main():
     spmv_bytes = Ad.nnzb * (8 * Ad.b * Ad.b + 4) + 4 * (Ad.nb + 1) + 16 * Ad.n
     spmv = {"kernel": "k_bsr_spmv<3>", "mean_launch_us": round(ms * 1e3, 2), "algorithmic_bytes": spmv_bytes,
             "achieved_GBs": round(spmv_bytes / (ms * 1e-3) / 1e9, 1),
-            "frac": round(spmv_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": read_traffic("k_bsr_spmv")}
+            "frac": round(spmv_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": read_traffic("k_bsr_spmv<3>")}
     log(f"BSR SpMV {spmv}")
     assert s.error_flag() == 0
 
