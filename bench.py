@@ -197,7 +197,8 @@ def main():
                     "bhgs_sweep": "k_bhgs_part" if s2chunk else "k_bhgs_multi", "bsr_spmv": "k_bsr_spmv",
                     "csr_spmv": "k_csr_spmv/k_csr_spmv_grp (all levels)", "coarse_solve": "k_dense_gemv",
                     "mgs": "k_dot_mgs", "vector_ops": "k_scale/k_axpy/k_lincomb", "bilu_solve": "k_btri"}
-    traffic = read_traffic(dom)
+    traffic_key = {"bhgs_sweep": "k_bhgs_part<3, 4, double>" if s2chunk else "k_bhgs_multi<3>"}.get(dom, dom)
+    traffic = read_traffic(traffic_key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "kernel": kernel_names[dom], "launches_per_solve": n_l,
