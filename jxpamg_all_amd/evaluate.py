@@ -1,0 +1,129 @@
+"""Evaluation of the mixed-precision and learned configurations on one GPU.
+
+    python -m jxpamg_all_amd.evaluate C4      # 100^3, 4x4 blocks, FP16 stage 1, rtol 1e-8
+    python -m jxpamg_all_amd.evaluate C5      # 120x440x85, FP32 stage 1, learned steps, 5 unseen seeds
+    python -m jxpamg_all_amd.evaluate C3      # the 16x16x8 system with an FP32 hierarchy
+
+Each configuration is solved with the plain FP64 GPU path and with the
+mixed / learned variant; printed (one JSON object per configuration): time
+to rtol, iterations, the FP64 true residual, and the error of the solution
+against the FP64 solve.  Host setup (generation, decoupling, AMG) is timed
+separately.
+"""
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def log(msg):
+    print(f"[evaluate] {msg}", file=sys.stderr, flush=True)
+
+
+def _setup(cfg):
+    from . import amg_setup, decouple_abf, extract_pressure, generate
+    t0 = time.time()
+    A, rhs = generate(**cfg)
+    Ad, bd = decouple_abf(A, rhs)
+    H = amg_setup(*extract_pressure(Ad))
+    return Ad, bd, H, time.time() - t0
+
+
+def _solve(Ad, bd, H, params, reps=2, mlp=None):
+    import torch
+
+    from . import Solver
+    s = Solver(Ad, H, params)
+    used = None
+    if mlp is not None:
+        used, f_l, f_d = s.apply_mlp(mlp.P.data_ptr())
+        used = {"levels": used, "factor_learned": round(f_l, 4), "factor_default": round(f_d, 4)}
+    b = torch.tensor(bd, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(b)
+    hist = np.zeros(params.max_iter + 1)
+    s.solve_device(b.data_ptr(), x.data_ptr(), hist)
+    ts = []
+    for _ in range(reps):
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = s.solve_device(b.data_ptr(), x.data_ptr(), hist)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    s.close()
+    out = {"s": round(float(np.mean(ts)), 4), "iterations": st.iterations, "converged": st.converged,
+           "true_rel_res": st.true_rel_res}
+    if used is not None:
+        out["learned"] = used
+    return out, x.cpu().numpy()
+
+
+def _rel(x, ref):
+    return float(np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def evaluate_c3():
+    from . import CONFIGS, default_params
+    Ad, bd, H, t = _setup(CONFIGS["C1"])
+    p64 = default_params(tol=1e-6, relax_chunk=4)
+    r64, x64 = _solve(Ad, bd, H, p64)
+    r32, x32 = _solve(Ad, bd, H, default_params(tol=1e-6, relax_chunk=4, precond_precision=1))
+    r32["error_vs_fp64"] = _rel(x32, x64)
+    return {"config": "C3 16x16x8 3x3, FP32 hierarchy and stage-1 vectors", "setup_s": round(t, 2), "fp64": r64,
+            "fp32_stage1": r32, "iteration_delta": r32["iterations"] - r64["iterations"]}
+
+
+def evaluate_c4():
+    from . import CONFIGS, default_params
+    log("C4: generating 100^3 4x4 system")
+    Ad, bd, H, t = _setup(CONFIGS["C4"])
+    log(f"C4: setup {t:.1f}s, levels {[len(m[0]) - 1 for m in H.A]}")
+    common = dict(tol=1e-8, max_iter=200, relax_chunk=4, stage2_chunk=4)
+    r64, x64 = _solve(Ad, bd, H, default_params(**common))
+    log(f"C4 fp64: {r64}")
+    r16, x16 = _solve(Ad, bd, H, default_params(precond_precision=2, **common))
+    r16["error_vs_fp64"] = _rel(x16, x64)
+    log(f"C4 fp16: {r16}")
+    return {"config": "C4 100^3 4x4, FP16 stage 1 (power-of-two row scales, FP32 accumulation), rtol 1e-8",
+            "setup_s": round(t, 2), "fp64": r64, "fp16_stage1": r16,
+            "iteration_delta": r16["iterations"] - r64["iterations"], "speedup": round(r64["s"] / r16["s"], 3)}
+
+
+def evaluate_c5(seeds=(2001, 2002, 2003, 2004, 2005), train_seeds=20):
+    from . import CONFIGS, default_params
+    from . import learned as lrn
+    t0 = time.time()
+    Xtr, Ttr, info = lrn.make_dataset(range(1001, 1001 + train_seeds), precision=1)
+    net = lrn.MLP(seed=0)
+    losses = net.fit(Xtr, Ttr, steps=3000, lr=3e-3)
+    log(f"C5: trained on {len(Xtr)} levels of {train_seeds} systems, loss {losses[0]:.4f} -> {losses[-1]:.5f}")
+    out = {"config": "C5 120x440x85 3x3, FP32 stage 1, degree-4 polynomial smoother, 5 unseen seeds",
+           "training": {"samples": len(Xtr), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                        "seconds": round(time.time() - t0, 2)}, "seeds": []}
+    s2 = dict(stage2_chunk=4)
+    for seed in seeds:
+        log(f"C5 seed {seed}: setup")
+        Ad, bd, H, t = _setup(dict(CONFIGS["C5"], seed=seed))
+        r64, x64 = _solve(Ad, bd, H, default_params(tol=1e-6, relax_chunk=4, **s2), reps=1)
+        pm = default_params(tol=1e-6, precond_precision=1, poly_degree=4, **s2)
+        rdef, xdef = _solve(Ad, bd, H, pm, reps=1)
+        rlrn, xlrn = _solve(Ad, bd, H, pm, reps=1, mlp=net)
+        for r, x in ((rdef, xdef), (rlrn, xlrn)):
+            r["error_vs_fp64"] = _rel(x, x64)
+        rec = {"seed": seed, "setup_s": round(t, 2), "fp64_relax6": r64, "fp32_default_poly": rdef,
+               "fp32_learned_poly": rlrn}
+        log(f"C5 seed {seed}: {json.dumps(rec)}")
+        out["seeds"].append(rec)
+    return out
+
+
+def main(argv):
+    which = argv[1:] or ["C3"]
+    for w in which:
+        fn = {"C3": evaluate_c3, "C4": evaluate_c4, "C5": evaluate_c5}[w]
+        print(json.dumps(fn()), flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv)
