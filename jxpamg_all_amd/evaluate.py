@@ -119,6 +119,9 @@ def evaluate_c5(seeds=(2001, 2002, 2003, 2004, 2005), train_seeds=20):
 
 
 def main(argv):
+    import torch  # initialise the device through torch before the library touches it
+    assert torch.cuda.is_available(), "evaluate needs a GPU"
+    torch.cuda.set_device(0)
     which = argv[1:] or ["C3"]
     for w in which:
         fn = {"C3": evaluate_c3, "C4": evaluate_c4, "C5": evaluate_c5}[w]
