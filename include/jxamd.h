@@ -75,6 +75,12 @@ int jxamd_rhs_read_text(const char *path, int n, double *v);
 /* ABF decoupling in place, jxf_BSRMatrixDecouple(..., JXF_DECOUP_ABF, 0)
  * (jxfpamg-0220/csrc/cpr/jxf_bsr_decoup.c:1273-1348); rhs may be NULL */
 int jxamd_bsr_decouple_abf(int nb, int b, const int *ia, const int *ja, double *a, double *rhs);
+/* the same decoupling with true block inverses and an out-of-place rhs
+ * product: D^{-1} A, D^{-1} b.  No reference counterpart; it differs from
+ * jxamd_bsr_decouple_abf for b >= 4 (the reference's 4x4 closed form,
+ * jxf_bsr_decoup.c:112-183, and its Gauss-Jordan above 4x4, :200-266, do not
+ * return the inverse) and in the rhs for b > 1 */
+int jxamd_bsr_decouple_abf_exact(int nb, int b, const int *ia, const int *ja, double *a, double *rhs);
 /* pressure matrix A_p[i,j] = A_ij[p][p] on A's pattern, jxf_BSRMatrixGetSubmatrix
  * (jxfpamg-0220/csrc/mat_vec_data/seq_bsr_matrix.c:214-250); p_ia[nb+1], p_ja/p_a[nnzb] */
 int jxamd_bsr_extract_pressure(int nb, int b, const int *ia, const int *ja, const double *a, int pidx, int *p_ia,

@@ -87,6 +87,7 @@ def _declare(L):
         "jxamd_gen_sizes": [vp, vp, vp],
         "jxamd_gen_reservoir": [vp, vp, vp, vp, vp],
         "jxamd_bsr_decouple_abf": [i32, i32, vp, vp, vp, vp],
+        "jxamd_bsr_decouple_abf_exact": [i32, i32, vp, vp, vp, vp],
         "jxamd_bsr_extract_pressure": [i32, i32, vp, vp, vp, i32, vp, vp, vp],
         "jxamd_bsr_write_binary": [ctypes.c_char_p, i32, i32, i32, i32, vp, vp, vp],
         "jxamd_bsr_read_binary_sizes": [ctypes.c_char_p, vp, vp, vp, vp],
@@ -241,12 +242,14 @@ def bilu0_factor(A):
     return lu, dinv
 
 
-def decouple_abf(A, rhs=None):
-    """ABF decoupling (copy); returns (A_dec, rhs_dec)."""
+def decouple_abf(A, rhs=None, exact=False):
+    """ABF decoupling (copy); returns (A_dec, rhs_dec).  exact=False is the
+    reference's arithmetic bit for bit; exact=True uses true block inverses
+    and an out-of-place rhs product (the reference's 4x4 inverse is wrong)."""
     a = A.a.copy()
     r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).copy()
-    check(lib().jxamd_bsr_decouple_abf(A.nb, A.b, _p(A.ia), _p(A.ja), _p(a), None if r is None else _p(r)),
-          "jxamd_bsr_decouple_abf")
+    fn = "jxamd_bsr_decouple_abf_exact" if exact else "jxamd_bsr_decouple_abf"
+    check(getattr(lib(), fn)(A.nb, A.b, _p(A.ia), _p(A.ja), _p(a), None if r is None else _p(r)), fn)
     return BSRMatrix(A.nb, A.b, A.ia, A.ja, a), r
 
 

@@ -12,7 +12,25 @@
  * is kept on purpose because it defines the system that is solved: the
  * right-hand side is multiplied in place (jxf_smat_vec_mul(binv, svec, svec),
  * jxf_bsr_decoup.c:335-343 and :346-357), so components already overwritten
- * feed the later ones and the result is not D^{-1} b.
+ * feed the later ones and the result is not D^{-1} b.  A second one is kept
+ * for the same reason: the closed-form 4x4 inverse (jxf_smat_inv_4x4,
+ * jxf_bsr_decoup.c:112-183) is not the inverse.  Its diagonal cofactors are
+ * right, but off the diagonal it returns (-1)^(i+j) adj(A)_ji in place of
+ * adj(A)_ij, and its determinant (line 127) pairs a12, a13, a14 with minors
+ * that themselves contain the first row.  The decoupled 4-unknown blocks are
+ * therefore not D^{-1} A_ij, the pressure rows lose the diagonal dominance
+ * the AMG relies on, and the reference's CPR-GMRES stalls on such systems.
+ *
+ * The same holds above 4x4: the reference's Gauss-Jordan (jxf_bsr_decoup.c:
+ * 200-266) row-reduces the block in place without an identity beside it, so
+ * it returns a column permutation of the identity, not the inverse; the
+ * restatement below follows it.
+ *
+ * jxamd_bsr_decouple_abf_exact is the documented deviation for b >= 4:
+ * a true inverse (Gauss-Jordan on [D | I]) for every block size above 3
+ * (1x1 to 3x3 keep the reference's exact closed forms), and an out-of-place
+ * right-hand-side product, so the decoupled system is D^{-1} A x = D^{-1} b
+ * with the solution of A x = b.
  *
  * Pressure extraction (jxf_BSRMatrixGetSubmatrix, jxfpamg-0220/csrc/
  * mat_vec_data/seq_bsr_matrix.c:214-250): A_p keeps A's block pattern with
@@ -131,9 +149,54 @@ static void inv_gj(double *a, int n)
     }
 }
 
+/* true inverse of an n x n block (n <= 10): Gauss-Jordan on [A | I] with
+ * partial pivoting; a singular block gives the identity, as the reference's
+ * routines do */
+static void inv_exact(double *a, int n)
+{
+    double m[10][20];
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < 2 * n; c++) m[r][c] = c < n ? a[r * n + c] : (c - n == r ? 1.0 : 0.0);
+    for (int i = 0; i < n; i++) {
+        int p = i;
+        for (int r = i + 1; r < n; r++)
+            if (fabs(m[r][i]) > fabs(m[p][i])) p = r;
+        if (fabs(m[p][i]) < 1e-15) {
+            for (int r = 0; r < n; r++)
+                for (int c = 0; c < n; c++) a[r * n + c] = r == c ? 1.0 : 0.0;
+            return;
+        }
+        if (p != i)
+            for (int c = 0; c < 2 * n; c++) { double t = m[i][c]; m[i][c] = m[p][c]; m[p][c] = t; }
+        const double piv = m[i][i];
+        for (int c = 0; c < 2 * n; c++) m[i][c] /= piv;
+        for (int r = 0; r < n; r++) {
+            if (r == i) continue;
+            const double f = m[r][i];
+            if (f != 0.0)
+                for (int c = 0; c < 2 * n; c++) m[r][c] -= f * m[i][c];
+        }
+    }
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < n; c++) a[r * n + c] = m[r][n + c];
+}
+
+static int decouple(int nb, int b, const int *ia, double *a, double *rhs, int exact);
+
 int jxamd_bsr_decouple_abf(int nb, int b, const int *ia, const int *ja, double *a, double *rhs)
 {
     (void)ja;
+    return decouple(nb, b, ia, a, rhs, 0);
+}
+
+int jxamd_bsr_decouple_abf_exact(int nb, int b, const int *ia, const int *ja, double *a, double *rhs)
+{
+    (void)ja;
+    return decouple(nb, b, ia, a, rhs, 1);
+}
+
+static int decouple(int nb, int b, const int *ia, double *a, double *rhs, int exact)
+{
     if (nb < 0 || b < 1 || b > 10 || !ia || !a) return JXAMD_ERROR_ARG;
     const int bb = b * b;
     double binv[100], smat[100];
@@ -144,6 +207,7 @@ int jxamd_bsr_decouple_abf(int nb, int b, const int *ia, const int *ja, double *
         if (b == 1) binv[0] = fabs(binv[0]) > 1e-15 ? 1.0 / binv[0] : 1.0;
         else if (b == 2) inv2(binv);
         else if (b == 3) inv3(binv);
+        else if (exact) inv_exact(binv, b); /* b >= 4 */
         else if (b == 4) inv4(binv);
         else inv_gj(binv, b);
         for (int k = ia[i]; k < ia[i + 1]; k++) {
@@ -173,7 +237,14 @@ int jxamd_bsr_decouple_abf(int nb, int b, const int *ia, const int *ja, double *
             }
             memcpy(blk, smat, sizeof(double) * bb);
         }
-        if (rhs) {
+        if (rhs && exact) {
+            double *v = rhs + (size_t)i * b, w[10];
+            for (int r = 0; r < b; r++) {
+                w[r] = 0.0;
+                for (int q = 0; q < b; q++) w[r] += binv[r * b + q] * v[q];
+            }
+            memcpy(v, w, sizeof(double) * b);
+        } else if (rhs) {
             /* in-place product with aliased input and output, as in the reference */
             double *v = rhs + (size_t)i * b;
             if (b == 2) {

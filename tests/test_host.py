@@ -76,6 +76,76 @@ def test_abf_rhs_keeps_reference_in_place_product():
     assert not np.allclose(rd, Bi @ r)
 
 
+def _inv4_reference_form(D):
+    """The closed form of jxf_smat_inv_4x4 (jxf_bsr_decoup.c:112-183) written
+    as what it computes: diagonal cofactors, off-diagonal (i, j) entry
+    (-1)^(i+j) adj(D)_ji, over its own determinant expansion."""
+    adj = np.linalg.inv(D) * np.linalg.det(D)
+    sgn = np.array([[(-1) ** (i + j) for j in range(4)] for i in range(4)], float)
+    out = sgn * adj.T
+    np.fill_diagonal(out, np.diag(adj))
+    # line 127: a11*M11 - a12*M12 + a13*M13 - a14*M14 with M1k as the code forms them
+    m = [np.linalg.det(np.delete(np.delete(D, k, 0), 0, 1)) for k in range(4)]
+    det = D[0, 0] * m[0] + D[0, 1] * m[1] + D[0, 2] * m[2] + D[0, 3] * m[3]
+    return out / det
+
+
+def test_abf_4x4_keeps_reference_inverse_and_exact_variant_is_exact():
+    """b = 4: the bit-exact decoupling reproduces the reference's closed-form
+    4x4 "inverse" (which is not the inverse; the b4 fixture pins it), and the
+    exact variant gives D^-1 A_ij and D^-1 b, so its solution is A's."""
+    rng = np.random.default_rng(4)
+    D = rng.standard_normal((4, 4)) + 5 * np.eye(4)
+    C = rng.standard_normal((4, 4))
+    ia, ja = np.array([0, 2, 4], np.int32), np.array([0, 1, 1, 0], np.int32)
+    D2 = rng.standard_normal((4, 4)) + 6 * np.eye(4)
+    C2 = rng.standard_normal((4, 4))
+    A = jx.BSRMatrix(2, 4, ia, ja, np.concatenate([D.ravel(), C.ravel(), D2.ravel(), C2.ravel()]))
+    r = rng.standard_normal(8)
+    Ad, _ = jx.decouple_abf(A, r)
+    Wrong = _inv4_reference_form(D)
+    assert np.allclose(Ad.a[16:32].reshape(4, 4), Wrong @ C, rtol=1e-10, atol=1e-12)
+    assert not np.allclose(Wrong, np.linalg.inv(D), atol=1e-3)
+    Ae, re_ = jx.decouple_abf(A, r, exact=True)
+    assert np.allclose(Ae.a[16:32].reshape(4, 4), np.linalg.solve(D, C), rtol=1e-12, atol=1e-14)
+    assert np.allclose(Ae.a[48:64].reshape(4, 4), np.linalg.solve(D2, C2), rtol=1e-12, atol=1e-14)
+    assert np.allclose(re_, np.concatenate([np.linalg.solve(D, r[:4]), np.linalg.solve(D2, r[4:])]), rtol=1e-12)
+    x = np.linalg.solve(A.to_scipy().toarray(), r)
+    assert np.allclose(Ae.to_scipy() @ x, re_, rtol=1e-10, atol=1e-12)
+
+
+def test_abf_5x5_reference_gauss_jordan_is_a_permutation():
+    """Above 4x4 the reference's Gauss-Jordan (jxf_bsr_decoup.c:200-266)
+    row-reduces in place with no identity beside it: the "inverse" it returns
+    is a permutation matrix, so the off-diagonal blocks are only permuted.
+    The exact variant gives D^-1 C."""
+    rng = np.random.default_rng(5)
+    D = rng.standard_normal((5, 5)) + 2 * np.eye(5)
+    C = rng.standard_normal((5, 5))
+    A = jx.BSRMatrix(1, 5, np.array([0, 2], np.int32), np.array([0, 0], np.int32),
+                     np.concatenate([D.ravel(), C.ravel()]))
+    Ad, _ = jx.decouple_abf(A)
+    B = Ad.a[25:].reshape(5, 5)
+    P = np.linalg.solve(C.T, B.T).T  # B = P C
+    assert np.allclose(P, np.round(P), atol=1e-9)
+    Pr = np.round(P)
+    assert np.array_equal(np.sort(Pr.sum(0)), np.ones(5)) and np.array_equal(np.sort(Pr.sum(1)), np.ones(5))
+    Ae, _ = jx.decouple_abf(A, exact=True)
+    assert np.allclose(Ae.a[25:].reshape(5, 5), np.linalg.solve(D, C), rtol=1e-12, atol=1e-13)
+
+
+def test_abf_exact_matches_reference_matrix_for_3x3():
+    """b = 3: the reference's closed-form inverse is exact, so the matrix part
+    of both variants is identical bit for bit; only the rhs differs."""
+    d, A, rhs, Ad, bd, H, meta = system("c1")
+    Ae, be = jx.decouple_abf(A, rhs, exact=True)
+    assert np.array_equal(Ae.a, Ad.a)
+    x = np.random.default_rng(0).standard_normal(A.n)
+    b = A.to_scipy() @ x
+    _, bx = jx.decouple_abf(A, b, exact=True)
+    assert np.allclose(Ae.to_scipy() @ x, bx, rtol=1e-10, atol=1e-10 * np.abs(bx).max())
+
+
 def test_pressure_extraction_matches_reference():
     d, A, rhs, Ad, bd, H, meta = system("c1")
     ia, ja, a = jx.extract_pressure(Ad, 0)
