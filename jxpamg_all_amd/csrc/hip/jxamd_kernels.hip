@@ -21,6 +21,13 @@
 
 #define WAVE 64
 #define SPIN_LIMIT (1 << 24)
+// entries whose loads one lane issues before it consumes any of them in the
+// level kernels (SELL slots, row-group strides): the gathers behind a column
+// index are a second dependent load, and a lane with one chain in flight
+// leaves HBM latency-bound
+#define SELL_U 4
+#define GRP_U 8
+#define DOT_U 4
 
 extern "C" {
 #include "jxamd_dev.h"
@@ -155,7 +162,24 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
     if (s >= nsl) return;
     const int row = s * WAVE + lane;
     X acc = 0;
-    for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+    const int k1 = sptr[s + 1];
+    int k = sptr[s];
+    for (; k + SELL_U <= k1; k += SELL_U) {
+        int j[SELL_U];
+        V v[SELL_U];
+#pragma unroll
+        for (int u = 0; u < SELL_U; u++) {
+            const size_t e = (size_t)(k + u) * WAVE + lane;
+            j[u] = col[e];
+            v[u] = val[e];
+        }
+        X xv[SELL_U];
+#pragma unroll
+        for (int u = 0; u < SELL_U; u++) xv[u] = x[(size_t)j[u] * xs];
+#pragma unroll
+        for (int u = 0; u < SELL_U; u++) acc += (X)cvt(v[u]) * xv[u];
+    }
+    for (; k < k1; k++) {
         const size_t e = (size_t)k * WAVE + lane;
         acc += (X)cvt(val[e]) * x[(size_t)col[e] * xs];
     }
@@ -182,7 +206,24 @@ __global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict_
     const int row = s * WAVE + lane;
     X acc = 0;
     if (uo) {
-        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+        const int k1 = sptr[s + 1];
+        int k = sptr[s];
+        for (; k + SELL_U <= k1; k += SELL_U) {
+            int j[SELL_U];
+            V v[SELL_U];
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++) {
+                const size_t e = (size_t)(k + u) * WAVE + lane;
+                j[u] = col[e];
+                v[u] = val[e];
+            }
+            X xv[SELL_U];
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++) xv[u] = uo[j[u]];
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++) acc += (X)cvt(v[u]) * xv[u];
+        }
+        for (; k < k1; k++) {
             const size_t e = (size_t)k * WAVE + lane;
             acc += (X)cvt(val[e]) * uo[col[e]];
         }
@@ -722,7 +763,34 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         X acc = f[(size_t)i * fs];
         const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
         const int s = i / WAVE, lane = i % WAVE;
-        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
+        const int k1 = sptr[s + 1];
+        int k = sptr[s];
+        for (; k + SELL_U <= k1; k += SELL_U) {
+            int j[SELL_U];
+            V v[SELL_U];
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++) {
+                const size_t e = (size_t)(k + u) * WAVE + lane;
+                j[u] = col[e];
+                v[u] = val[e];
+            }
+            X xv[SELL_U];
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++) {
+                const bool in = j[u] >= ns && j[u] < ns + len;
+                xv[u] = (!in && uo) ? uo[j[u]] : (X)0;
+            }
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++) {
+                const X vv = sc * (X)cvt(v[u]);
+                if (j[u] >= ns && j[u] < ns + len) {
+                    if (j[u] != i) Brow[j[u] - ns] += vv;
+                } else if (uo) {
+                    acc -= vv * xv[u];
+                }
+            }
+        }
+        for (; k < k1; k++) {
             const size_t e = (size_t)k * WAVE + lane;
             const int j = col[e];
             const X v = sc * (X)cvt(val[e]);
@@ -795,8 +863,25 @@ __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restri
     const int gl = threadIdx.x % G;
     const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
     X acc = 0;
-    if (row < n)
-        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * x[(size_t)ja[k] * xs];
+    if (row < n) {
+        const int ke = ia[row + 1];
+        for (int k = ia[row] + gl; k < ke; k += GRP_U * G) {
+            int j[GRP_U];
+            V v[GRP_U];
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++) {
+                const int kk = k + u * G;
+                j[u] = kk < ke ? ja[kk] : -1;
+                v[u] = kk < ke ? val[kk] : (V)0;
+            }
+            X xv[GRP_U];
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++) xv[u] = j[u] >= 0 ? x[(size_t)j[u] * xs] : (X)0;
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++)
+                if (j[u] >= 0) acc += (X)cvt(v[u]) * xv[u];
+        }
+    }
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -814,8 +899,25 @@ __global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restr
     const int gl = threadIdx.x % G;
     const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
     X acc = 0;
-    if (row < n && uo)
-        for (int k = ia[row] + gl; k < ia[row + 1]; k += G) acc += (X)cvt(val[k]) * uo[ja[k]];
+    if (row < n && uo) {
+        const int ke = ia[row + 1];
+        for (int k = ia[row] + gl; k < ke; k += GRP_U * G) {
+            int j[GRP_U];
+            V v[GRP_U];
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++) {
+                const int kk = k + u * G;
+                j[u] = kk < ke ? ja[kk] : -1;
+                v[u] = kk < ke ? val[kk] : (V)0;
+            }
+            X xv[GRP_U];
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++) xv[u] = j[u] >= 0 ? uo[j[u]] : (X)0;
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++)
+                if (j[u] >= 0) acc += (X)cvt(v[u]) * xv[u];
+        }
+    }
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -852,13 +954,31 @@ __global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cp
         X *Brow = B + (size_t)(c - c0) * BST + (i - ns) * LMAX;
         const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
         X acc = 0;
-        for (int k = ia[i] + 1 + gl; k < ia[i + 1]; k += G) {
-            const int j = ja[k];
-            const X v = sc * (X)cvt(val[k]);
-            if (j >= ns && j < ns + len) {
-                if (j != i) atomicAdd(&Brow[j - ns], v);
-            } else if (uo) {
-                acc -= v * uo[j];
+        const int ke = ia[i + 1];
+        for (int k = ia[i] + 1 + gl; k < ke; k += GRP_U * G) {
+            int j[GRP_U];
+            V v[GRP_U];
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++) {
+                const int kk = k + u * G;
+                j[u] = kk < ke ? ja[kk] : -1;
+                v[u] = kk < ke ? val[kk] : (V)0;
+            }
+            X xv[GRP_U];
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++) {
+                const bool out = j[u] >= 0 && !(j[u] >= ns && j[u] < ns + len);
+                xv[u] = (out && uo) ? uo[j[u]] : (X)0;
+            }
+#pragma unroll
+            for (int u = 0; u < GRP_U; u++) {
+                if (j[u] < 0) continue;
+                const X vv = sc * (X)cvt(v[u]);
+                if (j[u] >= ns && j[u] < ns + len) {
+                    if (j[u] != i) atomicAdd(&Brow[j[u] - ns], vv);
+                } else if (uo) {
+                    acc -= vv * xv[u];
+                }
             }
         }
         acc = group_sum<G>(acc);
@@ -1001,13 +1121,44 @@ __global__ void __launch_bounds__(256) k_dot_mgs(size_t n, const double *u, doub
     double acc = 0.0;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
+    // u may be y (a squared norm): then the product uses the updated y.
+    // Entries i, i + stride, ... touch disjoint elements, so each thread
+    // loads DOT_U of them before it stores any (the compiler cannot know
+    // that u and y only meet at equal indices).
+    const bool self = (u == y);
     if (v_prev) {
+        for (; i + (DOT_U - 1) * stride < n; i += DOT_U * stride) {
+            double yv[DOT_U], vv[DOT_U], uv[DOT_U];
+#pragma unroll
+            for (int q = 0; q < DOT_U; q++) {
+                yv[q] = y[i + q * stride];
+                vv[q] = v_prev[i + q * stride];
+                uv[q] = self ? 0.0 : u[i + q * stride];
+            }
+#pragma unroll
+            for (int q = 0; q < DOT_U; q++) {
+                const double yi = yv[q] - h * vv[q];
+                y[i + q * stride] = yi;
+                acc += (self ? yi : uv[q]) * yi;
+            }
+        }
         for (; i < n; i += stride) {
-            double yi = y[i] - h * v_prev[i];
+            const double yi = y[i] - h * v_prev[i];
+            const double ui = self ? yi : u[i];
             y[i] = yi;
-            acc += u[i] * yi;
+            acc += ui * yi;
         }
     } else {
+        for (; i + (DOT_U - 1) * stride < n; i += DOT_U * stride) {
+            double yv[DOT_U], uv[DOT_U];
+#pragma unroll
+            for (int q = 0; q < DOT_U; q++) {
+                yv[q] = y[i + q * stride];
+                uv[q] = u[i + q * stride];
+            }
+#pragma unroll
+            for (int q = 0; q < DOT_U; q++) acc += uv[q] * yv[q];
+        }
         for (; i < n; i += stride) acc += u[i] * y[i];
     }
 #pragma unroll

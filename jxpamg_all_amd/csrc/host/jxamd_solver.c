@@ -576,15 +576,15 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsc
 }
 
 /* lanes per row for a matrix of mean row length nnz/n: one (SELL-64) for
- * short rows, else a power of two near a third of the row length (at most 32) */
+ * short rows, else the smallest power of two from 4 to 32 that leaves each
+ * lane at most one batch of the kernels' 8 in-flight entries of a mean row */
 static int choose_group(long nnz, int n)
 {
     const double m = n > 0 ? (double)nnz / n : 0.0;
     if (m < 12.0) return 1;
-    if (m < 20.0) return 4;
-    if (m < 40.0) return 8;
-    if (m < 80.0) return 16;
-    return 32;
+    int G = 4;
+    while (G < 32 && G * 8 < m) G *= 2;
+    return G;
 }
 
 /* Upload a CSR matrix for the row-group kernels; F16 rows are scaled by the

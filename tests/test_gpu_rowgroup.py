@@ -46,7 +46,7 @@ def trivial_bsr(n):
                         np.tile(np.eye(2).ravel(), n))
 
 
-@pytest.mark.parametrize("rowlen", [16, 32, 64, 140])
+@pytest.mark.parametrize("rowlen", [16, 32, 64, 100, 140])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_rowgroup_smoothers_vs_oracle(rowlen, prec):
     n = 2000
