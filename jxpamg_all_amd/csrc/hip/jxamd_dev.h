@@ -70,11 +70,13 @@ int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, co
 int jxamd_k_gs_chunk(int vt, int xt, int n, int nchunks, const int *wstep, const int *sptr, const int *col,
                      const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
                      void *un, void *stream);
-/* row-group variants over plain CSR (G = 4, 8, 16 or 32 lanes per row) */
-int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+/* row-group variants (G = 4, 8, 16 or 32 lanes per row): SpMV and polynomial
+ * step over the SELL-G layout (64/G rows per slice; sptr: slot-row offsets
+ * per slice), the chunked smoother over plain CSR */
+int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *sptr, const int *col, const void *val,
                          const float *rsc, const void *x, int xs, const void *rhs, int rs, void *y, int mode,
                          void *stream);
-int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *sptr, const int *col, const void *val,
                           const float *rsc, const void *dinv, const void *f, int fs, const void *uo, void *un,
                           double omega, void *stream);
 int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *ia, const int *ja, const void *val,

@@ -25,7 +25,7 @@
 // level kernels (SELL slots, row-group strides): the gathers behind a column
 // index are a second dependent load, and a lane with one chain in flight
 // leaves HBM latency-bound
-#define SELL_U 4
+#define SELL_U 8
 #define GRP_U 8
 #define DOT_U 4
 
@@ -144,6 +144,113 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
     }
 }
 
+// Sum of val[e] * x[col[e] * xs] over slot rows [k, k1) of a SELL layout,
+// e = slot * 64 + lane, in slot order.  k1 is the same for the whole
+// wavefront, so each batch of SELL_U slots is predicated uniformly: the lane
+// issues all of a batch's index and value loads, then all of its gathers,
+// before it adds (a lane with one load chain in flight at a time leaves HBM
+// latency-bound).
+template <typename V, typename X>
+__device__ __forceinline__ X sell_dot(int k, int k1, int lane, const int *__restrict__ col,
+                                      const V *__restrict__ val, const X *__restrict__ x, int xs)
+{
+    X acc = 0;
+    for (; k < k1; k += SELL_U) {
+        int j[SELL_U];
+        V v[SELL_U];
+#pragma unroll
+        for (int u = 0; u < SELL_U; u++) {
+            if (k + u < k1) {
+                const size_t e = (size_t)(k + u) * WAVE + lane;
+                j[u] = col[e];
+                v[u] = val[e];
+            }
+        }
+        X xv[SELL_U];
+#pragma unroll
+        for (int u = 0; u < SELL_U; u++)
+            if (k + u < k1) xv[u] = x[(size_t)j[u] * xs];
+#pragma unroll
+        for (int u = 0; u < SELL_U; u++)
+            if (k + u < k1) acc += (X)cvt(v[u]) * xv[u];
+    }
+    return acc;
+}
+
+// Phase A of the chunked smoothers for one row i (chunk rows [ns, ns+len))
+// held by one lane of a SELL-64 layout: over slot rows [k, k1) at lane offset
+// lo, entries of in-chunk columns other than i go to the row's LDS block
+// Brow, and the others fold into acc -= a_ij uo_j.  Batches of U slots,
+// predicated per lane (a block's rows may lie in two slices).
+template <int U, typename V, typename X>
+__device__ __forceinline__ void chunk_row_gather(int k, int k1, size_t lo, int i, int ns, int len, X sc,
+                                                 const int *__restrict__ col, const V *__restrict__ val,
+                                                 const X *__restrict__ uo, X *Brow, X &acc)
+{
+    for (; k < k1; k += U) {
+        int j[U];
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (k + u < k1) {
+                const size_t e = (size_t)(k + u) * WAVE + lo;
+                j[u] = col[e];
+                v[u] = val[e];
+            }
+        }
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool out = k + u < k1 && !(j[u] >= ns && j[u] < ns + len);
+            xv[u] = (out && uo) ? uo[j[u]] : (X)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (k + u >= k1) continue;
+            const X vv = sc * (X)cvt(v[u]);
+            if (j[u] >= ns && j[u] < ns + len) {
+                if (j[u] != i) Brow[j[u] - ns] += vv;
+            } else if (uo) {
+                acc -= vv * xv[u];
+            }
+        }
+    }
+}
+
+// The same for one CSR row shared by a group of G lanes (LDS adds are
+// atomic): entries k, k+G, ... below ke, U of them in flight per batch.
+template <int U, int G, typename V, typename X>
+__device__ __forceinline__ void csr_row_gather(int k, int ke, int i, int ns, int len, X sc, const int *__restrict__ ja,
+                                               const V *__restrict__ val, const X *__restrict__ uo, X *Brow, X &acc)
+{
+    for (; k < ke; k += U * G) {
+        int j[U];
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kk = k + u * G;
+            j[u] = kk < ke ? ja[kk] : -1;
+            v[u] = kk < ke ? val[kk] : (V)0;
+        }
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool out = j[u] >= 0 && !(j[u] >= ns && j[u] < ns + len);
+            xv[u] = (out && uo) ? uo[j[u]] : (X)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (j[u] < 0) continue;
+            const X vv = sc * (X)cvt(v[u]);
+            if (j[u] >= ns && j[u] < ns + len) {
+                if (j[u] != i) atomicAdd(&Brow[j[u] - ns], vv);
+            } else if (uo) {
+                acc -= vv * xv[u];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Scalar CSR SpMV, SELL-64 (levels: A_l residual, R_l restriction, P_l
 // prolongation).  mode 0: y = A x; 1: y = rhs - A x; 2: y += A x.
@@ -161,28 +268,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
     const int nsl = (n + WAVE - 1) / WAVE;
     if (s >= nsl) return;
     const int row = s * WAVE + lane;
-    X acc = 0;
-    const int k1 = sptr[s + 1];
-    int k = sptr[s];
-    for (; k + SELL_U <= k1; k += SELL_U) {
-        int j[SELL_U];
-        V v[SELL_U];
-#pragma unroll
-        for (int u = 0; u < SELL_U; u++) {
-            const size_t e = (size_t)(k + u) * WAVE + lane;
-            j[u] = col[e];
-            v[u] = val[e];
-        }
-        X xv[SELL_U];
-#pragma unroll
-        for (int u = 0; u < SELL_U; u++) xv[u] = x[(size_t)j[u] * xs];
-#pragma unroll
-        for (int u = 0; u < SELL_U; u++) acc += (X)cvt(v[u]) * xv[u];
-    }
-    for (; k < k1; k++) {
-        const size_t e = (size_t)k * WAVE + lane;
-        acc += (X)cvt(val[e]) * x[(size_t)col[e] * xs];
-    }
+    X acc = sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, x, xs);
     if (row >= n) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
     if (mode == 0) y[row] = acc;
@@ -204,30 +290,7 @@ __global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict_
     const int nsl = (n + WAVE - 1) / WAVE;
     if (s >= nsl) return;
     const int row = s * WAVE + lane;
-    X acc = 0;
-    if (uo) {
-        const int k1 = sptr[s + 1];
-        int k = sptr[s];
-        for (; k + SELL_U <= k1; k += SELL_U) {
-            int j[SELL_U];
-            V v[SELL_U];
-#pragma unroll
-            for (int u = 0; u < SELL_U; u++) {
-                const size_t e = (size_t)(k + u) * WAVE + lane;
-                j[u] = col[e];
-                v[u] = val[e];
-            }
-            X xv[SELL_U];
-#pragma unroll
-            for (int u = 0; u < SELL_U; u++) xv[u] = uo[j[u]];
-#pragma unroll
-            for (int u = 0; u < SELL_U; u++) acc += (X)cvt(v[u]) * xv[u];
-        }
-        for (; k < k1; k++) {
-            const size_t e = (size_t)k * WAVE + lane;
-            acc += (X)cvt(val[e]) * uo[col[e]];
-        }
-    }
+    X acc = uo ? sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, uo, 1) : (X)0;
     if (row >= n) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
     const X r = f[(size_t)row * fs] - acc;
@@ -763,43 +826,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         X acc = f[(size_t)i * fs];
         const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
         const int s = i / WAVE, lane = i % WAVE;
-        const int k1 = sptr[s + 1];
-        int k = sptr[s];
-        for (; k + SELL_U <= k1; k += SELL_U) {
-            int j[SELL_U];
-            V v[SELL_U];
-#pragma unroll
-            for (int u = 0; u < SELL_U; u++) {
-                const size_t e = (size_t)(k + u) * WAVE + lane;
-                j[u] = col[e];
-                v[u] = val[e];
-            }
-            X xv[SELL_U];
-#pragma unroll
-            for (int u = 0; u < SELL_U; u++) {
-                const bool in = j[u] >= ns && j[u] < ns + len;
-                xv[u] = (!in && uo) ? uo[j[u]] : (X)0;
-            }
-#pragma unroll
-            for (int u = 0; u < SELL_U; u++) {
-                const X vv = sc * (X)cvt(v[u]);
-                if (j[u] >= ns && j[u] < ns + len) {
-                    if (j[u] != i) Brow[j[u] - ns] += vv;
-                } else if (uo) {
-                    acc -= vv * xv[u];
-                }
-            }
-        }
-        for (; k < k1; k++) {
-            const size_t e = (size_t)k * WAVE + lane;
-            const int j = col[e];
-            const X v = sc * (X)cvt(val[e]);
-            if (j >= ns && j < ns + len) {
-                if (j != i) Brow[j - ns] += v;
-            } else if (uo) {
-                acc -= v * uo[j];
-            }
-        }
+        chunk_row_gather<SELL_U, V, X>(sptr[s], sptr[s + 1], (size_t)lane, i, ns, len, sc, col, val, uo, Brow, acc);
         dd[r] = diag[i];
         g[r] = acc;
         ux[r] = uo ? uo[i] : (X)0;
@@ -841,10 +868,12 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
 
 // ---------------------------------------------------------------------------
 // Row-group variants for levels with long rows (coarse AMG levels hold rows of
-// tens to hundreds of entries): G lanes share one row of a plain CSR matrix
-// (ia, ja, val in natural order, the diagonal first), read its entries
-// G-strided (contiguous across the group) and reduce with cross-lane adds.
-// rsc[i] scales row i for FP16 storage.
+// tens to hundreds of entries), over the SELL-G layout: a slice holds
+// R = 64/G consecutive rows, G lanes per row, and slot row u of the slice
+// holds entries u*G .. u*G+G-1 of each row (row order, diagonal first; short
+// rows padded with zero values), so lane g of a group sums entries g, g+G, ...
+// and every slot load of a wavefront is one contiguous transaction.  Groups
+// reduce with cross-lane adds.  rsc[i] scales row i for FP16 storage.
 // ---------------------------------------------------------------------------
 template <int G, typename X> __device__ __forceinline__ X group_sum(X v)
 {
@@ -855,33 +884,18 @@ template <int G, typename X> __device__ __forceinline__ X group_sum(X v)
 
 //   mode 0: y = A x;  1: y = rhs - A x;  2: y += A x
 template <int G, typename V, typename X>
-__global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+__global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                       const V *__restrict__ val, const float *__restrict__ rsc,
                                                       const X *__restrict__ x, int xs, const X *__restrict__ rhs,
                                                       int rs, X *__restrict__ y, int mode)
 {
-    const int gl = threadIdx.x % G;
-    const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
-    X acc = 0;
-    if (row < n) {
-        const int ke = ia[row + 1];
-        for (int k = ia[row] + gl; k < ke; k += GRP_U * G) {
-            int j[GRP_U];
-            V v[GRP_U];
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++) {
-                const int kk = k + u * G;
-                j[u] = kk < ke ? ja[kk] : -1;
-                v[u] = kk < ke ? val[kk] : (V)0;
-            }
-            X xv[GRP_U];
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++) xv[u] = j[u] >= 0 ? x[(size_t)j[u] * xs] : (X)0;
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++)
-                if (j[u] >= 0) acc += (X)cvt(v[u]) * xv[u];
-        }
-    }
+    constexpr int R = WAVE / G;
+    const int nsl = (n + R - 1) / R;
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (s >= nsl) return; // whole wavefront
+    const int lane = threadIdx.x % WAVE, gl = lane % G;
+    const int row = s * R + lane / G;
+    X acc = sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, x, xs);
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -891,33 +905,18 @@ __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restri
 }
 
 template <int G, typename V, typename X>
-__global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+__global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                        const V *__restrict__ val, const float *__restrict__ rsc,
                                                        const X *__restrict__ dinv, const X *__restrict__ f, int fs,
                                                        const X *__restrict__ uo, X *__restrict__ un, X omega)
 {
-    const int gl = threadIdx.x % G;
-    const int row = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / G) + threadIdx.x / G;
-    X acc = 0;
-    if (row < n && uo) {
-        const int ke = ia[row + 1];
-        for (int k = ia[row] + gl; k < ke; k += GRP_U * G) {
-            int j[GRP_U];
-            V v[GRP_U];
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++) {
-                const int kk = k + u * G;
-                j[u] = kk < ke ? ja[kk] : -1;
-                v[u] = kk < ke ? val[kk] : (V)0;
-            }
-            X xv[GRP_U];
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++) xv[u] = j[u] >= 0 ? uo[j[u]] : (X)0;
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++)
-                if (j[u] >= 0) acc += (X)cvt(v[u]) * xv[u];
-        }
-    }
+    constexpr int R = WAVE / G;
+    const int nsl = (n + R - 1) / R;
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (s >= nsl) return;
+    const int lane = threadIdx.x % WAVE, gl = lane % G;
+    const int row = s * R + lane / G;
+    X acc = uo ? sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, uo, 1) : (X)0;
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -926,6 +925,9 @@ __global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restr
 }
 
 // chunked hybrid SSOR (as k_gs_chunk_lds) with phase A done by row groups
+// over plain CSR (ia, ja, val; diagonal first, kept apart in diag): a block
+// covers whole chunks, so its rows start anywhere in a slice of any layout;
+// plain rows carry no padding
 template <int LMAX, int G, typename V, typename X>
 __global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cpb, const int *__restrict__ ia,
                                                       const int *__restrict__ ja, const V *__restrict__ val,
@@ -954,33 +956,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cp
         X *Brow = B + (size_t)(c - c0) * BST + (i - ns) * LMAX;
         const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
         X acc = 0;
-        const int ke = ia[i + 1];
-        for (int k = ia[i] + 1 + gl; k < ke; k += GRP_U * G) {
-            int j[GRP_U];
-            V v[GRP_U];
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++) {
-                const int kk = k + u * G;
-                j[u] = kk < ke ? ja[kk] : -1;
-                v[u] = kk < ke ? val[kk] : (V)0;
-            }
-            X xv[GRP_U];
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++) {
-                const bool out = j[u] >= 0 && !(j[u] >= ns && j[u] < ns + len);
-                xv[u] = (out && uo) ? uo[j[u]] : (X)0;
-            }
-#pragma unroll
-            for (int u = 0; u < GRP_U; u++) {
-                if (j[u] < 0) continue;
-                const X vv = sc * (X)cvt(v[u]);
-                if (j[u] >= ns && j[u] < ns + len) {
-                    if (j[u] != i) atomicAdd(&Brow[j[u] - ns], vv);
-                } else if (uo) {
-                    acc -= vv * xv[u];
-                }
-            }
-        }
+        csr_row_gather<GRP_U, G, V, X>(ia[i] + 1 + gl, ia[i + 1], i, ns, len, sc, ja, val, uo, Brow, acc);
         acc = group_sum<G>(acc);
         if (gl == 0) {
             dd[r] = diag[i];
@@ -1505,28 +1481,38 @@ int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, co
         else return JXAMD_ERROR_ARG;                                                                                   \
     } while (0)
 
-int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+// grid of the SELL-G kernels: one wavefront per slice of 64/G rows
+static int grp_grid(int n, int G)
+{
+    const int rows = WAVE / G;
+    const int nsl = (n + rows - 1) / rows;
+    return (nsl + 3) / 4;
+}
+
+int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *sptr, const int *col, const void *val,
                          const float *rsc, const void *x, int xs, const void *rhs, int rs, void *y, int mode,
                          void *stream)
 {
     if (n == 0) return 0;
-    const int grid = (int)(((long)n * G + 255) / 256);
+    if (G < 4 || G > 32) return JXAMD_ERROR_ARG;
+    const int grid = grp_grid(n, G);
     DISPATCH_G(G, DISPATCH_VX(vt, xt,
                               (k_csr_spmv_grp<G_, V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
-                                  n, ia, ja, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y,
+                                  n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y,
                                   mode))));
     return chk(hipGetLastError());
 }
 
-int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
+int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *sptr, const int *col, const void *val,
                           const float *rsc, const void *dinv, const void *f, int fs, const void *uo, void *un,
                           double omega, void *stream)
 {
     if (n == 0) return 0;
-    const int grid = (int)(((long)n * G + 255) / 256);
+    if (G < 4 || G > 32) return JXAMD_ERROR_ARG;
+    const int grid = grp_grid(n, G);
     DISPATCH_G(G, DISPATCH_VX(vt, xt,
                               (k_poly_step_grp<G_, V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
-                                  n, ia, ja, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs,
+                                  n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs,
                                   (const X_ *)uo, (X_ *)un, (X_)omega))));
     return chk(hipGetLastError());
 }

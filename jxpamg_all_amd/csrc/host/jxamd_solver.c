@@ -40,7 +40,8 @@ typedef struct {
     void *val, *diag;        /* val: storage type vt; diag: vector type xt */
     float *rsc;              /* vt == F16: power-of-two scale per position */
     int vt, xt;
-    int G;                   /* > 1: plain CSR (sptr = ia, col = ja), G lanes per row */
+    int G;                   /* > 1: G lanes per row over SELL-G (sellg) or plain CSR (sptr = ia, col = ja) */
+    int sellg;
     double abytes;           /* algorithmic HBM bytes of one launch over it */
 } dsell;
 
@@ -587,10 +588,102 @@ static int choose_group(long nnz, int n)
     return G;
 }
 
-/* Upload a CSR matrix for the row-group kernels; F16 rows are scaled by the
- * smallest power of two at or above their largest off-diagonal magnitude
- * (the whole row if it has no off-diagonal); want_diag: also the diagonal
- * (first entry of each row) as a vector of type xt. */
+/* Upload a CSR matrix for the row-group kernels in the SELL-G layout: a
+ * slice holds R = 64/G consecutive rows, one group of G lanes per row; slot
+ * row u of the slice holds entries u*G .. u*G+G-1 of every row (row order,
+ * the diagonal first), so lane g of a group sums entries g, g+G, ... and a
+ * wavefront's load of one slot row is one contiguous transaction.  A slice
+ * has as many slot rows as its longest row needs; short rows are padded with
+ * a zero value and a valid column (the row itself when it is a column).
+ * F16 rows are scaled by the smallest power of two at or above their largest
+ * off-diagonal magnitude (the whole row if it has no off-diagonal).
+ * Used for the residual, restriction, prolongation and polynomial-smoother
+ * operators. */
+static int dsellg_upload(dsell *d, int n, int ncols, const int *ia, const int *ja, const double *a, int vt, int xt,
+                         int G, void *st)
+{
+    memset(d, 0, sizeof(*d));
+    d->n = n;
+    d->ncols = ncols;
+    d->vt = vt;
+    d->xt = xt;
+    d->G = G;
+    d->sellg = 1;
+    if (G < 2 || G > SELLW || SELLW % G) return JXAMD_ERROR_ARG;
+    const int R = SELLW / G;
+    const int nsl = (n + R - 1) / R;
+    d->nsl = nsl;
+    int *sp = (int *)malloc(sizeof(int) * ((size_t)nsl + 1));
+    if (!sp) return JXAMD_ERROR_MEMORY;
+    sp[0] = 0;
+    for (int q = 0; q < nsl; q++) {
+        int steps = 0;
+        for (int i = q * R; i < n && i < (q + 1) * R; i++) {
+            const int t = (ia[i + 1] - ia[i] + G - 1) / G;
+            if (t > steps) steps = t;
+        }
+        sp[q + 1] = sp[q] + steps;
+    }
+    const size_t tot = (size_t)sp[nsl] * SELLW;
+    int *cl = (int *)malloc(sizeof(int) * (tot ? tot : 1));
+    double *vl = (double *)calloc(tot ? tot : 1, sizeof(double));
+    float *sc = vt == JXAMD_T_F16 ? (float *)malloc(sizeof(float) * (n > 0 ? n : 1)) : NULL;
+    if (!cl || !vl || (vt == JXAMD_T_F16 && !sc)) { free(sp); free(cl); free(vl); free(sc); return JXAMD_ERROR_MEMORY; }
+    for (int q = 0; q < nsl; q++)
+        for (int u = sp[q]; u < sp[q + 1]; u++)
+            for (int ln = 0; ln < SELLW; ln++) {
+                const int i = q * R + ln / G;
+                cl[(size_t)u * SELLW + ln] = (i < n && i < ncols) ? i : 0;
+            }
+    for (int i = 0; i < n; i++) {
+        const int q = i / R, lr = i % R;
+        double scale = 1.0;
+        if (sc) {
+            double mx = 0.0;
+            const int k0 = ia[i + 1] - ia[i] > 1 ? ia[i] + 1 : ia[i];
+            for (int k = k0; k < ia[i + 1]; k++)
+                if (fabs(a[k]) > mx) mx = fabs(a[k]);
+            int ex = 0;
+            if (mx > 0.0) {
+                double m = frexp(mx, &ex);
+                if (m == 0.5) ex -= 1;
+            }
+            scale = mx > 0.0 ? ldexp(1.0, ex) : 1.0;
+            sc[i] = (float)scale;
+        }
+        for (int pp = 0; pp < ia[i + 1] - ia[i]; pp++) {
+            const size_t e = (size_t)(sp[q] + pp / G) * SELLW + (size_t)lr * G + pp % G;
+            cl[e] = ja[ia[i] + pp];
+            vl[e] = sc ? a[ia[i] + pp] / scale : a[ia[i] + pp];
+        }
+    }
+    int rc = upload_ints(&d->sptr, sp, (size_t)nsl + 1, st);
+    if (!rc) rc = upload_ints(&d->col, cl, tot, st);
+    if (!rc) rc = jxamd_dev_malloc(&d->val, type_size(vt) * (tot ? tot : 1));
+    void *tmp = NULL;
+    if (!rc && vt != JXAMD_T_F64) {
+        tmp = malloc(type_size(vt) * (tot ? tot : 1));
+        if (!tmp) rc = JXAMD_ERROR_MEMORY;
+        else if (vt == JXAMD_T_F32)
+            for (size_t k = 0; k < tot; k++) ((float *)tmp)[k] = (float)vl[k];
+        else
+            for (size_t k = 0; k < tot; k++) ((uint16_t *)tmp)[k] = half_bits((float)vl[k]);
+    }
+    if (!rc) rc = jxamd_h2d(d->val, tmp ? tmp : (void *)vl, type_size(vt) * tot, st);
+    if (!rc && sc) {
+        rc = jxamd_dev_malloc((void **)&d->rsc, sizeof(float) * (n > 0 ? n : 1));
+        if (!rc) rc = jxamd_h2d(d->rsc, sc, sizeof(float) * n, st);
+    }
+    if (!rc) rc = jxamd_stream_sync(st);
+    free(sp); free(cl); free(vl); free(sc); free(tmp);
+    CK(rc);
+    return 0;
+}
+
+/* Upload a CSR matrix as it is (sptr = ia, col = ja) for the chunked
+ * row-group smoother, whose blocks cover chunk-aligned row ranges; F16 rows
+ * are scaled as in dsellg_upload; want_diag: also the diagonal (first entry
+ * of each row) as a vector of type xt. */
 static int dcsr_upload(dsell *d, int n, int ncols, const int *ia, const int *ja, const double *a, int vt, int xt,
                        int G, int want_diag, void *st)
 {
@@ -1020,7 +1113,7 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         const jxamd_csr *Pm = &H->P[l];
         const int GP = choose_group(Pm->ia[Pm->n], Pm->n);
         if (GP > 1) {
-            e = dcsr_upload(&s->P[l], Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, s->vt, s->xt, GP, 0, s->stream);
+            e = dsellg_upload(&s->P[l], Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, s->vt, s->xt, GP, s->stream);
             memset(&h, 0, sizeof(h));
         } else {
             e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
@@ -1036,7 +1129,7 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         CK(csr_transpose(Pm->n, Pm->ncols, Pm->ia, Pm->ja, Pm->a, &ri, &rj, &ra));
         const int GR = choose_group(ri[Pm->ncols], Pm->ncols);
         if (GR > 1) {
-            e = dcsr_upload(&s->R[l], Pm->ncols, Pm->n, ri, rj, ra, s->vt, s->xt, GR, 0, s->stream);
+            e = dsellg_upload(&s->R[l], Pm->ncols, Pm->n, ri, rj, ra, s->vt, s->xt, GR, s->stream);
             memset(&h, 0, sizeof(h));
         } else {
             e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
@@ -1180,7 +1273,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     /* natural order, full rows: residual F - A U (and the polynomial smoother) */
     const int GA = choose_group(A->ia[n], n);
     if (GA > 1) {
-        CK(dcsr_upload(&s->Ares[l], n, n, A->ia, A->ja, A->a, s->vt, s->xt, GA, 0, s->stream));
+        CK(dsellg_upload(&s->Ares[l], n, n, A->ia, A->ja, A->a, s->vt, s->xt, GA, s->stream));
     } else {
         e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 0, 0);
         if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL, s->vt, s->xt);
