@@ -140,22 +140,27 @@ def main():
 
     mixed = None
     if not args.no_learned:
-        # mixed precision + learned polynomial smoother: FP32 stage 1 with the
-        # network's step sizes (trained here, on the GPU, on data the plain
-        # solver labels itself), FP64 stage 2 and GMRES; stage 2 as in the
-        # reference's mixed driver (two block sweeps, w 1.0, omega 0.8:
-        # src/test_bsr_cprgmres_hybrid.c:373-375, 648), its blocks stored in FP32
+        # mixed precision + learned polynomial smoother.  Reference-style mixed
+        # apply first: FP32 stage 1 (relax-6 chunks), FP64 GMRES, stage 2 as in
+        # the reference's mixed driver (two block sweeps, w 1.0, omega 0.8:
+        # src/test_bsr_cprgmres_hybrid.c:373-375, 648) with its blocks stored in
+        # FP32.  Then the mixed + learned configuration: FP16 stage 1 (row
+        # scaled, FP32 vectors) with a degree-2 polynomial smoother whose roots
+        # the network predicts (trained here, on the GPU, on data the plain
+        # solver labels itself; default Chebyshev roots where they do not win),
+        # one FP32-block stage-2 sweep
         from jxpamg_all_amd import learned as lrn
         t0 = time.time()
-        Xtr, Ttr, _ = lrn.make_dataset(range(1001, 1001 + args.train_seeds), precision=1)
+        Xtr, Ttr, _ = lrn.make_dataset(range(1001, 1001 + args.train_seeds), precision=2, m=2)
         net = lrn.MLP(seed=0)
         losses = net.fit(Xtr, Ttr, steps=3000, lr=3e-3)
         t_train = time.time() - t0
         log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
         legs = {}
-        stage2_kw = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
-        mixed_kw = dict(precond_precision=1, poly_degree=4, **stage2_kw)
-        for name, kw in (("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_kw)),
+        stage2_ref = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
+        stage2_one = dict(stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
+        mixed_kw = dict(precond_precision=2, poly_degree=2, **stage2_one)
+        for name, kw in (("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_ref)),
                          ("mixed_default_poly", mixed_kw), ("mixed_learned", mixed_kw)):
             pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, **kw)
             sm = jx.Solver(Ad, H, pm)

@@ -232,8 +232,9 @@ int jxamd_solver_num_levels(jxamd_solver *s, int *num_levels);
 /* ------------------------------------------------------------------------
  * Learned polynomial smoother (no reference counterpart).  A 3-layer MLP
  * (JXAMD_MLP_NFEAT level features -> 64 -> 64 -> JXAMD_MLP_NOUT, tanh hidden
- * units) predicts the four roots of each level's degree-4 polynomial smoother
- * in units of lambda_max(D^{-1} A): step w_k = 1 / (tau_k lambda_max).
+ * units) predicts the roots of each level's polynomial smoother of degree
+ * m <= 4 (its first m outputs) in units of lambda_max(D^{-1} A): step
+ * w_k = 1 / (tau_k lambda_max).  One network serves one degree.
  * BF16 operands on the matrix cores, FP32 accumulation and master weights.
  * Parameter layout (FP32): W1[64][16] b1[64] W2[64][64] b2[64] W3[4][64] b3[4].
  * ---------------------------------------------------------------------- */
@@ -243,18 +244,20 @@ int jxamd_solver_num_levels(jxamd_solver *s, int *num_levels);
 /* y[n][4] = MLP(x[n][16]), device pointers */
 int jxamd_mlp_forward(const float *d_params, const float *d_x, int n, float *d_y, void *stream);
 /* steps Adam steps (lr, betas 0.9/0.999, eps 1e-8; update = 0 only computes
- * the gradient) on the mean squared error over mini-batches of 32 samples
- * taken cyclically from (x[n][16], t[n][4]) starting at batch t0; d_grad
- * receives the last step's gradient, d_loss[steps] (optional) the losses */
+ * the gradient) on the mean squared error of the first nout (1..4) outputs
+ * over mini-batches of 32 samples taken cyclically from (x[n][16],
+ * t[n][nout]) starting at batch t0; d_grad receives the last step's
+ * gradient, d_loss[steps] (optional) the losses */
 int jxamd_mlp_train(float *d_params, float *d_m, float *d_v, float *d_grad, const float *d_x, const float *d_t, int n,
-                    int steps, int t0, float lr, int update, float *d_loss, void *stream);
-/* the features of one smoothing level (solver created with poly_degree 4) */
+                    int nout, int steps, int t0, float lr, int update, float *d_loss, void *stream);
+/* the features of one smoothing level (solver created with poly_degree > 0) */
 int jxamd_solver_poly_features(jxamd_solver *s, int level, float *feat);
 /* V-cycle contraction factor: ||e_k|| / ||e_0|| to the power 1/ncycles for
  * ncycles V-cycles on f = 0 from a fixed pseudo-random e_0 */
 int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor);
-/* Predict every level's polynomial roots tau_k (w_k = 1 / (tau_k lambda_max))
- * with the MLP (d_params on the device) and, level by level, keep them only
+/* Predict every level's polynomial roots tau_k (w_k = 1 / (tau_k lambda_max),
+ * the first poly_degree outputs; poly_degree <= 4) with the MLP (d_params on
+ * the device) and, level by level, keep them only
  * where they lower the measured V-cycle contraction factor; elsewhere the
  * default Chebyshev steps stay (the fallback).  *used = number of levels on
  * learned steps; final and default factors reported if non-NULL. */

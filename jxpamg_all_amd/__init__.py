@@ -119,7 +119,7 @@ def _declare(L):
         "jxamd_solver_get_poly": [vp, i32, i32, vp, vp],
         "jxamd_solver_num_levels": [vp, vp],
         "jxamd_mlp_forward": [vp, vp, i32, vp, vp],
-        "jxamd_mlp_train": [vp, vp, vp, vp, vp, vp, i32, i32, i32, ctypes.c_float, i32, vp, vp],
+        "jxamd_mlp_train": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, ctypes.c_float, i32, vp, vp],
         "jxamd_solver_poly_features": [vp, i32, vp],
         "jxamd_solver_vcycle_factor": [vp, i32, vp],
         "jxamd_solver_apply_mlp": [vp, vp, vp, vp, vp],

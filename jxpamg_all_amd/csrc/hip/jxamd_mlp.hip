@@ -98,13 +98,14 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ P
     for (int e = threadIdx.x; e < nb * 4; e += blockDim.x) y[(size_t)b0 * 4 + e] = a.Y[e];
 }
 
-// steps Adam steps on the mean squared error over mini-batches of MB samples
-// taken cyclically from (x, t); step s uses samples (t0 + s) * MB + r mod n.
+// steps Adam steps on the mean squared error of the first nout outputs over
+// mini-batches of MB samples taken cyclically from (x, t[n][nout]); step s
+// uses samples (t0 + s) * MB + r mod n.  The other outputs get no gradient.
 // G receives the gradient of the last step, loss[s] the loss before step s.
 __global__ void __launch_bounds__(256) k_mlp_train(float *__restrict__ P, float *__restrict__ M, float *__restrict__ V,
                                                    float *__restrict__ G, const float *__restrict__ x,
-                                                   const float *__restrict__ t, int n, int steps, int t0, float lr,
-                                                   float beta1, float beta2, float eps, int update,
+                                                   const float *__restrict__ t, int n, int nout, int steps, int t0,
+                                                   float lr, float beta1, float beta2, float eps, int update,
                                                    float *__restrict__ loss)
 {
     __shared__ Acts a;
@@ -118,11 +119,11 @@ __global__ void __launch_bounds__(256) k_mlp_train(float *__restrict__ P, float 
         }
         __syncthreads();
         forward(P, a, nb);
-        // dY = 2 (y - t) / nb into a.S (nb x 4), loss
+        // dY = 2 (y - t) / nb into a.S (nb x 4; zero beyond nout), loss
         float part = 0.f;
         for (int e = threadIdx.x; e < nb * 4; e += blockDim.x) {
-            const int r = e / 4;
-            const float d = a.Y[e] - t[(size_t)((base + r) % n) * 4 + e % 4];
+            const int r = e / 4, o = e % 4;
+            const float d = o < nout ? a.Y[e] - t[(size_t)((base + r) % n) * nout + o] : 0.f;
             part += d * d;
             a.S[e] = 2.f * d / (float)nb;
         }
@@ -194,10 +195,12 @@ int jxamd_mlp_forward(const float *d_params, const float *d_x, int n, float *d_y
 }
 
 int jxamd_mlp_train(float *d_params, float *d_m, float *d_v, float *d_grad, const float *d_x, const float *d_t, int n,
-                    int steps, int t0, float lr, int update, float *d_loss, void *stream)
+                    int nout, int steps, int t0, float lr, int update, float *d_loss, void *stream)
 {
-    if (n < 1 || steps < 1 || t0 < 0 || !d_params || !d_grad || (update && (!d_m || !d_v))) return JXAMD_ERROR_ARG;
-    k_mlp_train<<<1, 256, 0, (hipStream_t)stream>>>(d_params, d_m, d_v, d_grad, d_x, d_t, n, steps, t0, lr, 0.9f,
+    if (n < 1 || nout < 1 || nout > JXAMD_MLP_NOUT || steps < 1 || t0 < 0 || !d_params || !d_grad ||
+        (update && (!d_m || !d_v)))
+        return JXAMD_ERROR_ARG;
+    k_mlp_train<<<1, 256, 0, (hipStream_t)stream>>>(d_params, d_m, d_v, d_grad, d_x, d_t, n, nout, steps, t0, lr, 0.9f,
                                                     0.999f, 1e-8f, update, d_loss);
     return chk(hipGetLastError());
 }

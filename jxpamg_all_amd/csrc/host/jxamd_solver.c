@@ -1897,7 +1897,8 @@ int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor)
 int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, double *factor_learned,
                            double *factor_default)
 {
-    if (!s || !d_params || s->prm.poly_degree != JXAMD_MLP_NOUT) return JXAMD_ERROR_ARG;
+    if (!s || !d_params || s->prm.poly_degree < 1 || s->prm.poly_degree > JXAMD_MLP_NOUT) return JXAMD_ERROR_ARG;
+    const int m = s->prm.poly_degree; /* the network's first m outputs */
     const int Ls = s->L > 1 ? s->L - 1 : 1;
     float hy[JXAMD_MAX_LEVELS * JXAMD_MLP_NOUT];
     float *dx = NULL, *dy = NULL;
@@ -1920,21 +1921,21 @@ int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, do
     for (int l = 0; l < Ls; l++) {
         double keep[8], w[8];
         int ok = 1;
-        for (int k = 0; k < JXAMD_MLP_NOUT; k++) {
+        for (int k = 0; k < m; k++) {
             const double tau = (double)hy[l * JXAMD_MLP_NOUT + k];
             if (!(tau > 5e-3) || !(tau < 4.0)) ok = 0; /* also rejects NaN */
             w[k] = 1.0 / (tau * s->lam[l]);
             keep[k] = s->poly_w[l][k];
         }
         if (!ok) continue;
-        for (int k = 0; k < JXAMD_MLP_NOUT; k++) s->poly_w[l][k] = w[k];
+        for (int k = 0; k < m; k++) s->poly_w[l][k] = w[k];
         double f = 0.0;
         CK(jxamd_solver_vcycle_factor(s, 4, &f));
         if (f < best) {
             best = f;
             kept++;
         } else {
-            for (int k = 0; k < JXAMD_MLP_NOUT; k++) s->poly_w[l][k] = keep[k];
+            for (int k = 0; k < m; k++) s->poly_w[l][k] = keep[k];
         }
     }
     if (used) *used = kept;

@@ -38,12 +38,14 @@ def forward(P, X):
     return Y, (X, H1, H2)
 
 
-def loss_and_grad(P, X, T):
-    """Mean over samples of the summed squared error, and its gradient."""
+def loss_and_grad(P, X, T, nout=4):
+    """Mean over samples of the summed squared error of the first nout
+    outputs (T: [n][nout]), and its gradient."""
     p = unpack(P)
     Y, (X, H1, H2) = forward(P, X)
     nb = len(X)
-    R = Y - np.asarray(T, dtype=np.float64)
+    R = np.zeros_like(Y)
+    R[:, :nout] = Y[:, :nout] - np.asarray(T, dtype=np.float64).reshape(nb, nout)
     loss = float((R * R).sum() / nb)
     dY = 2.0 * R / nb
     g = {"W3": bf16(dY).T @ bf16(H2), "b3": dY.sum(0)}
@@ -56,11 +58,11 @@ def loss_and_grad(P, X, T):
     return loss, pack(g)
 
 
-def loss_exact(P, X, T):
+def loss_exact(P, X, T, nout=4):
     """The same loss without BF16 rounding (float64 throughout)."""
     p = unpack(P)
     H1 = np.tanh(X @ p["W1"].T + p["b1"])
     H2 = np.tanh(H1 @ p["W2"].T + p["b2"])
     Y = H2 @ p["W3"].T + p["b3"]
-    R = Y - T
+    R = Y[:, :nout] - np.asarray(T).reshape(len(X), nout)
     return float((R * R).sum() / len(X))

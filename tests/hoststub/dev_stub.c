@@ -103,8 +103,8 @@ int jxamd_mlp_forward(const float *d_params, const float *d_x, int n, float *d_y
     return 0;
 }
 int jxamd_mlp_train(float *d_params, float *d_m, float *d_v, float *d_grad, const float *d_x, const float *d_t, int n,
-                    int steps, int t0, float lr, int update, float *d_loss, void *stream)
-{ (void)d_params; (void)d_m; (void)d_v; (void)d_grad; (void)d_x; (void)d_t; (void)n; (void)steps; (void)t0; (void)lr; (void)update; (void)d_loss; (void)stream; return 0; }
+                    int nout, int steps, int t0, float lr, int update, float *d_loss, void *stream)
+{ (void)d_params; (void)d_m; (void)d_v; (void)d_grad; (void)d_x; (void)d_t; (void)n; (void)nout; (void)steps; (void)t0; (void)lr; (void)update; (void)d_loss; (void)stream; return 0; }
 
 int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *ia, const int *ja, const void *val,
                          const float *rsc, const void *x, int xs, const void *rhs, int rs, void *y, int mode,

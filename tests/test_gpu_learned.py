@@ -33,6 +33,13 @@ def test_mlp_forward_and_gradient_vs_checker():
     assert abs(loss - loss_ref) < 1e-2 * loss_ref
     assert np.max(np.abs(g - g_ref)) < GRAD_TOL * np.max(np.abs(g_ref))
     assert np.corrcoef(g, g_ref)[0, 1] > 0.999
+    # targets for the first two outputs only: the others get no gradient
+    loss2, g2 = net.gradient(X, T[:, :2])
+    loss2_ref, g2_ref = mlp_ref.loss_and_grad(P, X, T[:, :2], 2)
+    assert abs(loss2 - loss2_ref) < 1e-2 * loss2_ref
+    assert np.max(np.abs(g2 - g2_ref)) < GRAD_TOL * np.max(np.abs(g2_ref))
+    p2 = mlp_ref.unpack(g2)
+    assert not np.any(p2["W3"][2:]) and not np.any(p2["b3"][2:])
     # more samples than one batch: every batch of 32 runs the same network
     X2 = rng.normal(0, 1, (77, 16)).astype(np.float32)
     assert relerr(net.forward(X2), mlp_ref.forward(P, X2)[0]) < FWD_TOL
@@ -52,12 +59,13 @@ def test_mlp_training_on_gpu_fits_a_smooth_map():
     assert final < 0.05 * first
 
 
-def test_learned_smoother_gain_on_unseen_systems():
+@pytest.mark.parametrize("deg,prec", [(4, 1), (2, 2)])
+def test_learned_smoother_gain_on_unseen_systems(deg, prec):
     """Train on systems the plain solver labels itself, then compare the
     V-cycle contraction and the GMRES iterations of learned and default
     steps on unseen seeds; learned steps are kept only where they win."""
-    print("[gpu] learned smoother: dataset", flush=True)
-    X, T, info = L.make_dataset(range(1001, 1021), precision=1)
+    print(f"[gpu] learned smoother degree {deg} precision {prec}: dataset", flush=True)
+    X, T, info = L.make_dataset(range(1001, 1021), precision=prec, m=deg)
     print(f"[gpu] dataset {X.shape} best/default factors {[(round(b, 3), round(d, 3)) for _, b, d in info]}",
           flush=True)
     net = L.MLP(seed=0)
@@ -65,9 +73,9 @@ def test_learned_smoother_gain_on_unseen_systems():
     print(f"[gpu] training loss {losses[0]:.4f} -> {losses[-1]:.5f}", flush=True)
     assert losses[-1] < 0.2 * losses[0]
     gains = []
-    for seed, grid in ((2001, (20, 20, 10)), (2002, (24, 24, 12)), (2003, (28, 20, 10))):
-        Ad, bd, H = L.training_system(seed, *grid)
-        p = jx.default_params(poly_degree=4, precond_precision=1, tol=1e-6)
+    for seed, grid, layered in ((2001, (20, 20, 10), False), (2002, (24, 24, 12), False), (2003, (30, 70, 26), True)):
+        Ad, bd, H = L.training_system(seed, *grid, layered=layered)
+        p = jx.default_params(poly_degree=deg, precond_precision=prec, tol=1e-6)
         s_def = jx.Solver(Ad, H, p)
         _, st_def, _ = s_def.solve(bd)
         f_def = s_def.vcycle_factor()

@@ -338,24 +338,28 @@ def test_vcycle_with_poly_smoother_is_linear_and_converges():
     assert st.converged == 1 and np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd) <= 1e-8
 
 
-def test_mlp_checker_gradient_matches_finite_differences():
+@pytest.mark.parametrize("nout", [4, 2])
+def test_mlp_checker_gradient_matches_finite_differences(nout):
     """The checker's backward pass (BF16-rounded operands) against central
     differences of the unrounded loss: agreement to the BF16 rounding level
-    pins the transposes, signs and tanh derivatives."""
+    pins the transposes, signs and tanh derivatives; with nout < 4 the unused
+    outputs' weights get exactly zero gradient."""
     from oracle import mlp_ref as M
     rng = np.random.default_rng(0)
     P = rng.normal(0, 0.2, M.NPARAM)
-    X, T = rng.normal(0, 1, (20, 16)), rng.normal(0, 1, (20, 4))
-    _, g = M.loss_and_grad(P, X, T)
+    X, T = rng.normal(0, 1, (20, 16)), rng.normal(0, 1, (20, nout))
+    _, g = M.loss_and_grad(P, X, T, nout)
     idx = rng.choice(M.NPARAM, 40, replace=False)
     fd = []
     for i in idx:
         e = np.zeros(M.NPARAM)
         e[i] = 1e-5
-        fd.append((M.loss_exact(P + e, X, T) - M.loss_exact(P - e, X, T)) / 2e-5)
+        fd.append((M.loss_exact(P + e, X, T, nout) - M.loss_exact(P - e, X, T, nout)) / 2e-5)
     fd = np.array(fd)
     assert np.max(np.abs(fd - g[idx])) < 0.03 * np.max(np.abs(fd))
     assert np.corrcoef(fd, g[idx])[0, 1] > 0.999
+    p = M.unpack(g)
+    assert not np.any(p["W3"][nout:]) and not np.any(p["b3"][nout:])
 
 
 @pytest.mark.parametrize("name", ["tiny", "b4", "b2"])
