@@ -171,10 +171,17 @@ typedef struct {
      * (jxfpamg-0220/csrc/amg/par_relax_bsr.c:296-330, 523-557). */
     int stage2_chunk;
     /* Storage of the partitioned stage-2 sweeps' matrix blocks and inverted
-     * diagonal blocks: 0 FP64, 1 FP32 (FP64 vectors and arithmetic); used
-     * with stage2_chunk > 0.  No reference counterpart (its mixed driver
-     * keeps stage 2 in FP64). */
+     * diagonal blocks: 0 FP64, 1 FP32, 2 FP16 blocks scaled by a power of
+     * two per block row with FP32 inverted diagonals (FP64 vectors and
+     * arithmetic throughout); used with stage2_chunk > 0.  No reference
+     * counterpart (its mixed driver keeps stage 2 in FP64). */
     int stage2_precision;
+    /* Arnoldi orthogonalisation kernels: 0 one cooperative launch per MGS
+     * step with the new vector held in registers (when the device supports
+     * cooperative launches and the vector fits the resident grid), 1 one
+     * launch per projection.  Same arithmetic (MGS, pgmres.c:1258-1290
+     * order); the reductions group their partial sums differently. */
+    int arnoldi_kernel;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread

@@ -52,10 +52,12 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
                        long long xstride, double *xout, const int *dlo_ptr, const int *dlo_list, const int *dall_ptr,
                        const int *dall_list, int *sflag, int epoch, double w, double omega,
                        unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
-/* one stage-2 sweep as on nparts ranks (partitions of at most 8 rows) */
+/* one stage-2 sweep as on nparts ranks (partitions of at most 8 rows);
+ * blocks vt = F64 (dinv F64), F32 (dinv F32) or F16 scaled by rsc per block
+ * row (dinv F32) */
 int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const void *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
-                      double w, double omega, void *stream);
+                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const double *xp,
+                      int pidx, double *xn, double w, double omega, void *stream);
 /* BILU(0) block triangular solve (vt: F64 or F32 factor storage) */
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                  const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
@@ -93,6 +95,12 @@ int jxamd_k_copy(size_t n, const double *x, double *y, void *stream);
 int jxamd_k_axpy(size_t n, const double *pa, double a, int neg, const double *x, double *y, void *stream);
 int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *stream);
 int jxamd_dot_grid(size_t n);
+/* one MGS step of the Arnoldi process (k_mgs_coop): plan (grid 0: not
+ * available) and cooperative launch; hval[0..m-1] = projections, hval[m] =
+ * squared norm before scaling; part >= (m + 1) * grid doubles */
+int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm);
+int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, void *stream);
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream);
 int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream);

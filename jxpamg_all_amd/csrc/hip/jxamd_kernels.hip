@@ -514,10 +514,14 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
 // lower blocks go to LDS and one lane per partition runs the short forward
 // recurrence (phase B).  x_old: xo (B values per row) or, when xo ==
 // nullptr, the prolongated pressure xp (slot pidx, zeros elsewhere).
-template <int B, int LMAX, typename V>
+// Blocks are stored as V (double, float, or __half scaled per block row:
+// A_ij = rsc[i] * H_ij), inverted diagonal blocks as DV; arithmetic in
+// double.  PU slots per batch.
+template <int B, int LMAX, typename V, typename DV, int PU>
 __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, const int *__restrict__ sptr,
                                                    const int *__restrict__ col, const V *__restrict__ val,
-                                                   const V *__restrict__ dinv, const double *__restrict__ rhs,
+                                                   const float *__restrict__ rsc, const DV *__restrict__ dinv,
+                                                   const double *__restrict__ rhs,
                                                    const double *__restrict__ xo, const double *__restrict__ xp,
                                                    int pidx, double *__restrict__ xn, double w, double omega)
 {
@@ -550,28 +554,44 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
             res2[q] = 0.0;
         }
         const int s = i / WAVE, lane = i % WAVE;
-        for (int k = sptr[s]; k < sptr[s + 1]; k++) {
-            const int j = col[(size_t)k * WAVE + lane];
-            if (j == i) continue;
-            const V *vv = val + (size_t)k * B * B * WAVE + lane;
-            double xj[B];
+        const double sc = row_scaled<V>::value ? (double)rsc[i] : 1.0;
+        const int k1 = sptr[s + 1];
+        for (int k = sptr[s]; k < k1; k += PU) {
+            // a batch of PU slots: indices, then the neighbours' x, then the
+            // blocks (their loads do not wait on the gathers)
+            int jj[PU];
+            double xj[PU][B];
 #pragma unroll
-            for (int q = 0; q < B; q++) xj[q] = xo ? xo[(size_t)j * B + q] : (q == pidx ? xp[j] : 0.0);
-            const bool in = j >= ns && j < ns + len;
-            if (in && j < i) {
-                double *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
+            for (int u = 0; u < PU; u++) jj[u] = k + u < k1 ? col[(size_t)(k + u) * WAVE + lane] : i;
 #pragma unroll
-                for (int e = 0; e < B * B; e++) L[e] = (double)ldnt(vv + e * WAVE);
+            for (int u = 0; u < PU; u++) {
+                const int j = jj[u];
+#pragma unroll
+                for (int q = 0; q < B; q++)
+                    xj[u][q] = j == i ? 0.0 : (xo ? xo[(size_t)j * B + q] : (q == pidx ? xp[j] : 0.0));
             }
 #pragma unroll
-            for (int rr = 0; rr < B; rr++) {
-                double acc = 0.0;
+            for (int u = 0; u < PU; u++) {
+                const int j = jj[u];
+                if (j == i) continue; // the diagonal block, padding, or past the slice
+                const V *vv = val + (size_t)(k + u) * B * B * WAVE + lane;
+                const bool in = j >= ns && j < ns + len;
+                if (in && j < i) {
+                    double *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
 #pragma unroll
-                for (int q = 0; q < B; q++) acc += (double)ldnt(vv + (rr * B + q) * WAVE) * xj[q];
-                if (!in) res[rr] -= acc;
-                else {
-                    res2[rr] += acc;
-                    if (j > i) res0[rr] -= acc;
+                    for (int e = 0; e < B * B; e++) L[e] = sc * (double)cvt(ldnt(vv + e * WAVE));
+                }
+#pragma unroll
+                for (int rr = 0; rr < B; rr++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < B; q++) acc += (double)cvt(ldnt(vv + (rr * B + q) * WAVE)) * xj[u][q];
+                    acc *= sc;
+                    if (!in) res[rr] -= acc;
+                    else {
+                        res2[rr] += acc;
+                        if (j > i) res0[rr] -= acc;
+                    }
                 }
             }
         }
@@ -606,7 +626,7 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
                             for (int q = 0; q < B; q++) tmp[rr] -= L[rr * B + q] * xnew[u][q];
                     }
                 }
-                const V *D = dinv + (size_t)(ns + t) * B * B;
+                const DV *D = dinv + (size_t)(ns + t) * B * B;
 #pragma unroll
                 for (int rr = 0; rr < B; rr++) {
                     double o = 0.0;
@@ -1168,6 +1188,121 @@ __global__ void __launch_bounds__(256) k_dot_mgs(size_t n, const double *u, doub
     }
 }
 
+// Sum of v over the workgroup (all threads call it), in a fixed order:
+// wavefront shuffles, then the wavefront sums in order.  Every thread gets
+// the total; red holds blockDim.x / 64 doubles and is free again on return.
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int q = 0; q < (int)(blockDim.x / WAVE); q++) s += red[q];
+    __syncthreads();
+    return s;
+}
+
+// One whole modified Gram-Schmidt step of the Arnoldi process in a single
+// cooperative launch (every workgroup resident).  Each thread keeps TM
+// entries of w in registers through the step:
+//   for k = 0..m-1:  h_k = <v_k, w>;  w -= h_k v_k
+//   h_m = <w, w>;    w /= sqrt(h_m)   (w kept as is when h_m = 0)
+// so a step reads every v_k once and w once and writes w once (the launch-
+// per-projection form reads and writes w for every k).  Each reduction is
+// deterministic: per-workgroup partials in part[k * gridDim.x + b], summed in
+// block order by the last workgroup to arrive (counter cnt), which publishes
+// h_k in hval[k] and raises *flag to base + k + 1; the others wait on the
+// flag (bounded: an overflow sets err[0] and the step goes on).
+#define MGS_BS 1024 // threads per workgroup of k_mgs_coop: few workgroups, few arrivals per reduction
+
+template <int TM>
+__global__ void __launch_bounds__(MGS_BS) k_mgs_coop(size_t n, const double *__restrict__ V, size_t ld, int m,
+                                                     double *__restrict__ w, double *hval, double *part,
+                                                     unsigned int *cnt, int *flag, int base, int *err)
+{
+    __shared__ double red[MGS_BS / WAVE];
+    __shared__ double hb;
+    __shared__ int last;
+    // wavefront q owns the contiguous entries [q * 64 * TM, (q + 1) * 64 * TM):
+    // entry t of a lane is wb + 64 t + lane, so every load is one 512-byte
+    // row at a wave-uniform address plus the lane offset (no per-entry
+    // address registers)
+    const unsigned int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+    const size_t wb = (size_t)wave * WAVE * TM;
+    const unsigned int lane = threadIdx.x % WAVE;
+    double wr[TM];
+#pragma unroll
+    for (int t = 0; t < TM; t++) {
+        const size_t e = wb + (size_t)t * WAVE + lane;
+        wr[t] = e < n ? w[e] : 0.0;
+    }
+    for (int k = 0; k <= m; k++) {
+        double vr[TM];
+        double acc = 0.0;
+        if (k < m) {
+            const double *v = V + (size_t)k * ld + wb + lane;
+#pragma unroll
+            for (int t = 0; t < TM; t++) vr[t] = wb + (size_t)t * WAVE + lane < n ? v[t * WAVE] : 0.0;
+#pragma unroll
+            for (int t = 0; t < TM; t++) acc += vr[t] * wr[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < TM; t++) acc += wr[t] * wr[t];
+        }
+        const double b = block_sum(acc, red);
+        const int target = base + k + 1;
+        double *pk = part + (size_t)k * gridDim.x;
+        if (threadIdx.x == 0) {
+            st_coherent(pk + blockIdx.x, b);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_s_waitcnt(0);
+            const unsigned int prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = prev == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (last) {
+            // the last workgroup to arrive sums the partials (every thread a
+            // strided share, then the fixed-order block sum) and publishes h_k
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            double q = 0.0;
+            for (unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x) q += ld_coherent(pk + i);
+            const double sum = block_sum(q, red);
+            if (threadIdx.x == 0) {
+                st_coherent(hval + k, sum);
+                __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                st_flag_release(flag, target);
+                hb = sum;
+            }
+        } else if (threadIdx.x == 0) {
+            int spins = 0;
+            while (!flag_ready(ld_flag(flag), target)) {
+                if (++spins > SPIN_LIMIT) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            hb = ld_coherent(hval + k);
+        }
+        __syncthreads();
+        const double h = hb;
+        if (k < m) {
+#pragma unroll
+            for (int t = 0; t < TM; t++) wr[t] -= h * vr[t];
+        } else {
+            const double nrm = sqrt(h);
+            const double sc = nrm != 0.0 ? 1.0 / nrm : 1.0;
+#pragma unroll
+            for (int t = 0; t < TM; t++) {
+                const size_t e = wb + (size_t)t * WAVE + lane;
+                if (e < n) w[e] = wr[t] * sc;
+            }
+        }
+        __syncthreads(); // hb and last are rewritten by the next reduction
+    }
+}
+
 // w = sum_j y[j] * V[j]   (j = 0..m-1, V rows of length n, ld apart), y on host
 __global__ void k_lincomb(size_t n, int m, const double *__restrict__ V, size_t ld, const double *__restrict__ y,
                           double *__restrict__ w)
@@ -1360,25 +1495,33 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
 }
 
 int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const void *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
-                      double w, double omega, void *stream)
+                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const double *xp,
+                      int pidx, double *xn, double w, double omega, void *stream)
 {
-    if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32) return JXAMD_ERROR_ARG;
+    if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32 && vt != JXAMD_T_F16) return JXAMD_ERROR_ARG;
     if (nb == 0) return 0;
     const int maxlen = jxamd_gs_chunk_lds_rows(nb, nparts);
-    if (maxlen < 1 || maxlen > 8 || (!xo && !xp)) return JXAMD_ERROR_ARG;
+    if (maxlen < 1 || maxlen > 8 || (!xo && !xp) || (vt == JXAMD_T_F16 && !rsc)) return JXAMD_ERROR_ARG;
     hipStream_t st = (hipStream_t)stream;
+    // FP64 blocks keep one slot per step (nine doubles per block already keep
+    // a lane's loads busy); narrow blocks take four slots per batch
 #define LAUNCH_PART(BS, LM)                                                                                            \
     do {                                                                                                               \
         const int cpb = 256 / LM;                                                                                      \
         const int grid = (nparts + cpb - 1) / cpb;                                                                     \
         const size_t lds = sizeof(double) * (2 * 256 * BS + (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1));       \
         if (vt == JXAMD_T_F64)                                                                                        \
-            k_bhgs_part<BS, LM, double><<<grid, 256, lds, st>>>(nb, nparts, cpb, sptr, col, (const double *)val,      \
-                                                                (const double *)dinv, rhs, xo, xp, pidx, xn, w, omega); \
+            k_bhgs_part<BS, LM, double, double, 1><<<grid, 256, lds, st>>>(                                            \
+                nb, nparts, cpb, sptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo, xp, pidx, xn, w,  \
+                omega);                                                                                                \
+        else if (vt == JXAMD_T_F32)                                                                                    \
+            k_bhgs_part<BS, LM, float, float, 4><<<grid, 256, lds, st>>>(                                              \
+                nb, nparts, cpb, sptr, col, (const float *)val, rsc, (const float *)dinv, rhs, xo, xp, pidx, xn, w,    \
+                omega);                                                                                                \
         else                                                                                                           \
-            k_bhgs_part<BS, LM, float><<<grid, 256, lds, st>>>(nb, nparts, cpb, sptr, col, (const float *)val,        \
-                                                               (const float *)dinv, rhs, xo, xp, pidx, xn, w, omega);  \
+            k_bhgs_part<BS, LM, __half, float, 4><<<grid, 256, lds, st>>>(                                             \
+                nb, nparts, cpb, sptr, col, (const __half *)val, rsc, (const float *)dinv, rhs, xo, xp, pidx, xn, w,   \
+                omega);                                                                                                \
     } while (0)
 #define PART_B(BS)                                                                                                     \
     do {                                                                                                               \
@@ -1617,6 +1760,52 @@ int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *str
     if (n == 0) return 0;
     k_scale<<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, pnorm2, a, y);
     return chk(hipGetLastError());
+}
+
+// Cooperative MGS plan: the largest resident grid of MGS_BS-thread
+// workgroups for the smallest register tile TM in {4, 8, 16} that covers n;
+// 0 if none does or the device has no cooperative launch.
+int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm)
+{
+    *grid = 0;
+    *tm = 0;
+    int dev = 0, coop = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return JXAMD_ERROR_DEVICE;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess) return JXAMD_ERROR_DEVICE;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return JXAMD_ERROR_DEVICE;
+    if (!coop || cus < 1) return 0;
+    const void *fn[3] = {(const void *)k_mgs_coop<4>, (const void *)k_mgs_coop<8>, (const void *)k_mgs_coop<16>};
+    const int tms[3] = {4, 8, 16};
+    for (int q = 0; q < 3; q++) {
+        int per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn[q], MGS_BS, 0) != hipSuccess)
+            return JXAMD_ERROR_DEVICE;
+        if (per < 1) continue;
+        const size_t g = (size_t)per * cus;
+        if (g * MGS_BS * (size_t)tms[q] >= n) {
+            // the fewest workgroups that still cover n with this tile
+            size_t need = (n + MGS_BS * (size_t)tms[q] - 1) / (MGS_BS * (size_t)tms[q]);
+            *grid = (int)(need < g ? need : g);
+            *tm = tms[q];
+            return 0;
+        }
+    }
+    return 0;
+}
+
+int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, void *stream)
+{
+    if (grid < 1 || m < 0 || (size_t)grid * MGS_BS * (size_t)tm < n) return JXAMD_ERROR_ARG;
+    void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err};
+    const void *fn;
+    switch (tm) {
+    case 4: fn = (const void *)k_mgs_coop<4>; break;
+    case 8: fn = (const void *)k_mgs_coop<8>; break;
+    case 16: fn = (const void *)k_mgs_coop<16>; break;
+    default: return JXAMD_ERROR_ARG;
+    }
+    return chk(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(MGS_BS), args, 0, (hipStream_t)stream));
 }
 
 int jxamd_dot_grid(size_t n)

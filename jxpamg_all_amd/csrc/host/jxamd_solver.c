@@ -54,8 +54,9 @@ typedef struct {
     int *gperm, *gsptr, *gcol, *flag, *gdep_ptr, *gdep_list;
     int *gall_ptr, *gall_list, *gvorder; /* neighbourhood slices, sweep-overlapped order */
     double *gval, *dinv, *xs;  /* xs: x^(k) of the K sweeps in slice order, then x^(-1) */
-    void *val32, *dinv32;      /* FP32 copies for the partitioned stage 2 (stage2_precision 1) */
-    double spmv_bytes, sweep_bytes, sweep_bytes32; /* algorithmic bytes: y = A x, one block-HGS sweep (FP64, FP32 blocks) */
+    void *val32, *dinv32;      /* FP32 (stage2_precision 1) or FP16 blocks (2; FP32 dinv) for the partitioned stage 2 */
+    float *rsc16;              /* stage2_precision 2: power-of-two scale per block row */
+    double spmv_bytes, sweep_bytes, sweep_bytes32; /* algorithmic bytes: y = A x, one block-HGS sweep (FP64, narrow blocks) */
 } dbsr;
 
 /* one BILU(0) triangle on the device: level-ordered block SELL of the
@@ -109,7 +110,10 @@ struct jxamd_solver_s {
     double *Rv, *Wv, *Bd, *Xd; /* GMRES work, host-solve staging */
     double *dsc;           /* device scalars: h[0..k+1], y[0..k], tmp */
     double *part;
-    unsigned int *count;
+    unsigned int *count;       /* [0] k_dot_mgs, [1] k_mgs_coop arrivals */
+    int mgs_grid, mgs_tm, mgs_base; /* cooperative MGS step: grid (0: off), register tile, flag base */
+    double *mpart;             /* its partial sums, (k_dim + 1) * mgs_grid */
+    int *mflag;
     unsigned long long *ticket;
     unsigned long long tbase;
     int *err;
@@ -859,6 +863,54 @@ static int upload_real(void **dst, const double *src, size_t n, int vt, void *st
     return rc;
 }
 
+/* storage type of the partitioned stage-2 blocks for stage2_precision p */
+static int stage2_vt(int p) { return p == 2 ? JXAMD_T_F16 : (p == 1 ? JXAMD_T_F32 : JXAMD_T_F64); }
+
+/* FP16 copy of the stage-2 SELL blocks v (slot-major, b*b planes of 64
+ * lanes): block row i is scaled by the smallest power of two at or above the
+ * largest magnitude of its off-diagonal blocks, so every stored entry is at
+ * most 1 in magnitude; the scales go to d->rsc16 */
+static int upload_blocks_f16(dbsr *d, const hsell *h, const double *v, int nb, int bb, void *st)
+{
+    const size_t slots = (size_t)(h->nsl ? h->sptr[h->nsl] : 0);
+    float *sc = (float *)malloc(sizeof(float) * (nb > 0 ? nb : 1));
+    uint16_t *t = (uint16_t *)calloc(slots * bb * SELLW + 1, sizeof(uint16_t));
+    if (!sc || !t) { free(sc); free(t); return JXAMD_ERROR_MEMORY; }
+    for (int i = 0; i < nb; i++) {
+        const int sl = i / SELLW, l = i % SELLW;
+        double mx = 0.0;
+        for (int k = h->sptr[sl]; k < h->sptr[sl + 1]; k++) {
+            if (h->col[(size_t)k * SELLW + l] == i) continue; /* the diagonal block and padding */
+            for (int e = 0; e < bb; e++) {
+                const double a = fabs(v[((size_t)k * bb + e) * SELLW + l]);
+                if (a > mx) mx = a;
+            }
+        }
+        int ex = 0;
+        if (mx > 0.0) {
+            double m = frexp(mx, &ex);
+            if (m == 0.5) ex -= 1;
+        }
+        const double scale = mx > 0.0 ? ldexp(1.0, ex) : 1.0;
+        sc[i] = (float)scale;
+        for (int k = h->sptr[sl]; k < h->sptr[sl + 1]; k++) {
+            if (h->col[(size_t)k * SELLW + l] == i) continue;
+            for (int e = 0; e < bb; e++) {
+                const size_t o = ((size_t)k * bb + e) * SELLW + l;
+                t[o] = half_bits((float)(v[o] / scale));
+            }
+        }
+    }
+    int rc = jxamd_dev_malloc(&d->val32, sizeof(uint16_t) * (slots * bb * SELLW + 1));
+    if (!rc) rc = jxamd_h2d(d->val32, t, sizeof(uint16_t) * slots * bb * SELLW, st);
+    if (!rc) rc = jxamd_dev_malloc((void **)&d->rsc16, sizeof(float) * (nb > 0 ? nb : 1));
+    if (!rc) rc = jxamd_h2d(d->rsc16, sc, sizeof(float) * nb, st);
+    if (!rc) rc = jxamd_stream_sync(st);
+    free(sc);
+    free(t);
+    return rc;
+}
+
 /* ---------------------------------------------------------------------- */
 /* create / destroy                                                        */
 /* ---------------------------------------------------------------------- */
@@ -896,7 +948,10 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         d->nsl = h.nsl;
         d->spmv_bytes = (double)A->nnzb * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 16.0 * nb * b;
         d->sweep_bytes = d->spmv_bytes + 16.0 * nb * b + 8.0 * nb * bb; /* + old x, rhs, Dinv */
-        d->sweep_bytes32 = (double)A->nnzb * (4.0 * bb + 4.0) + 4.0 * (nb + 1) + 32.0 * nb * b + 4.0 * nb * bb;
+        d->sweep_bytes32 = s->prm.stage2_precision == 2
+                               ? (double)A->nnzb * (2.0 * bb + 4.0) + 4.0 * (nb + 1) + 32.0 * nb * b + 4.0 * nb * bb +
+                                     4.0 * nb
+                               : (double)A->nnzb * (4.0 * bb + 4.0) + 4.0 * (nb + 1) + 32.0 * nb * b + 4.0 * nb * bb;
         int rc = jxamd_dev_malloc((void **)&d->sptr, sizeof(int) * (h.nsl + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->col, sizeof(int) * (slots * SELLW + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->val, sizeof(double) * (slots * bb * SELLW + 1));
@@ -905,6 +960,8 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         if (!rc) rc = jxamd_h2d(d->val, v, sizeof(double) * slots * bb * SELLW, s->stream);
         if (!rc && s->prm.stage2_chunk > 0 && s->prm.stage2_precision == 1)
             rc = upload_real(&d->val32, v, slots * bb * SELLW, JXAMD_T_F32, s->stream);
+        if (!rc && s->prm.stage2_chunk > 0 && s->prm.stage2_precision == 2)
+            rc = upload_blocks_f16(d, &h, v, nb, bb, s->stream);
         if (!rc) rc = jxamd_stream_sync(s->stream);
         free(v);
         hsell_free(&h);
@@ -980,7 +1037,7 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         if (!rc) rc = jxamd_h2d(d->gval, v, sizeof(double) * slots * bb * SELLW, s->stream);
         if (!rc) rc = upload_ints(&d->gperm, perm, (size_t)npos, s->stream);
         if (!rc) rc = jxamd_h2d(d->dinv, dinv, sizeof(double) * (size_t)nb * bb, s->stream);
-        if (!rc && s->prm.stage2_chunk > 0 && s->prm.stage2_precision == 1)
+        if (!rc && s->prm.stage2_chunk > 0 && s->prm.stage2_precision >= 1)
             rc = upload_real(&d->dinv32, dinv, (size_t)nb * bb, JXAMD_T_F32, s->stream);
         if (!rc) rc = upload_ints(&d->gdep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
         if (!rc) rc = upload_ints(&d->gdep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
@@ -1372,7 +1429,8 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->stage2_solver_type != 2 && p->stage2_solver_type != 5) return JXAMD_ERROR_ARG;
     if (p->poly_degree < 0 || p->poly_degree > 8) return JXAMD_ERROR_ARG;
     if (p->stage2_chunk < 0 || p->stage2_chunk > 8) return JXAMD_ERROR_ARG;
-    if (p->stage2_precision < 0 || p->stage2_precision > 1) return JXAMD_ERROR_ARG;
+    if (p->stage2_precision < 0 || p->stage2_precision > 2) return JXAMD_ERROR_ARG;
+    if (p->arnoldi_kernel < 0 || p->arnoldi_kernel > 1) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -1429,6 +1487,11 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_dev_malloc((void **)&s->Xd, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->dsc, sizeof(double) * (2 * (size_t)p->k_dim + 16));
     if (!e) e = jxamd_dev_malloc((void **)&s->part, sizeof(double) * 4096); /* >= jxamd_dot_grid() blocks */
+    if (!e && p->arnoldi_kernel == 0) e = jxamd_mgs_coop_plan(N, &s->mgs_grid, &s->mgs_tm);
+    if (!e && s->mgs_grid > 0) {
+        e = jxamd_dev_malloc((void **)&s->mpart, sizeof(double) * (size_t)(p->k_dim + 1) * s->mgs_grid);
+        if (!e) e = zero_ints(&s->mflag, 1, s->stream);
+    }
     if (!e) e = jxamd_dev_malloc((void **)&s->count, sizeof(unsigned int) * 4);
     if (!e) e = jxamd_dev_malloc((void **)&s->ticket, sizeof(unsigned long long));
     if (!e) e = jxamd_dev_malloc((void **)&s->err, sizeof(int) * 4);
@@ -1461,7 +1524,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(d->gval); jxamd_dev_free(d->dinv); jxamd_dev_free(d->flag);
     jxamd_dev_free(d->gdep_ptr); jxamd_dev_free(d->gdep_list);
     jxamd_dev_free(d->gall_ptr); jxamd_dev_free(d->gall_list); jxamd_dev_free(d->gvorder); jxamd_dev_free(d->xs);
-    jxamd_dev_free(d->val32); jxamd_dev_free(d->dinv32);
+    jxamd_dev_free(d->val32); jxamd_dev_free(d->dinv32); jxamd_dev_free(d->rsc16);
     for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
         dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]); dsell_free(&s->Gc[l]);
         jxamd_dev_free(s->wstep[l]);
@@ -1474,6 +1537,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(s->xtmp); jxamd_dev_free(s->V); jxamd_dev_free(s->Rv); jxamd_dev_free(s->Wv);
     jxamd_dev_free(s->Bd); jxamd_dev_free(s->Xd);
     jxamd_dev_free(s->dsc); jxamd_dev_free(s->part); jxamd_dev_free(s->count);
+    jxamd_dev_free(s->mpart); jxamd_dev_free(s->mflag);
     jxamd_dev_free(s->ticket); jxamd_dev_free(s->err);
     for (int q = 0; q < PROF_POOL; q++) {
         jxamd_event_destroy(s->prof.ev0[q]);
@@ -1701,11 +1765,12 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
         const double *src = NULL;
         for (int it = 0; it < m; it++) {
             int pq = prof_begin(s);
-            const int f32 = s->prm.stage2_precision == 1;
-            CK(jxamd_k_bhgs_part(b, f32 ? JXAMD_T_F32 : JXAMD_T_F64, s->nb, parts, d->sptr, d->col,
-                                 f32 ? d->val32 : (const void *)d->val, f32 ? d->dinv32 : (const void *)d->dinv, r, src,
-                                 xp, pi, dst, s->prm.stage2_weight, s->prm.stage2_omega, s->stream));
-            prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, f32 ? d->sweep_bytes32 : d->sweep_bytes);
+            const int nar = s->prm.stage2_precision; /* 0 FP64, 1 FP32, 2 FP16 blocks */
+            CK(jxamd_k_bhgs_part(b, stage2_vt(nar), s->nb, parts, d->sptr, d->col,
+                                 nar ? d->val32 : (const void *)d->val, d->rsc16,
+                                 nar ? d->dinv32 : (const void *)d->dinv, r, src, xp, pi, dst, s->prm.stage2_weight,
+                                 s->prm.stage2_omega, s->stream));
+            prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, nar ? d->sweep_bytes32 : d->sweep_bytes);
             src = dst;
             dst = (dst == x) ? s->xtmp : x;
         }
@@ -1956,10 +2021,11 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
     if (!s) return JXAMD_ERROR_ARG;
     if (s->prm.stage2_chunk > 0) {
         const int parts = (s->nb + s->prm.stage2_chunk - 1) / s->prm.stage2_chunk;
-        const int f32 = s->prm.stage2_precision == 1;
-        CK(jxamd_k_bhgs_part(s->b, f32 ? JXAMD_T_F32 : JXAMD_T_F64, s->nb, parts, s->A.sptr, s->A.col,
-                             f32 ? s->A.val32 : (const void *)s->A.val, f32 ? s->A.dinv32 : (const void *)s->A.dinv,
-                             d_b, d_x, NULL, 0, s->xtmp, w, omega, s->stream));
+        const int nar = s->prm.stage2_precision;
+        CK(jxamd_k_bhgs_part(s->b, stage2_vt(nar), s->nb, parts, s->A.sptr, s->A.col,
+                             nar ? s->A.val32 : (const void *)s->A.val, s->A.rsc16,
+                             nar ? s->A.dinv32 : (const void *)s->A.dinv, d_b, d_x, NULL, 0, s->xtmp, w, omega,
+                             s->stream));
     } else {
         CK(bhgs_run(s, 1, d_b, d_x, NULL, s->xtmp, w, omega));
     }
@@ -2105,11 +2171,19 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             TRY(precond_timed(s, VEC(i - 1), s->Rv));
             nprec++;
             TRY(bsr_apply(s, s->Rv, NULL, VEC(i), 0));
-            /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
-            TRY(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
-            for (j = 1; j < i; j++) TRY(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
-            TRY(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i));
-            {
+            if (s->mgs_grid > 0) {
+                /* the whole MGS step in one launch: v_0..v_{i-1} read once,
+                 * w = v_i read and written once */
+                int pq = prof_begin(s);
+                TRY(jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1,
+                                     s->mflag, s->mgs_base, s->err, s->stream));
+                prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
+                s->mgs_base += i + 1;
+            } else {
+                /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
+                TRY(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
+                for (j = 1; j < i; j++) TRY(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
+                TRY(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i));
                 int pq = prof_begin(s);
                 TRY(jxamd_k_scale(N, h + i, 0.0, VEC(i), s->stream));
                 prof_end(s, pq, JXAMD_PROF_VEC, 16.0 * N);

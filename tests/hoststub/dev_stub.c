@@ -90,6 +90,10 @@ int jxamd_k_axpy(size_t n, const double *pa, double a, int neg, const double *x,
 int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *stream)
 { (void)n; (void)pnorm2; (void)a; (void)y; (void)stream; return 0; }
 int jxamd_dot_grid(size_t n) { (void)n; return 1; }
+int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm) { *grid = n > 0 ? 2 : 0; *tm = 4; return 0; }
+int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, void *stream)
+{ (void)grid; (void)tm; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)stream; return 0; }
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream)
 { (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }
@@ -120,6 +124,6 @@ int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *i
 { (void)G; (void)vt; (void)xt; (void)n; (void)nchunks; (void)ia; (void)ja; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
 
 int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const void *dinv, const double *rhs, const double *xo, const double *xp, int pidx, double *xn,
-                      double w, double omega, void *stream)
-{ (void)b; (void)vt; (void)nb; (void)nparts; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }
+                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const double *xp,
+                      int pidx, double *xn, double w, double omega, void *stream)
+{ (void)b; (void)vt; (void)nb; (void)nparts; (void)sptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }
