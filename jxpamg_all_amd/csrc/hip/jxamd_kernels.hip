@@ -84,6 +84,10 @@ template <typename V> struct row_scaled { static constexpr bool value = false; }
 template <typename T> __device__ __forceinline__ T ldnt(const T *p) { return __builtin_nontemporal_load(p); }
 template <> __device__ __forceinline__ __half ldnt<__half>(const __half *p) { return *p; }
 template <> struct row_scaled<__half> { static constexpr bool value = true; };
+// LDS type for matrix blocks staged by the stage-2 kernel: values stored
+// narrower than double are exact in float
+template <typename V> struct lds_block { typedef float T; };
+template <> struct lds_block<double> { typedef double T; };
 
 // XCD-aware tile order for streaming kernels: consecutive tiles land on the
 // same XCD (dispatch is round-robin over the 8 XCDs), so the x entries a
@@ -527,16 +531,20 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
 {
     constexpr int NL = LMAX * (LMAX - 1) / 2; // strictly lower pairs of a partition
     constexpr int LST = NL * B * B + 1;       // odd stride between partitions
+    // in-partition lower blocks in LDS as LT: float for narrow block storage
+    // (exact), which halves the footprint and doubles the resident workgroups
+    typedef typename lds_block<V>::T LT;
     extern __shared__ __align__(16) unsigned char smraw[];
     double *sm = reinterpret_cast<double *>(smraw);
-    double *h = sm, *xold = sm + 256 * B, *Lb = sm + 2 * 256 * B;
+    double *h = sm, *xold = sm + 256 * B;
+    LT *Lb = reinterpret_cast<LT *>(sm + 2 * 256 * B);
     const int size = nb / nparts, rest = nb - size * nparts;
     const int c0 = xcd_tile(blockIdx.x, gridDim.x) * cpb;
     const int c1 = min(c0 + cpb, nparts);
     const int row0 = c0 < rest ? c0 * (size + 1) : c0 * size + rest;
     const int row1 = c1 < rest ? c1 * (size + 1) : c1 * size + rest;
     // pairs without a coupling stay zero
-    for (int e = threadIdx.x; e < cpb * LST; e += blockDim.x) Lb[e] = 0.0;
+    for (int e = threadIdx.x; e < cpb * LST; e += blockDim.x) Lb[e] = (LT)0;
     __syncthreads();
     const int r = threadIdx.x;
     const int i = row0 + r;
@@ -577,9 +585,9 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
                 const V *vv = val + (size_t)(k + u) * B * B * WAVE + lane;
                 const bool in = j >= ns && j < ns + len;
                 if (in && j < i) {
-                    double *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
+                    LT *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
 #pragma unroll
-                    for (int e = 0; e < B * B; e++) L[e] = sc * (double)cvt(ldnt(vv + e * WAVE));
+                    for (int e = 0; e < B * B; e++) L[e] = (LT)(sc * (double)cvt(ldnt(vv + e * WAVE)));
                 }
 #pragma unroll
                 for (int rr = 0; rr < B; rr++) {
@@ -607,7 +615,7 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
         const int ns = c < rest ? c * (size + 1) : c * size + rest;
         const int len = c < rest ? size + 1 : size;
         const int rl = ns - row0;
-        const double *Lc = Lb + (size_t)(c - c0) * LST;
+        const LT *Lc = Lb + (size_t)(c - c0) * LST;
         const double prod = 1.0 - w * omega;
         double xnew[LMAX][B];
 #pragma unroll
@@ -619,11 +627,11 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
 #pragma unroll
                 for (int u = 0; u < LMAX - 1; u++) {
                     if (u < t) {
-                        const double *L = Lc + ((t * (t - 1)) / 2 + u) * B * B;
+                        const LT *L = Lc + ((t * (t - 1)) / 2 + u) * B * B;
 #pragma unroll
                         for (int rr = 0; rr < B; rr++)
 #pragma unroll
-                            for (int q = 0; q < B; q++) tmp[rr] -= L[rr * B + q] * xnew[u][q];
+                            for (int q = 0; q < B; q++) tmp[rr] -= (double)L[rr * B + q] * xnew[u][q];
                     }
                 }
                 const DV *D = dinv + (size_t)(ns + t) * B * B;
@@ -1509,7 +1517,8 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
     do {                                                                                                               \
         const int cpb = 256 / LM;                                                                                      \
         const int grid = (nparts + cpb - 1) / cpb;                                                                     \
-        const size_t lds = sizeof(double) * (2 * 256 * BS + (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1));       \
+        const size_t lb = (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1);                                          \
+        const size_t lds = sizeof(double) * 2 * 256 * BS + (vt == JXAMD_T_F64 ? sizeof(double) : sizeof(float)) * lb; \
         if (vt == JXAMD_T_F64)                                                                                        \
             k_bhgs_part<BS, LM, double, double, 1><<<grid, 256, lds, st>>>(                                            \
                 nb, nparts, cpb, sptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo, xp, pidx, xn, w,  \
