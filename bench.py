@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-learned", action="store_true", help="skip the mixed-precision + learned-smoother leg")
-    ap.add_argument("--train-seeds", type=int, default=20, help="config-1-style systems labelled for training")
+    ap.add_argument("--train-seeds", type=int, default=20,
+                    help="small systems labelled for training the learned smoother (0: no training, no learned leg)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three plain device copies of MIB MiB (counter calibration) and exit")
     ap.add_argument("--stage2-chunk", type=int, default=None,
@@ -148,20 +149,25 @@ def main():
         # scaled, FP32 vectors) with a degree-2 polynomial smoother whose roots
         # the network predicts (trained here, on the GPU, on data the plain
         # solver labels itself; default Chebyshev roots where they do not win),
-        # one FP32-block stage-2 sweep
+        # one stage-2 sweep with FP16 blocks (row-scaled)
         from jxpamg_all_amd import learned as lrn
-        t0 = time.time()
-        Xtr, Ttr, _ = lrn.make_dataset(range(1001, 1001 + args.train_seeds), precision=2, m=2)
-        net = lrn.MLP(seed=0)
-        losses = net.fit(Xtr, Ttr, steps=3000, lr=3e-3)
-        t_train = time.time() - t0
-        log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
+        net, t_train = None, 0.0
+        if args.train_seeds > 0:
+            t0 = time.time()
+            Xtr, Ttr, _ = lrn.make_dataset(range(1001, 1001 + args.train_seeds), precision=2, m=2)
+            net = lrn.MLP(seed=0)
+            losses = net.fit(Xtr, Ttr, steps=3000, lr=3e-3)
+            t_train = time.time() - t0
+            log(f"learned smoother: {len(Xtr)} samples, loss {losses[0]:.4f} -> {losses[-1]:.5f}, {t_train:.1f}s")
         legs = {}
         stage2_ref = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
-        stage2_one = dict(stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
+        stage2_one = dict(stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=2)
         mixed_kw = dict(precond_precision=2, poly_degree=2, **stage2_one)
-        for name, kw in (("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_ref)),
-                         ("mixed_default_poly", mixed_kw), ("mixed_learned", mixed_kw)):
+        runs = [("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_ref)),
+                ("mixed_default_poly", mixed_kw)]
+        if net is not None:
+            runs.append(("mixed_learned", mixed_kw))
+        for name, kw in runs:
             pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, **kw)
             sm = jx.Solver(Ad, H, pm)
             used = None
@@ -176,7 +182,7 @@ def main():
                 legs[name]["levels_on_learned_steps"] = used
             log(f"{name}: {t_m * 1e3:.1f} ms iters {st_m.iterations} true_rel {st_m.true_rel_res:.3e}")
             sm.close()
-        legs["training_s"] = round(t_train, 2)
+        legs["training_s"] = round(t_train, 2) if net is not None else None
         mixed = legs
     # the plain path with the one-rank (sequential-order) stage 2, for reference
     pe = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_chunk=0)
@@ -202,7 +208,7 @@ def main():
                     "bhgs_sweep": "k_bhgs_part" if s2chunk else "k_bhgs_multi", "bsr_spmv": "k_bsr_spmv",
                     "csr_spmv": "k_csr_spmv/k_csr_spmv_grp (all levels)", "coarse_solve": "k_dense_gemv",
                     "mgs": "k_dot_mgs", "vector_ops": "k_scale/k_axpy/k_lincomb", "bilu_solve": "k_btri"}
-    traffic_key = {"bhgs_sweep": "k_bhgs_part<3, 4, double>" if s2chunk else "k_bhgs_multi<3>"}.get(dom, dom)
+    traffic_key = {"bhgs_sweep": "k_bhgs_part<3, 4, double, double, 1>" if s2chunk else "k_bhgs_multi<3>"}.get(dom, dom)
     traffic = read_traffic(traffic_key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
