@@ -33,6 +33,11 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
 #define JXAMD_T_F64 0
 #define JXAMD_T_F32 1
 #define JXAMD_T_F16 2
+/* or-ed into vt for the SELL-layout level kernels (csr_spmv, poly_step,
+ * gs_chunk_lds and the SELL-G row-group forms): col holds int16 offsets
+ * column - row packed two slots per int word, and sptr[nsl + 1 + s] is the
+ * first word of slice s */
+#define JXAMD_T_PK16 0x100
 int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
                      const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream);
 int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,

@@ -148,26 +148,44 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
     }
 }
 
-// Sum of val[e] * x[col[e] * xs] over slot rows [k, k1) of a SELL layout,
+// Sum of val[e] * x[j(e) * xs] over slot rows [k0, k1) of a SELL layout,
 // e = slot * 64 + lane, in slot order.  k1 is the same for the whole
 // wavefront, so each batch of SELL_U slots is predicated uniformly: the lane
 // issues all of a batch's index and value loads, then all of its gathers,
 // before it adds (a lane with one load chain in flight at a time leaves HBM
-// latency-bound).
-template <typename V, typename X>
-__device__ __forceinline__ X sell_dot(int k, int k1, int lane, const int *__restrict__ col,
+// latency-bound).  Column indices: col[e] (PK false) or, packed, two int16
+// offsets j - row per int word, word w0 + (slot - k0) / 2 of the lane (PK
+// true: half the index bytes; the words are loaded whole, beyond k1 too, so
+// the array carries a batch of padding words at its end).
+template <typename V, typename X, bool PK>
+__device__ __forceinline__ X sell_dot(int k0, int k1, int w0, int lane, int row, const int *__restrict__ col,
                                       const V *__restrict__ val, const X *__restrict__ x, int xs)
 {
     X acc = 0;
-    for (; k < k1; k += SELL_U) {
+    for (int k = k0; k < k1; k += SELL_U) {
         int j[SELL_U];
         V v[SELL_U];
+        if (PK) {
+            int wv[SELL_U / 2];
+            const int wb = w0 + (k - k0) / 2;
 #pragma unroll
-        for (int u = 0; u < SELL_U; u++) {
-            if (k + u < k1) {
-                const size_t e = (size_t)(k + u) * WAVE + lane;
-                j[u] = col[e];
-                v[u] = val[e];
+            for (int u = 0; u < SELL_U / 2; u++) wv[u] = col[(size_t)(wb + u) * WAVE + lane];
+#pragma unroll
+            for (int u = 0; u < SELL_U; u += 2) {
+                j[u] = row + (int)(short)(wv[u / 2] & 0xffff);
+                j[u + 1] = row + (wv[u / 2] >> 16);
+            }
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++)
+                if (k + u < k1) v[u] = val[(size_t)(k + u) * WAVE + lane];
+        } else {
+#pragma unroll
+            for (int u = 0; u < SELL_U; u++) {
+                if (k + u < k1) {
+                    const size_t e = (size_t)(k + u) * WAVE + lane;
+                    j[u] = col[e];
+                    v[u] = val[e];
+                }
             }
         }
         X xv[SELL_U];
@@ -186,20 +204,36 @@ __device__ __forceinline__ X sell_dot(int k, int k1, int lane, const int *__rest
 // lo, entries of in-chunk columns other than i go to the row's LDS block
 // Brow, and the others fold into acc -= a_ij uo_j.  Batches of U slots,
 // predicated per lane (a block's rows may lie in two slices).
-template <int U, typename V, typename X>
-__device__ __forceinline__ void chunk_row_gather(int k, int k1, size_t lo, int i, int ns, int len, X sc,
+template <int U, typename V, typename X, bool PK>
+__device__ __forceinline__ void chunk_row_gather(int k0, int k1, int w0, size_t lo, int i, int ns, int len, X sc,
                                                  const int *__restrict__ col, const V *__restrict__ val,
                                                  const X *__restrict__ uo, X *Brow, X &acc)
 {
-    for (; k < k1; k += U) {
+    for (int k = k0; k < k1; k += U) {
         int j[U];
         V v[U];
+        if (PK) {
+            // packed int16 offsets, as in sell_dot (whole words, padded array)
+            int wv[U / 2];
+            const int wb = w0 + (k - k0) / 2;
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (k + u < k1) {
-                const size_t e = (size_t)(k + u) * WAVE + lo;
-                j[u] = col[e];
-                v[u] = val[e];
+            for (int u = 0; u < U / 2; u++) wv[u] = col[(size_t)(wb + u) * WAVE + lo];
+#pragma unroll
+            for (int u = 0; u < U; u += 2) {
+                j[u] = i + (int)(short)(wv[u / 2] & 0xffff);
+                j[u + 1] = i + (wv[u / 2] >> 16);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (k + u < k1) v[u] = val[(size_t)(k + u) * WAVE + lo];
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (k + u < k1) {
+                    const size_t e = (size_t)(k + u) * WAVE + lo;
+                    j[u] = col[e];
+                    v[u] = val[e];
+                }
             }
         }
         X xv[U];
@@ -261,7 +295,7 @@ __device__ __forceinline__ void csr_row_gather(int k, int ke, int i, int ns, int
 // x and rhs may be strided views (xs, rs) so the pressure block of a BSR
 // vector is read in place.
 // ---------------------------------------------------------------------------
-template <typename V, typename X>
+template <typename V, typename X, bool PK>
 __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                   const V *__restrict__ val, const float *__restrict__ rsc,
                                                   const X *__restrict__ x, int xs, const X *__restrict__ rhs, int rs,
@@ -272,7 +306,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
     const int nsl = (n + WAVE - 1) / WAVE;
     if (s >= nsl) return;
     const int row = s * WAVE + lane;
-    X acc = sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, x, xs);
+    X acc = sell_dot<V, X, PK>(sptr[s], sptr[s + 1], PK ? sptr[nsl + 1 + s] : 0, lane, row, col, val, x, xs);
     if (row >= n) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
     if (mode == 0) y[row] = acc;
@@ -283,7 +317,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
 // One step of the polynomial smoother on a level (SELL-64, full rows):
 //   un_i = uo_i + omega * dinv_i * (f_i - sum_j a_ij uo_j)
 // uo == nullptr stands for zeros.  dinv_i = 0 keeps a row at its old value.
-template <typename V, typename X>
+template <typename V, typename X, bool PK>
 __global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                    const V *__restrict__ val, const float *__restrict__ rsc,
                                                    const X *__restrict__ dinv, const X *__restrict__ f, int fs,
@@ -294,7 +328,8 @@ __global__ void __launch_bounds__(256) k_poly_step(int n, const int *__restrict_
     const int nsl = (n + WAVE - 1) / WAVE;
     if (s >= nsl) return;
     const int row = s * WAVE + lane;
-    X acc = uo ? sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, uo, 1) : (X)0;
+    X acc = uo ? sell_dot<V, X, PK>(sptr[s], sptr[s + 1], PK ? sptr[nsl + 1 + s] : 0, lane, row, col, val, uo, 1)
+               : (X)0;
     if (row >= n) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
     const X r = f[(size_t)row * fs] - acc;
@@ -826,7 +861,7 @@ __global__ void __launch_bounds__(256) k_gs_chunk(int n, int nchunks, const int 
 // Block b owns chunks [b*cpb, (b+1)*cpb), at most 256 rows.  Rows with a zero
 // diagonal keep their entry value.
 // ---------------------------------------------------------------------------
-template <int LMAX, typename V, typename X>
+template <int LMAX, typename V, typename X, bool PK>
 __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cpb, const int *__restrict__ sptr,
                                                       const int *__restrict__ col, const V *__restrict__ val,
                                                       const float *__restrict__ rsc, const X *__restrict__ diag,
@@ -854,7 +889,9 @@ __global__ void __launch_bounds__(256) k_gs_chunk_lds(int n, int nchunks, int cp
         X acc = f[(size_t)i * fs];
         const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
         const int s = i / WAVE, lane = i % WAVE;
-        chunk_row_gather<SELL_U, V, X>(sptr[s], sptr[s + 1], (size_t)lane, i, ns, len, sc, col, val, uo, Brow, acc);
+        const int nsl = (n + WAVE - 1) / WAVE;
+        chunk_row_gather<SELL_U, V, X, PK>(sptr[s], sptr[s + 1], PK ? sptr[nsl + 1 + s] : 0, (size_t)lane, i, ns, len,
+                                           sc, col, val, uo, Brow, acc);
         dd[r] = diag[i];
         g[r] = acc;
         ux[r] = uo ? uo[i] : (X)0;
@@ -911,7 +948,7 @@ template <int G, typename X> __device__ __forceinline__ X group_sum(X v)
 }
 
 //   mode 0: y = A x;  1: y = rhs - A x;  2: y += A x
-template <int G, typename V, typename X>
+template <int G, typename V, typename X, bool PK>
 __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                       const V *__restrict__ val, const float *__restrict__ rsc,
                                                       const X *__restrict__ x, int xs, const X *__restrict__ rhs,
@@ -923,7 +960,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restri
     if (s >= nsl) return; // whole wavefront
     const int lane = threadIdx.x % WAVE, gl = lane % G;
     const int row = s * R + lane / G;
-    X acc = sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, x, xs);
+    X acc = sell_dot<V, X, PK>(sptr[s], sptr[s + 1], PK ? sptr[nsl + 1 + s] : 0, lane, row, col, val, x, xs);
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -932,7 +969,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restri
     else y[row] += acc;
 }
 
-template <int G, typename V, typename X>
+template <int G, typename V, typename X, bool PK>
 __global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                        const V *__restrict__ val, const float *__restrict__ rsc,
                                                        const X *__restrict__ dinv, const X *__restrict__ f, int fs,
@@ -944,7 +981,8 @@ __global__ void __launch_bounds__(256) k_poly_step_grp(int n, const int *__restr
     if (s >= nsl) return;
     const int lane = threadIdx.x % WAVE, gl = lane % G;
     const int row = s * R + lane / G;
-    X acc = uo ? sell_dot<V, X>(sptr[s], sptr[s + 1], lane, col, val, uo, 1) : (X)0;
+    X acc = uo ? sell_dot<V, X, PK>(sptr[s], sptr[s + 1], PK ? sptr[nsl + 1 + s] : 0, lane, row, col, val, uo, 1)
+               : (X)0;
     acc = group_sum<G>(acc);
     if (row >= n || gl != 0) return;
     if (row_scaled<V>::value) acc *= (X)rsc[row];
@@ -1409,6 +1447,19 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
 }
 
 // (vt, xt) dispatch: F64/F64, F32/F32, F16/F32
+// level-matrix kernels over SELL layouts: vt may carry JXAMD_T_PK16 (packed
+// int16 column offsets; slice word offsets after sptr)
+#define DISPATCH_VXP(vt, xt, CALL)                                                                                     \
+    do {                                                                                                               \
+        if ((vt) & JXAMD_T_PK16) {                                                                                     \
+            constexpr bool P_ = true;                                                                                  \
+            DISPATCH_VX((vt) & ~JXAMD_T_PK16, xt, CALL);                                                               \
+        } else {                                                                                                       \
+            constexpr bool P_ = false;                                                                                 \
+            DISPATCH_VX(vt, xt, CALL);                                                                                 \
+        }                                                                                                              \
+    } while (0)
+
 #define DISPATCH_VX(vt, xt, CALL)                                                                                      \
     do {                                                                                                               \
         if ((vt) == JXAMD_T_F64 && (xt) == JXAMD_T_F64) { typedef double V_; typedef double X_; CALL; }                \
@@ -1423,9 +1474,9 @@ int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, con
     const int nsl = (n + WAVE - 1) / WAVE;
     const int grid = (nsl + 3) / 4;
     if (grid == 0) return 0;
-    DISPATCH_VX(vt, xt,
-                (k_csr_spmv<V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
-                    n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y, mode)));
+    DISPATCH_VXP(vt, xt,
+                 (k_csr_spmv<V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                     n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y, mode)));
     return chk(hipGetLastError());
 }
 
@@ -1435,10 +1486,10 @@ int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, co
     const int nsl = (n + WAVE - 1) / WAVE;
     const int grid = (nsl + 3) / 4;
     if (grid == 0) return 0;
-    DISPATCH_VX(vt, xt,
-                (k_poly_step<V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
-                    n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs, (const X_ *)uo,
-                    (X_ *)un, (X_)omega)));
+    DISPATCH_VXP(vt, xt,
+                 (k_poly_step<V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                     n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs, (const X_ *)uo,
+                     (X_ *)un, (X_)omega)));
     return chk(hipGetLastError());
 }
 
@@ -1613,10 +1664,11 @@ int jxamd_k_gs_chunk_lds(int vt, int xt, int n, int nchunks, const int *sptr, co
     const int grid = (nchunks + cpb - 1) / cpb;
     hipStream_t st = (hipStream_t)stream;
 #define LAUNCH_LDS(LM)                                                                                                 \
-    DISPATCH_VX(vt, xt,                                                                                                \
-                (k_gs_chunk_lds<LM, V_, X_><<<grid, 256, sizeof(X_) * (768 + (size_t)cpb * (LM * LM + 1)), st>>>(    \
-                    n, nchunks, cpb, sptr, col, (const V_ *)val, rsc, (const X_ *)diag, (const X_ *)f, fs,            \
-                    (const X_ *)uo, (X_ *)un)))
+    DISPATCH_VXP(vt, xt,                                                                                               \
+                 (k_gs_chunk_lds<LM, V_, X_, P_><<<grid, 256, sizeof(X_) * (768 + (size_t)cpb * (LM * LM + 1)),      \
+                                                   st>>>(n, nchunks, cpb, sptr, col, (const V_ *)val, rsc,             \
+                                                         (const X_ *)diag, (const X_ *)f, fs, (const X_ *)uo,          \
+                                                         (X_ *)un)))
     if (maxlen <= 4) LAUNCH_LDS(4);
     else if (maxlen <= 8) LAUNCH_LDS(8);
     else LAUNCH_LDS(16);
@@ -1648,10 +1700,10 @@ int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *sptr, const in
     if (n == 0) return 0;
     if (G < 4 || G > 32) return JXAMD_ERROR_ARG;
     const int grid = grp_grid(n, G);
-    DISPATCH_G(G, DISPATCH_VX(vt, xt,
-                              (k_csr_spmv_grp<G_, V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
-                                  n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y,
-                                  mode))));
+    DISPATCH_G(G, DISPATCH_VXP(vt, xt,
+                               (k_csr_spmv_grp<G_, V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                                   n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs,
+                                   (X_ *)y, mode))));
     return chk(hipGetLastError());
 }
 
@@ -1662,10 +1714,10 @@ int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *sptr, const i
     if (n == 0) return 0;
     if (G < 4 || G > 32) return JXAMD_ERROR_ARG;
     const int grid = grp_grid(n, G);
-    DISPATCH_G(G, DISPATCH_VX(vt, xt,
-                              (k_poly_step_grp<G_, V_, X_><<<grid, 256, 0, (hipStream_t)stream>>>(
-                                  n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs,
-                                  (const X_ *)uo, (X_ *)un, (X_)omega))));
+    DISPATCH_G(G, DISPATCH_VXP(vt, xt,
+                               (k_poly_step_grp<G_, V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                                   n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs,
+                                   (const X_ *)uo, (X_ *)un, (X_)omega))));
     return chk(hipGetLastError());
 }
 

@@ -42,6 +42,7 @@ typedef struct {
     int vt, xt;
     int G;                   /* > 1: G lanes per row over SELL-G (sellg) or plain CSR (sptr = ia, col = ja) */
     int sellg;
+    int pk16;                /* col: int16 offsets column - row, two slots per word; word offsets after sptr */
     double abytes;           /* algorithmic HBM bytes of one launch over it */
 } dsell;
 
@@ -447,6 +448,68 @@ static int upload_ints(int **dst, const int *src, size_t n, void *st)
     return jxamd_h2d(*dst, src, sizeof(int) * n, st);
 }
 
+/* Packed int16 column offsets of a square level matrix in a slice layout
+ * (slices of R rows, G lanes per row; R = 64, G = 1 for SELL-64 in natural
+ * order): lane ln of slot k of slice q belongs to row q*R + ln/G.  Slice q's
+ * slots [sp[q], sp[q+1]) take ceil(ns/2) word rows from wp[q]; word t of a
+ * lane holds the offsets of slots 2t (low half) and 2t+1 (high half).
+ * Zero entries (padding) get offset 0; lanes of rows past n point at row
+ * n - 1.  PK16_PAD word rows of zeros follow (the kernels load a batch's
+ * words whole).  Returns 1 with *words (nw rows of 64) and *wp when every
+ * offset fits in int16, else 0. */
+#define PK16_PAD 8
+static int pack16_slices(const int *col, const double *val, const int *sp, int nsl, int R, int G, int n,
+                         int32_t **words, int **wp, size_t *nw)
+{
+    int *w = (int *)malloc(sizeof(int) * ((size_t)nsl + 1));
+    if (!w) return 0;
+    w[0] = 0;
+    for (int q = 0; q < nsl; q++) w[q + 1] = w[q] + (sp[q + 1] - sp[q] + 1) / 2;
+    const size_t rows = (size_t)w[nsl] + PK16_PAD;
+    int32_t *o = (int32_t *)calloc(rows * SELLW, sizeof(int32_t));
+    if (!o) { free(w); return 0; }
+    for (int q = 0; q < nsl; q++)
+        for (int k = sp[q]; k < sp[q + 1]; k++)
+            for (int ln = 0; ln < SELLW; ln++) {
+                const size_t e = (size_t)k * SELLW + ln;
+                const int row = q * R + ln / G;
+                long off;
+                if (row >= n) off = (long)(n - 1) - row;
+                else if (val[e] == 0.0) off = 0;
+                else off = (long)col[e] - row;
+                if (off < -32768 || off > 32767) { free(o); free(w); return 0; }
+                const int t = k - sp[q];
+                const size_t we = ((size_t)w[q] + t / 2) * SELLW + ln;
+                const uint32_t h = (uint32_t)(uint16_t)(int16_t)off;
+                o[we] = (int32_t)((uint32_t)o[we] | (t % 2 ? h << 16 : h));
+            }
+    *words = o;
+    *wp = w;
+    *nw = rows;
+    return 1;
+}
+
+/* replace d->sptr / d->col by [sptr | word offsets] and the packed words */
+static int upload_pk16(dsell *d, const int *sp, int nsl, const int32_t *o, const int *wp, size_t rows, void *st)
+{
+    int *sp2 = (int *)malloc(sizeof(int) * 2 * ((size_t)nsl + 1));
+    if (!sp2) return JXAMD_ERROR_MEMORY;
+    memcpy(sp2, sp, sizeof(int) * ((size_t)nsl + 1));
+    memcpy(sp2 + nsl + 1, wp, sizeof(int) * ((size_t)nsl + 1));
+    int *dsp = NULL, *dcol = NULL;
+    int rc = upload_ints(&dsp, sp2, 2 * ((size_t)nsl + 1), st);
+    if (!rc) rc = upload_ints(&dcol, (const int *)o, rows * SELLW, st);
+    if (!rc) rc = jxamd_stream_sync(st);
+    free(sp2);
+    if (rc) { jxamd_dev_free(dsp); jxamd_dev_free(dcol); return rc; }
+    jxamd_dev_free(d->sptr);
+    jxamd_dev_free(d->col);
+    d->sptr = dsp;
+    d->col = dcol;
+    d->pk16 = 1;
+    return 0;
+}
+
 static int zero_ints(int **dst, size_t n, void *st)
 {
     int *z = (int *)calloc(n ? n : 1, sizeof(int));
@@ -489,16 +552,20 @@ static size_t type_size(int t) { return t == JXAMD_T_F64 ? 8 : (t == JXAMD_T_F32
 
 /* algorithmic bytes of one pass over a level matrix: values (+ F16 row
  * scales), indices, row pointer, plus nvec vector entries per row */
+/* storage code of a level matrix for its kernels (with the packed-index flag) */
+static int lvt(const dsell *d) { return d->vt | (d->pk16 ? JXAMD_T_PK16 : 0); }
+
 static double level_bytes(const dsell *d, long nnz, int n, double nvec)
 {
-    double b = (double)nnz * (type_size(d->vt) + 4.0) + 4.0 * (n + 1) + nvec * type_size(d->xt) * n;
+    double b = (double)nnz * (type_size(d->vt) + (d->pk16 ? 2.0 : 4.0)) + 4.0 * (n + 1) + nvec * type_size(d->xt) * n;
     if (d->vt == JXAMD_T_F16) b += 4.0 * n;
     return b;
 }
 
 /* Upload a host SELL layout, values converted to storage type vt and the
  * diagonal to vector type xt. */
-static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsched *sched, int vt, int xt)
+static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsched *sched, int vt, int xt,
+                        int pack)
 {
     memset(d, 0, sizeof(*d));
     d->n = h->n;
@@ -554,6 +621,17 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsc
         free(sc);
         CK(rc);
     }
+    if (pack && !h->perm && ncols == h->n) {
+        int32_t *o = NULL;
+        int *wp = NULL;
+        size_t rows = 0;
+        if (pack16_slices(h->col, h->val, h->sptr, h->nsl, SELLW, 1, h->n, &o, &wp, &rows)) {
+            const int rc = upload_pk16(d, h->sptr, h->nsl, o, wp, rows, st);
+            free(o);
+            free(wp);
+            CK(rc);
+        }
+    }
     if (h->perm) {
         CK(jxamd_dev_malloc((void **)&d->perm, sizeof(int) * (h->n ? h->n : 1)));
         CK(jxamd_h2d(d->perm, h->perm, sizeof(int) * h->n, st));
@@ -602,7 +680,8 @@ static int choose_group(long nnz, int n)
  * F16 rows are scaled by the smallest power of two at or above their largest
  * off-diagonal magnitude (the whole row if it has no off-diagonal).
  * Used for the residual, restriction, prolongation and polynomial-smoother
- * operators. */
+ * operators.  Square matrices whose couplings lie within 32767 rows of the
+ * diagonal get packed int16 column offsets (pack16_slices). */
 static int dsellg_upload(dsell *d, int n, int ncols, const int *ia, const int *ja, const double *a, int vt, int xt,
                          int G, void *st)
 {
@@ -679,6 +758,12 @@ static int dsellg_upload(dsell *d, int n, int ncols, const int *ia, const int *j
         if (!rc) rc = jxamd_h2d(d->rsc, sc, sizeof(float) * n, st);
     }
     if (!rc) rc = jxamd_stream_sync(st);
+    int32_t *o = NULL;
+    int *wp = NULL;
+    size_t rows = 0;
+    if (!rc && n == ncols && pack16_slices(cl, vl, sp, nsl, R, G, n, &o, &wp, &rows))
+        rc = upload_pk16(d, sp, nsl, o, wp, rows, st);
+    free(o); free(wp);
     free(sp); free(cl); free(vl); free(sc); free(tmp);
     CK(rc);
     return 0;
@@ -1174,7 +1259,7 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
             memset(&h, 0, sizeof(h));
         } else {
             e = hsell_build(&h, Pm->n, Pm->ia, Pm->ja, Pm->a, NULL, 0, 0);
-            if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL, s->vt, s->xt);
+            if (!e) e = dsell_upload(&s->P[l], &h, Pm->ncols, s->stream, NULL, s->vt, s->xt, 0);
         }
         /* y += P x: x once, y read and written */
         s->P[l].abytes = level_bytes(&s->P[l], Pm->ia[Pm->n], Pm->n, 2.0) + (double)type_size(s->xt) * Pm->ncols;
@@ -1190,7 +1275,7 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
             memset(&h, 0, sizeof(h));
         } else {
             e = hsell_build(&h, Pm->ncols, ri, rj, ra, NULL, 0, 0);
-            if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL, s->vt, s->xt);
+            if (!e) e = dsell_upload(&s->R[l], &h, Pm->n, s->stream, NULL, s->vt, s->xt, 0);
         }
         s->R[l].abytes = level_bytes(&s->R[l], Pm->ia[Pm->n], Pm->ncols, 1.0) + (double)type_size(s->xt) * Pm->n;
         if (!e) e = jxamd_stream_sync(s->stream);
@@ -1333,7 +1418,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         CK(dsellg_upload(&s->Ares[l], n, n, A->ia, A->ja, A->a, s->vt, s->xt, GA, s->stream));
     } else {
         e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 0, 0);
-        if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL, s->vt, s->xt);
+        if (!e) e = dsell_upload(&s->Ares[l], &h, n, s->stream, NULL, s->vt, s->xt, 1);
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
         CK(e);
@@ -1355,7 +1440,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
             CK(dcsr_upload(&s->Gc[l], n, n, A->ia, A->ja, A->a, s->vt, s->xt, GA, 1, s->stream));
         } else {
             e = hsell_build(&h, n, A->ia, A->ja, A->a, NULL, 1, 1);
-            if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt);
+            if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt, 1);
             if (!e) e = jxamd_stream_sync(s->stream);
             hsell_free(&h);
             CK(e);
@@ -1388,7 +1473,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
                     perm[(size_t)(wstep[w] + t) * SELLW + ln] = row;
                 }
         e = hsell_build(&h, npos, A->ia, A->ja, A->a, perm, 1, 1);
-        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt);
+        if (!e) e = dsell_upload(&s->Gc[l], &h, n, s->stream, NULL, s->vt, s->xt, 0);
         if (!e) e = upload_ints(&s->wstep[l], wstep, (size_t)waves + 1, s->stream);
         if (!e) e = jxamd_stream_sync(s->stream);
         hsell_free(&h);
@@ -1405,7 +1490,7 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
             int *perm = sc.perm;
             sc.perm = NULL; /* owned by h below */
             e = hsell_build(&h, sc.npos, A->ia, A->ja, A->a, perm, 1, 1);
-            if (!e) e = dsell_upload(dir > 0 ? &s->Gf[l] : &s->Gb[l], &h, n, s->stream, &sc, s->vt, s->xt);
+            if (!e) e = dsell_upload(dir > 0 ? &s->Gf[l] : &s->Gb[l], &h, n, s->stream, &sc, s->vt, s->xt, 0);
             hsell_free(&h);
             hsched_free(&sc);
             CK(e);
@@ -1592,7 +1677,7 @@ static int hsgs_chunk(jxamd_solver *s, int l, const void *f, int fs, const void 
         CK(jxamd_k_gs_chunk_grp(g->G, g->vt, g->xt, s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->rsc,
                                 g->diag, f, fs, uo, un, s->stream));
     else
-        CK(jxamd_k_gs_chunk_lds(g->vt, g->xt, s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->rsc, g->diag, f,
+        CK(jxamd_k_gs_chunk_lds(lvt(g), g->xt, s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->rsc, g->diag, f,
                                 fs, uo, un, s->stream));
     prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
     return 0;
@@ -1609,10 +1694,10 @@ static int poly_apply(jxamd_solver *s, int l, const void *f, int fs, void *u, in
         void *dst = (k == m - 1 && m > 1) ? u : ((k % 2 == 0) ? s->T[l] : s->Vt[l]);
         int pq = prof_begin(s);
         if (A->G > 1)
-            CK(jxamd_k_poly_step_grp(A->G, A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs,
+            CK(jxamd_k_poly_step_grp(A->G, lvt(A), A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs,
                                      src, dst, s->poly_w[l][k], s->stream));
         else
-            CK(jxamd_k_poly_step(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs, src, dst,
+            CK(jxamd_k_poly_step(lvt(A), A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs, src, dst,
                                  s->poly_w[l][k], s->stream));
         /* a step from zeros reads only f and dinv */
         prof_end(s, pq, JXAMD_PROF_GS_SWEEP, src ? A->abytes : 3.0 * (double)s->xsz * A->n);
@@ -1650,10 +1735,10 @@ static int csr_launch(jxamd_solver *s, const dsell *A, const void *x, const void
 {
     int pq = prof_begin(s);
     if (A->G > 1)
-        CK(jxamd_k_csr_spmv_grp(A->G, A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode,
+        CK(jxamd_k_csr_spmv_grp(A->G, lvt(A), A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode,
                                 s->stream));
     else
-        CK(jxamd_k_csr_spmv(A->vt, A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode,
+        CK(jxamd_k_csr_spmv(lvt(A), A->xt, A->n, A->sptr, A->col, A->val, A->rsc, x, 1, rhs, rs, y, mode,
                             s->stream));
     prof_end(s, pq, JXAMD_PROF_CSR_SPMV, A->abytes);
     return 0;
