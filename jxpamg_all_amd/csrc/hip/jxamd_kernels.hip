@@ -1729,9 +1729,10 @@ int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *i
     const int maxlen = jxamd_gs_chunk_lds_rows(n, nchunks);
     if (maxlen < 1 || maxlen > JXAMD_CHUNK_LDS_MAX) return JXAMD_ERROR_ARG;
     if (G < 1 || G > 32) return JXAMD_ERROR_ARG;
-    // about two rows per lane group per block: long rows make few rows a
-    // block's worth of work, and many blocks keep every CU busy
-    const int cpb = max(1, (2 * 256 / G) / maxlen);
+    // one row per lane group per block (two or four were slower on the
+    // 60x220x85 hierarchy): long rows make few rows a block's worth of work,
+    // and many blocks keep every CU busy
+    const int cpb = max(1, (256 / G) / maxlen);
     const int grid = (nchunks + cpb - 1) / cpb;
     hipStream_t st = (hipStream_t)stream;
 #define LAUNCH_GRP(LM)                                                                                                 \
