@@ -197,7 +197,8 @@ typedef struct {
     double true_rel_res;         /* ||b - A x|| / ||b|| recomputed at exit, FP64 */
     double b_norm;
     double t_solve_ms;           /* device time of the whole solve (HIP events) */
-    double t_precond_ms;         /* device time inside CPR applications */
+    double t_precond_ms;         /* device time inside CPR applications, measured when print_level > 0
+                                  * or profiling is on (each application then ends with a sync); else 0 */
 } jxamd_stats;
 
 typedef struct jxamd_solver_s jxamd_solver;

@@ -23,6 +23,11 @@ int jxamd_event_create(void **e);
 int jxamd_event_destroy(void *e);
 int jxamd_event_record(void *e, void *s);
 int jxamd_event_elapsed(float *ms, void *e0, void *e1);
+/* events without timing (completion only), waiting on one, pinned host memory */
+int jxamd_event_create_sync(void **e);
+int jxamd_event_sync(void *e);
+int jxamd_host_alloc(void **p, size_t bytes);
+int jxamd_host_free(void *p);
 int jxamd_last_error(void);
 
 int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,

@@ -1415,6 +1415,16 @@ int jxamd_stream_create(void **s)
 }
 int jxamd_stream_destroy(void *s) { return s ? chk(hipStreamDestroy((hipStream_t)s)) : 0; }
 int jxamd_stream_sync(void *s) { return chk(hipStreamSynchronize((hipStream_t)s)); }
+int jxamd_event_create_sync(void **e)
+{
+    hipEvent_t ev;
+    int r = chk(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    *e = (void *)ev;
+    return r;
+}
+int jxamd_event_sync(void *e) { return chk(hipEventSynchronize((hipEvent_t)e)); }
+int jxamd_host_alloc(void **p, size_t bytes) { return chk(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault)); }
+int jxamd_host_free(void *p) { return p ? chk(hipHostFree(p)) : 0; }
 int jxamd_event_create(void **e)
 {
     hipEvent_t ev;

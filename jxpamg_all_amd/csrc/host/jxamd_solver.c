@@ -84,6 +84,8 @@ struct jxamd_solver_s {
     jxamd_params prm;
     void *stream;
     void *ev0, *ev1, *ev2, *ev3;
+    void *evh[2];          /* Arnoldi columns copied to hpin[slot] (completion events) */
+    double *hpin;          /* pinned host memory: 2 x (k_dim + 2) doubles */
     int nb, b;
     size_t N;
     dbsr A;
@@ -1549,6 +1551,9 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_event_create(&s->ev1);
     if (!e) e = jxamd_event_create(&s->ev2);
     if (!e) e = jxamd_event_create(&s->ev3);
+    if (!e) e = jxamd_event_create_sync(&s->evh[0]);
+    if (!e) e = jxamd_event_create_sync(&s->evh[1]);
+    if (!e) e = jxamd_host_alloc((void **)&s->hpin, sizeof(double) * 2 * ((size_t)p->k_dim + 2));
     if (!e) e = build_bsr(s, A);
     if (!e && p->stage2_solver_type == 5) e = build_bilu(s, A);
     for (int l = 0; l < s->L && !e; l++) {
@@ -1630,6 +1635,8 @@ int jxamd_solver_destroy(jxamd_solver *s)
     }
     jxamd_event_destroy(s->ev0); jxamd_event_destroy(s->ev1);
     jxamd_event_destroy(s->ev2); jxamd_event_destroy(s->ev3);
+    jxamd_event_destroy(s->evh[0]); jxamd_event_destroy(s->evh[1]);
+    jxamd_host_free(s->hpin);
     jxamd_stream_destroy(s->stream);
     free(s);
     return 0;
@@ -2179,11 +2186,13 @@ static int host_norm(jxamd_solver *s, const double *v, double *nrm)
     return 0;
 }
 
-static int precond_timed(jxamd_solver *s, const double *r, double *x)
+/* CPR application, timed with events when the solve is timed (print_level
+ * > 0 or profiling: each timed application is followed by a stream sync) */
+static int precond_timed(jxamd_solver *s, const double *r, double *x, int timed)
 {
-    CK(jxamd_event_record(s->ev2, s->stream));
+    if (timed) CK(jxamd_event_record(s->ev2, s->stream));
     CK(cpr_apply(s, r, x));
-    CK(jxamd_event_record(s->ev3, s->stream));
+    if (timed) CK(jxamd_event_record(s->ev3, s->stream));
     return 0;
 }
 
@@ -2196,6 +2205,39 @@ static int accumulate_precond_time(jxamd_solver *s)
     return 0;
 }
 
+/* Enqueue Arnoldi step i: v_i = A M^{-1} v_{i-1}, MGS against v_0..v_{i-1}
+ * and normalisation (column in h on the device: projections, then the
+ * squared norm), then the column's copy to hpin[slot] and its completion
+ * event.  Nothing here waits for the device. */
+static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
+{
+    const size_t N = s->N;
+    double *V = s->V, *h = s->dsc;
+#define VEC(j) (V + (size_t)(j) * N)
+    CK(precond_timed(s, VEC(i - 1), s->Rv, timed));
+    CK(bsr_apply(s, s->Rv, NULL, VEC(i), 0));
+    if (s->mgs_grid > 0) {
+        /* the whole MGS step in one launch: v_0..v_{i-1} read once,
+         * w = v_i read and written once */
+        int pq = prof_begin(s);
+        CK(jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1, s->mflag,
+                            s->mgs_base, s->err, s->stream));
+        prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
+        s->mgs_base += i + 1;
+    } else {
+        /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
+        CK(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
+        for (int j = 1; j < i; j++) CK(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
+        CK(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i));
+        int pq = prof_begin(s);
+        CK(jxamd_k_scale(N, h + i, 0.0, VEC(i), s->stream));
+        prof_end(s, pq, JXAMD_PROF_VEC, 16.0 * N);
+    }
+#undef VEC
+    CK(jxamd_d2h(s->hpin + (size_t)slot * (s->prm.k_dim + 2), h, sizeof(double) * (i + 1), s->stream));
+    return jxamd_event_record(s->evh[slot], s->stream);
+}
+
 int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, double *res_history, jxamd_stats *st)
 {
     if (!s || !bvec || !x) return JXAMD_ERROR_ARG;
@@ -2204,7 +2246,6 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     const double r_tol = s->prm.tol, a_tol = s->prm.a_tol, epsmac = 1.e-16, convfac_threshold = 1.0e4;
     double *V = s->V;
 #define VEC(j) (V + (size_t)(j) * N)
-    double *h = s->dsc;              /* MGS column, k_dim+2 */
     double *ydev = s->dsc + k_dim + 4; /* back-solved coefficients, k_dim+1 */
     double *rs = (double *)calloc(k_dim + 1, sizeof(double));
     double *c = (double *)calloc(k_dim, sizeof(double));
@@ -2217,6 +2258,8 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     int i = 0, j, k, iter = 0;
     double t, gamma, r_norm = 0.0, b_norm = 0.0, den_norm, epsilon, r_norm_0 = 0.0;
     double normup, normdown, conv_factor = 0.0;
+    double true_norm = -1.0; /* ||b - A x|| of the current x when known */
+    const int timed = s->prm.print_level > 0 || s->prof.on;
     s->t_prec_ms = 0.0;
 
 #define TRY(x)                                                                                                         \
@@ -2245,36 +2288,38 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             /* false convergence 1 check (pgmres.c:1200-1241) */
             TRY(bsr_apply(s, x, bvec, s->Rv, 1));
             TRY(host_norm(s, s->Rv, &r_norm));
-            if (r_norm <= epsilon) { converged = 1; break; }
+            if (r_norm <= epsilon) { converged = 1; true_norm = r_norm; break; }
             if (s->prm.is_check_restarted == 0) break;
         }
         TRY(jxamd_k_scale(N, NULL, 1.0 / r_norm, VEC(0), s->stream));
         i = 0;
+        int ahead = 0; /* step i + 1 already enqueued */
         while (i < k_dim && iter < max_iter) {
             i++;
             iter++;
-            TRY(precond_timed(s, VEC(i - 1), s->Rv));
-            nprec++;
-            TRY(bsr_apply(s, s->Rv, NULL, VEC(i), 0));
-            if (s->mgs_grid > 0) {
-                /* the whole MGS step in one launch: v_0..v_{i-1} read once,
-                 * w = v_i read and written once */
-                int pq = prof_begin(s);
-                TRY(jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1,
-                                     s->mflag, s->mgs_base, s->err, s->stream));
-                prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
-                s->mgs_base += i + 1;
-            } else {
-                /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
-                TRY(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
-                for (j = 1; j < i; j++) TRY(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
-                TRY(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i));
-                int pq = prof_begin(s);
-                TRY(jxamd_k_scale(N, h + i, 0.0, VEC(i), s->stream));
-                prof_end(s, pq, JXAMD_PROF_VEC, 16.0 * N);
+            const int slot = i & 1;
+            if (!ahead) {
+                TRY(arnoldi_enqueue(s, i, slot, timed));
+                nprec++;
             }
-            TRY(jxamd_d2h(hcol, h, sizeof(double) * (i + 1), s->stream));
-            TRY(accumulate_precond_time(s)); /* also synchronises the stream */
+            ahead = 0;
+            /* Look ahead: enqueue step i + 1 before reading step i's column
+             * when step i is unlikely to end the cycle (the residual the last
+             * contraction predicts stays 4x above the target), so the device
+             * does not idle while the host runs the Givens update.  A step
+             * enqueued in vain only writes v_{i+1}, which the update of x
+             * does not read. */
+            if (!timed && i < k_dim && iter < max_iter) {
+                const double cf = conv_factor > 0.0 && conv_factor < 1.0 ? conv_factor : 0.5;
+                if (r_norm * cf > 4.0 * epsilon) {
+                    TRY(arnoldi_enqueue(s, i + 1, slot ^ 1, timed));
+                    nprec++;
+                    ahead = 1;
+                }
+            }
+            if (timed) TRY(accumulate_precond_time(s)); /* also synchronises the stream */
+            else TRY(jxamd_event_sync(s->evh[slot]));
+            memcpy(hcol, s->hpin + (size_t)slot * (k_dim + 2), sizeof(double) * (i + 1));
             for (j = 0; j < i; j++) hh[j][i - 1] = hcol[j];
             hh[i][i - 1] = sqrt(hcol[i]);
             for (j = 1; j < i; j++) {
@@ -2301,6 +2346,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             if (iter > 1 && conv_factor > convfac_threshold) break;
             if (r_norm <= epsilon) break;
         }
+        if (ahead) nprec--; /* a step enqueued ahead in vain is not an application of the method */
         /* back-solve and update x += M^{-1} (sum y_j v_j) (pgmres.c:1477-1503) */
         rs[i - 1] = rs[i - 1] / hh[i - 1][i - 1];
         for (k = i - 2; k >= 0; k--) {
@@ -2315,14 +2361,14 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             TRY(jxamd_k_lincomb(N, i, V, N, ydev, s->Wv, s->stream));
             prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 1));
         }
-        TRY(precond_timed(s, s->Wv, s->Rv));
+        TRY(precond_timed(s, s->Wv, s->Rv, timed));
         nprec++;
         TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Rv, x, s->stream));
-        TRY(accumulate_precond_time(s));
+        if (timed) TRY(accumulate_precond_time(s));
         if (r_norm <= epsilon) {
             TRY(bsr_apply(s, x, bvec, s->Rv, 1));
             TRY(host_norm(s, s->Rv, &r_norm));
-            if (r_norm <= epsilon) { converged = 1; break; }
+            if (r_norm <= epsilon) { converged = 1; true_norm = r_norm; break; }
             if (s->prm.is_check_restarted == 1) {
                 /* false convergence 2: restart from the true residual */
                 TRY(jxamd_k_copy(N, s->Rv, VEC(0), s->stream));
@@ -2362,9 +2408,14 @@ done:
         st->t_solve_ms = ms;
         st->t_precond_ms = s->t_prec_ms;
         if (!err) {
-            double tr = 0.0;
-            int e2 = bsr_apply(s, x, bvec, s->Rv, 1);
-            if (!e2) e2 = host_norm(s, s->Rv, &tr);
+            /* the loop's last check already holds ||b - A x|| of this x when
+             * it ended on convergence */
+            double tr = true_norm;
+            int e2 = 0;
+            if (tr < 0.0) {
+                e2 = bsr_apply(s, x, bvec, s->Rv, 1);
+                if (!e2) e2 = host_norm(s, s->Rv, &tr);
+            }
             st->true_rel_res = b_norm > 0.0 ? tr / b_norm : tr;
         }
     }

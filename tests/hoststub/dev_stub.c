@@ -24,6 +24,10 @@ int jxamd_event_create(void **e) { *e = &token; return 0; }
 int jxamd_event_destroy(void *e) { (void)e; return 0; }
 int jxamd_event_record(void *e, void *s) { (void)e; (void)s; return 0; }
 int jxamd_event_elapsed(float *ms, void *e0, void *e1) { (void)e0; (void)e1; *ms = 0.f; return 0; }
+int jxamd_event_create_sync(void **e) { *e = &token; return 0; }
+int jxamd_event_sync(void *e) { (void)e; return 0; }
+int jxamd_host_alloc(void **p, size_t bytes) { *p = calloc(1, bytes ? bytes : 1); return *p ? 0 : JXAMD_ERROR_MEMORY; }
+int jxamd_host_free(void *p) { free(p); return 0; }
 int jxamd_last_error(void) { return 0; }
 
 int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
