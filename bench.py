@@ -194,27 +194,44 @@ def main():
     iters = st.iterations
     assert st.converged == 1 and st.true_rel_res <= 1e-6, "solve did not reach rtol 1e-6"
 
-    # per-kernel-class device time of one more solve, with events around every launch
+    # per-kernel device time of one more solve, with events around every
+    # launch: classes that are one kernel, and the smoother per level (one
+    # kernel per level); the dominant kernel is the one with the most time
     s.profile(True)
     x_d.zero_()
     s.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
     prof = s.profile_read()
+    levels = s.profile_read_levels()
     s.profile(False)
-    dom = max(prof, key=lambda k: prof[k][1])
-    n_l, ms_l, by_l = prof[dom]
+    single = {"bhgs_sweep": ("k_bhgs_part<3, 4, double, double, 1>" if s2chunk else "k_bhgs_multi<3>"),
+              "bsr_spmv": "k_bsr_spmv<3>", "mgs": "k_mgs_coop<16>", "coarse_solve": "k_dense_gemv<double>",
+              "bilu_solve": "k_btri"}
+    cands = [(name, prof[c]) for c, name in single.items() if prof[c][0] > 0]
+
+    def smoother_kernel(l):
+        ia = H.A[l][0]
+        m = (ia[-1] / (len(ia) - 1)) if len(ia) > 1 else 0.0
+        if not chunk:
+            return "k_gs_sweep<double, double>"
+        if m < 12:
+            return "k_gs_chunk_lds<4, double, double, true>"
+        g = 4
+        while g < 32 and g * 8 < m:
+            g *= 2
+        return f"k_gs_chunk_grp<4, {g}, double, double>"
+    cands += [(smoother_kernel(l) + f" (level {l})", v) for l, v in levels.items()]
+    dom_name, (n_l, ms_l, by_l) = max(cands, key=lambda c: c[1][1])
     per_launch_ms = ms_l / n_l
     achieved = (by_l / n_l) / (per_launch_ms * 1e-3) / 1e9
-    kernel_names = {"gs_sweep": "k_gs_chunk_lds/k_gs_chunk_grp (relax-6 chunks, all levels)" if chunk else "k_gs_sweep",
-                    "bhgs_sweep": "k_bhgs_part" if s2chunk else "k_bhgs_multi", "bsr_spmv": "k_bsr_spmv",
-                    "csr_spmv": "k_csr_spmv/k_csr_spmv_grp (all levels)", "coarse_solve": "k_dense_gemv",
-                    "mgs": "k_dot_mgs", "vector_ops": "k_scale/k_axpy/k_lincomb", "bilu_solve": "k_btri"}
-    traffic_key = {"bhgs_sweep": "k_bhgs_part<3, 4, double, double, 1>" if s2chunk else "k_bhgs_multi<3>"}.get(dom, dom)
-    traffic = read_traffic(traffic_key)
+    traffic = read_traffic(dom_name.split(" (level")[0])
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "kernel": kernel_names[dom], "launches_per_solve": n_l,
+                "kernel": dom_name, "launches_per_solve": n_l,
                 "mean_launch_us": round(per_launch_ms * 1e3, 2),
                 "algorithmic_bytes_per_launch": round(by_l / n_l)}
+    smoother_levels = {f"level {l} {smoother_kernel(l)}": {
+        "launches": v[0], "ms": round(v[1], 3), "GBs": round(v[2] / (v[1] * 1e-3) / 1e9, 1),
+        "traffic": read_traffic(smoother_kernel(l))} for l, v in levels.items()}
     breakdown = {k: {"launches": v[0], "ms": round(v[1], 3),
                      "GBs": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 else None} for k, v in prof.items()}
     log(f"kernel classes: {json.dumps(breakdown)}")
@@ -256,7 +273,8 @@ def main():
                       "true_rel_res": st.true_rel_res, "amg_levels": [len(m[0]) - 1 for m in H.A],
                       "host_setup_s": {"generate_decouple": round(t_gen, 2), "amg": round(t_amg, 2),
                                        "upload": round(t_create, 2)}},
-           "roofline": roofline, "spmv": spmv, "kernel_classes": breakdown, "cpu_baseline": cpu,
+           "roofline": roofline, "spmv": spmv, "kernel_classes": breakdown, "smoother_levels": smoother_levels,
+           "cpu_baseline": cpu,
            "mixed_precision_learned": mixed, "plain_fp64_one_rank_stage2": exact_stage2}
     s.close()
     print(json.dumps(out), flush=True)

@@ -309,6 +309,9 @@ enum {
 };
 int jxamd_solver_profile(jxamd_solver *s, int enable); /* enable also clears the counters */
 int jxamd_solver_profile_read(jxamd_solver *s, int kind, long *launches, double *ms, double *bytes);
+/* the smoother launches (JXAMD_PROF_GS_SWEEP) of one level: one kernel per
+ * level, so this is a per-kernel figure */
+int jxamd_solver_profile_read_level(jxamd_solver *s, int level, long *launches, double *ms, double *bytes);
 /* number of sync-free waits that hit the spin bound since creation (0 when healthy) */
 int jxamd_solver_error_flag(jxamd_solver *s, int *flag);
 

@@ -110,6 +110,7 @@ def _declare(L):
         "jxamd_amg_params_default": [vp],
         "jxamd_solver_profile": [vp, i32],
         "jxamd_solver_profile_read": [vp, i32, vp, vp, vp],
+        "jxamd_solver_profile_read_level": [vp, i32, vp, vp, vp],
         "jxamd_amg_setup": [vp, vp, vp],
         "jxamd_hierarchy_free": [vp],
         "jxamd_cpr_precond_timed": [vp, vp, vp, i32, vp],
@@ -486,6 +487,18 @@ class Solver:
             check(self._L.jxamd_solver_profile_read(self._s, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)),
                   "jxamd_solver_profile_read")
             out[name] = (n.value, ms.value, by.value)
+        return out
+
+    def profile_read_levels(self):
+        """{level: (launches, total_ms, total_algorithmic_bytes)} of the
+        smoother launches (one kernel per level), levels with launches only"""
+        out = {}
+        for l in range(self.num_levels):
+            n, ms, by = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
+            check(self._L.jxamd_solver_profile_read_level(self._s, l, ctypes.byref(n), ctypes.byref(ms),
+                                                          ctypes.byref(by)), "jxamd_solver_profile_read_level")
+            if n.value:
+                out[l] = (n.value, ms.value, by.value)
         return out
 
     def error_flag(self):

@@ -74,10 +74,12 @@ typedef struct {
 typedef struct {
     int on, n;
     void *ev0[PROF_POOL], *ev1[PROF_POOL];
-    int kind[PROF_POOL];
+    int kind[PROF_POOL], level[PROF_POOL]; /* level: smoother launches (one kernel per level), else -1 */
     double bytes[PROF_POOL];
     long count[JXAMD_PROF_KINDS];
     double ms[JXAMD_PROF_KINDS], tbytes[JXAMD_PROF_KINDS];
+    long lcount[JXAMD_MAX_LEVELS];
+    double lms[JXAMD_MAX_LEVELS], lbytes[JXAMD_MAX_LEVELS];
 } prof_t;
 
 struct jxamd_solver_s {
@@ -136,6 +138,12 @@ static int prof_flush(jxamd_solver *s)
         p->count[p->kind[q]]++;
         p->ms[p->kind[q]] += ms;
         p->tbytes[p->kind[q]] += p->bytes[q];
+        const int l = p->level[q];
+        if (l >= 0 && l < JXAMD_MAX_LEVELS) {
+            p->lcount[l]++;
+            p->lms[l] += ms;
+            p->lbytes[l] += p->bytes[q];
+        }
     }
     p->n = 0;
     return 0;
@@ -151,15 +159,18 @@ static int prof_begin(jxamd_solver *s)
     return p->n;
 }
 
-static void prof_end(jxamd_solver *s, int slot, int kind, double bytes)
+static void prof_end_level(jxamd_solver *s, int slot, int kind, int level, double bytes)
 {
     prof_t *p = &s->prof;
     if (slot < 0) return;
     if (jxamd_event_record(p->ev1[slot], s->stream)) return;
     p->kind[slot] = kind;
+    p->level[slot] = level;
     p->bytes[slot] = bytes;
     p->n++;
 }
+
+static void prof_end(jxamd_solver *s, int slot, int kind, double bytes) { prof_end_level(s, slot, kind, -1, bytes); }
 
 
 void jxamd_params_default(jxamd_params *p)
@@ -1663,12 +1674,12 @@ static int hsgs(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_z
     CK(jxamd_k_gs_sweep(g->vt, g->xt, g->nsl, g->perm, g->sptr, g->col, g->val, g->rsc, g->diag, f, fs,
                         uo_zero ? NULL : u, s->T[l], g->dep_ptr, g->dep_list, g->flag, s->epoch, 1, s->ticket,
                         &s->tbase, s->err, s->stream));
-    prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
+    prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, l, g->abytes);
     s->epoch++;
     pq = prof_begin(s);
     CK(jxamd_k_gs_sweep(h->vt, h->xt, h->nsl, h->perm, h->sptr, h->col, h->val, h->rsc, h->diag, f, fs, s->T[l], u,
                         h->dep_ptr, h->dep_list, h->flag, s->epoch, -1, s->ticket, &s->tbase, s->err, s->stream));
-    prof_end(s, pq, JXAMD_PROF_GS_SWEEP, h->abytes);
+    prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, l, h->abytes);
     return 0;
 }
 
@@ -1686,7 +1697,7 @@ static int hsgs_chunk(jxamd_solver *s, int l, const void *f, int fs, const void 
     else
         CK(jxamd_k_gs_chunk_lds(lvt(g), g->xt, s->nlev[l], s->nchunks[l], g->sptr, g->col, g->val, g->rsc, g->diag, f,
                                 fs, uo, un, s->stream));
-    prof_end(s, pq, JXAMD_PROF_GS_SWEEP, g->abytes);
+    prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, l, g->abytes);
     return 0;
 }
 
@@ -1707,7 +1718,7 @@ static int poly_apply(jxamd_solver *s, int l, const void *f, int fs, void *u, in
             CK(jxamd_k_poly_step(lvt(A), A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs, src, dst,
                                  s->poly_w[l][k], s->stream));
         /* a step from zeros reads only f and dinv */
-        prof_end(s, pq, JXAMD_PROF_GS_SWEEP, src ? A->abytes : 3.0 * (double)s->xsz * A->n);
+        prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, l, src ? A->abytes : 3.0 * (double)s->xsz * A->n);
         src = dst;
     }
     if (m == 1) return jxamd_d2d(u, s->T[l], s->xsz * s->nlev[l], s->stream);
@@ -2161,6 +2172,9 @@ int jxamd_solver_profile(jxamd_solver *s, int enable)
     memset(p->count, 0, sizeof(p->count));
     memset(p->ms, 0, sizeof(p->ms));
     memset(p->tbytes, 0, sizeof(p->tbytes));
+    memset(p->lcount, 0, sizeof(p->lcount));
+    memset(p->lms, 0, sizeof(p->lms));
+    memset(p->lbytes, 0, sizeof(p->lbytes));
     p->on = enable ? 1 : 0;
     return 0;
 }
@@ -2172,6 +2186,16 @@ int jxamd_solver_profile_read(jxamd_solver *s, int kind, long *launches, double 
     if (launches) *launches = s->prof.count[kind];
     if (ms) *ms = s->prof.ms[kind];
     if (bytes) *bytes = s->prof.tbytes[kind];
+    return 0;
+}
+
+int jxamd_solver_profile_read_level(jxamd_solver *s, int level, long *launches, double *ms, double *bytes)
+{
+    if (!s || level < 0 || level >= JXAMD_MAX_LEVELS) return JXAMD_ERROR_ARG;
+    CK(prof_flush(s));
+    if (launches) *launches = s->prof.lcount[level];
+    if (ms) *ms = s->prof.lms[level];
+    if (bytes) *bytes = s->prof.lbytes[level];
     return 0;
 }
 
