@@ -554,8 +554,9 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
 // recurrence (phase B).  x_old: xo (B values per row) or, when xo ==
 // nullptr, the prolongated pressure xp (slot pidx, zeros elsewhere).
 // Blocks are stored as V (double, float, or __half scaled per block row:
-// A_ij = rsc[i] * H_ij), inverted diagonal blocks as DV; arithmetic in
-// double.  PU slots per batch.
+// A_ij = rsc[i] * H_ij), inverted diagonal blocks as DV (dinv == nullptr:
+// every diagonal block is the identity, as after ABF decoupling, and none
+// is read); arithmetic in double.  PU slots per batch.
 template <int B, int LMAX, typename V, typename DV, int PU>
 __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, const int *__restrict__ sptr,
                                                    const int *__restrict__ col, const V *__restrict__ val,
@@ -669,12 +670,16 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
                             for (int q = 0; q < B; q++) tmp[rr] -= (double)L[rr * B + q] * xnew[u][q];
                     }
                 }
-                const DV *D = dinv + (size_t)(ns + t) * B * B;
+                const DV *D = dinv ? dinv + (size_t)(ns + t) * B * B : nullptr;
 #pragma unroll
                 for (int rr = 0; rr < B; rr++) {
                     double o = 0.0;
+                    if (D) {
 #pragma unroll
-                    for (int q = 0; q < B; q++) o += (double)D[rr * B + q] * tmp[q];
+                        for (int q = 0; q < B; q++) o += (double)D[rr * B + q] * tmp[q];
+                    } else {
+                        o = tmp[rr]; // identity diagonal blocks (ABF-decoupled rows)
+                    }
                     xnew[t][rr] = prod * xold[(rl + t) * B + rr] + w * o;
                 }
             } else {

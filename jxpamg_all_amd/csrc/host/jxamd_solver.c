@@ -58,6 +58,8 @@ typedef struct {
     void *val32, *dinv32;      /* FP32 (stage2_precision 1) or FP16 blocks (2; FP32 dinv) for the partitioned stage 2 */
     float *rsc16;              /* stage2_precision 2: power-of-two scale per block row */
     double spmv_bytes, sweep_bytes, sweep_bytes32; /* algorithmic bytes: y = A x, one block-HGS sweep (FP64, narrow blocks) */
+    int unit_diag;             /* every diagonal block is exactly the identity: the partitioned sweeps skip Dinv */
+    double part_bytes, part_bytes32; /* algorithmic bytes of one partitioned sweep (FP64, narrow blocks) */
 } dbsr;
 
 /* one BILU(0) triangle on the device: level-ordered block SELL of the
@@ -1108,6 +1110,14 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             }
             memcpy(dinv + (size_t)i * bb, M, sizeof(double) * bb);
         }
+        d->unit_diag = 1;
+        for (int i = 0; i < nb && d->unit_diag; i++)
+            for (int r = 0; r < b; r++)
+                for (int c2 = 0; c2 < b; c2++)
+                    if (A->a[(size_t)A->ia[i] * bb + r * b + c2] != (r == c2 ? 1.0 : 0.0)) d->unit_diag = 0;
+        /* the partitioned sweeps then read no Dinv */
+        d->part_bytes = d->sweep_bytes - (d->unit_diag ? 8.0 * nb * bb : 0.0);
+        d->part_bytes32 = d->sweep_bytes32 - (d->unit_diag ? 4.0 * nb * bb : 0.0);
         /* columns as positions: ~pos for lower neighbours, the own position for padding */
         {
             int *inv = (int *)malloc(sizeof(int) * (size_t)nb);
@@ -1871,9 +1881,9 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
             const int nar = s->prm.stage2_precision; /* 0 FP64, 1 FP32, 2 FP16 blocks */
             CK(jxamd_k_bhgs_part(b, stage2_vt(nar), s->nb, parts, d->sptr, d->col,
                                  nar ? d->val32 : (const void *)d->val, d->rsc16,
-                                 nar ? d->dinv32 : (const void *)d->dinv, r, src, xp, pi, dst, s->prm.stage2_weight,
-                                 s->prm.stage2_omega, s->stream));
-            prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, nar ? d->sweep_bytes32 : d->sweep_bytes);
+                                 d->unit_diag ? NULL : (nar ? d->dinv32 : (const void *)d->dinv), r, src, xp, pi, dst,
+                                 s->prm.stage2_weight, s->prm.stage2_omega, s->stream));
+            prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, nar ? d->part_bytes32 : d->part_bytes);
             src = dst;
             dst = (dst == x) ? s->xtmp : x;
         }
@@ -2127,8 +2137,8 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
         const int nar = s->prm.stage2_precision;
         CK(jxamd_k_bhgs_part(s->b, stage2_vt(nar), s->nb, parts, s->A.sptr, s->A.col,
                              nar ? s->A.val32 : (const void *)s->A.val, s->A.rsc16,
-                             nar ? s->A.dinv32 : (const void *)s->A.dinv, d_b, d_x, NULL, 0, s->xtmp, w, omega,
-                             s->stream));
+                             s->A.unit_diag ? NULL : (nar ? s->A.dinv32 : (const void *)s->A.dinv), d_b, d_x, NULL, 0,
+                             s->xtmp, w, omega, s->stream));
     } else {
         CK(bhgs_run(s, 1, d_b, d_x, NULL, s->xtmp, w, omega));
     }
