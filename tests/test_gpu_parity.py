@@ -1,7 +1,7 @@
 """GPU parity: the HIP path through the C-ABI against the reference's golden
 vectors and the oracle.  FP64 throughout; the GPU sums with fused multiply-add
 and tree reductions, so operator results agree with the reference to a few
-ulps (tolerances below), and whole solves are held to the north-star bar:
+ulps (tolerances below), and whole solves are held to the parity bar:
 iteration count within +-2 of the reference and a true FP64 relative residual
 at or below rtol."""
 import numpy as np
