@@ -203,8 +203,8 @@ def main():
     prof = s.profile_read()
     levels = s.profile_read_levels()
     s.profile(False)
-    single = {"bhgs_sweep": ("k_bhgs_part<3, 4, double, double, 1>" if s2chunk else "k_bhgs_multi<3>"),
-              "bsr_spmv": "k_bsr_spmv<3>", "mgs": "k_mgs_coop<16>", "coarse_solve": "k_dense_gemv<double>",
+    single = {"bhgs_sweep": ("k_bhgs_part<3, 4, double, double, 4>" if s2chunk else "k_bhgs_multi<3>"),
+              "bsr_spmv": "k_bsr_spmv<3, 8>", "mgs": "k_mgs_coop<16>", "coarse_solve": "k_dense_gemv<double>",
               "bilu_solve": "k_btri"}
     cands = [(name, prof[c]) for c, name in single.items() if prof[c][0] > 0]
 
@@ -241,9 +241,9 @@ def main():
     s.bsr_matvec_timed(b_d.data_ptr(), y_d.data_ptr(), 5)
     ms = s.bsr_matvec_timed(b_d.data_ptr(), y_d.data_ptr(), args.spmv_reps)
     spmv_bytes = Ad.nnzb * (8 * Ad.b * Ad.b + 4) + 4 * (Ad.nb + 1) + 16 * Ad.n
-    spmv = {"kernel": "k_bsr_spmv<3>", "mean_launch_us": round(ms * 1e3, 2), "algorithmic_bytes": spmv_bytes,
+    spmv = {"kernel": "k_bsr_spmv<3, 8>", "mean_launch_us": round(ms * 1e3, 2), "algorithmic_bytes": spmv_bytes,
             "achieved_GBs": round(spmv_bytes / (ms * 1e-3) / 1e9, 1),
-            "frac": round(spmv_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": read_traffic("k_bsr_spmv<3>")}
+            "frac": round(spmv_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": read_traffic("k_bsr_spmv<3, 8>")}
     log(f"BSR SpMV {spmv}")
     assert s.error_flag() == 0
 

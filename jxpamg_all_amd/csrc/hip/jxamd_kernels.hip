@@ -105,7 +105,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks)
 // BSR SpMV, SELL-64, one lane per block row.
 //   mode 0: y = A x;  mode 1: y = rhs - A x;  mode 2: y = alpha*A x + beta*y
 // ---------------------------------------------------------------------------
-template <int B>
+template <int B, int SU>
 __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict__ sptr, const int *__restrict__ col,
                                                   const double *__restrict__ val, const double *__restrict__ x,
                                                   const double *__restrict__ rhs, double *__restrict__ y, int mode,
@@ -123,17 +123,27 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
 #pragma unroll
     for (int r = 0; r < B; r++) acc[r] = 0.0;
     // matrix entries are streamed once: non-temporal loads keep them from
-    // displacing the reused x entries in the caches
-    for (int k = k0; k < k1; k++) {
-        const int c = __builtin_nontemporal_load(col + (size_t)k * WAVE + lane);
-        double xv[B];
+    // displacing the reused x entries in the caches; SU slots per batch
+    for (int k = k0; k < k1; k += SU) {
+        int c[SU];
 #pragma unroll
-        for (int t = 0; t < B; t++) xv[t] = x[(size_t)c * B + t];
-        const double *v = val + (size_t)k * B * B * WAVE + lane;
+        for (int u = 0; u < SU; u++)
+            if (k + u < k1) c[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * WAVE + lane);
+        double xv[SU][B];
 #pragma unroll
-        for (int r = 0; r < B; r++) {
+        for (int u = 0; u < SU; u++)
+            if (k + u < k1)
 #pragma unroll
-            for (int t = 0; t < B; t++) acc[r] += __builtin_nontemporal_load(v + (r * B + t) * WAVE) * xv[t];
+                for (int t = 0; t < B; t++) xv[u][t] = x[(size_t)c[u] * B + t];
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            if (k + u >= k1) continue;
+            const double *v = val + (size_t)(k + u) * B * B * WAVE + lane;
+#pragma unroll
+            for (int r = 0; r < B; r++) {
+#pragma unroll
+                for (int t = 0; t < B; t++) acc[r] += __builtin_nontemporal_load(v + (r * B + t) * WAVE) * xv[u][t];
+            }
         }
     }
     if (row >= nb) return;
@@ -1453,9 +1463,11 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
     if (grid == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     switch (b) {
-    case 2: k_bsr_spmv<2><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
-    case 3: k_bsr_spmv<3><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
-    case 4: k_bsr_spmv<4><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
+    // eight slots per batch: a 7-point row in one batch (3x3 blocks on the
+    // 60x220x85 system: 117.8 us with one slot per step, 114.8 us with eight)
+    case 2: k_bsr_spmv<2, 8><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
+    case 3: k_bsr_spmv<3, 8><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
+    case 4: k_bsr_spmv<4, 8><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
     default: return JXAMD_ERROR_ARG;
     }
     return chk(hipGetLastError());
@@ -1577,8 +1589,8 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
     const int maxlen = jxamd_gs_chunk_lds_rows(nb, nparts);
     if (maxlen < 1 || maxlen > 8 || (!xo && !xp) || (vt == JXAMD_T_F16 && !rsc)) return JXAMD_ERROR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    // FP64 blocks keep one slot per step (nine doubles per block already keep
-    // a lane's loads busy); narrow blocks take four slots per batch
+    // four slots per batch (FP64 blocks: 3.90 -> 3.76 ms of stage 2 per
+    // 60x220x85 solve against one slot per step)
 #define LAUNCH_PART(BS, LM)                                                                                            \
     do {                                                                                                               \
         const int cpb = 256 / LM;                                                                                      \
@@ -1586,7 +1598,7 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
         const size_t lb = (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1);                                          \
         const size_t lds = sizeof(double) * 2 * 256 * BS + (vt == JXAMD_T_F64 ? sizeof(double) : sizeof(float)) * lb; \
         if (vt == JXAMD_T_F64)                                                                                        \
-            k_bhgs_part<BS, LM, double, double, 1><<<grid, 256, lds, st>>>(                                            \
+            k_bhgs_part<BS, LM, double, double, 4><<<grid, 256, lds, st>>>(                                            \
                 nb, nparts, cpb, sptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo, xp, pidx, xn, w,  \
                 omega);                                                                                                \
         else if (vt == JXAMD_T_F32)                                                                                    \
