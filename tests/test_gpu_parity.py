@@ -211,15 +211,21 @@ def test_c2_host_setup_and_gpu_solve_vs_reference():
     print(f"[gpu] case=c2 nb={Ad.nb} levels={H.num_levels}", flush=True)
     S = Ad.to_scipy()
     ref_it = int(d["iterations"][0])
-    for chunk in (0, jx.BENCH_RELAX_CHUNK):
-        s = jx.Solver(Ad, H, jx.default_params(tol=1e-6, max_iter=100, relax_chunk=chunk))
+    runs = [(0, {}), (jx.BENCH_RELAX_CHUNK, {}),
+            (jx.BENCH_RELAX_CHUNK, dict(stage2_chunk=jx.BENCH_STAGE2_CHUNK)),
+            # the mixed configuration: FP16 stage 1, degree-2 polynomial, one
+            # FP16-block stage-2 sweep; FP64 GMRES must still reach rtol
+            (0, dict(precond_precision=2, poly_degree=2, stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8,
+                     stage2_chunk=jx.BENCH_STAGE2_CHUNK, stage2_precision=2))]
+    for chunk, kw in runs:
+        s = jx.Solver(Ad, H, jx.default_params(tol=1e-6, max_iter=100, relax_chunk=chunk, **kw))
         x, st, hist = s.solve(bd)
         true = np.linalg.norm(bd - S @ x) / np.linalg.norm(bd)
-        print(f"[gpu] c2 chunk {chunk}: iters {st.iterations} (reference {ref_it}) true_rel {true:.3e} "
+        print(f"[gpu] c2 chunk {chunk} {kw}: iters {st.iterations} (reference {ref_it}) true_rel {true:.3e} "
               f"solve {st.t_solve_ms:.1f} ms", flush=True)
         assert abs(st.iterations - ref_it) <= 2
         assert st.converged == 1 and true <= 1e-6
-        if chunk == 0:
+        if chunk == 0 and not kw:
             n = min(len(hist), len(d["res_history"]))
             assert relerr(hist[:n], d["res_history"][:n]) < 1e-6
         assert s.error_flag() == 0
