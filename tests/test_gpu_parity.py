@@ -307,7 +307,7 @@ def test_bilu0_stage2_vs_oracle(name, prec):
     s.close()
 
 
-@pytest.mark.parametrize("name,chunk", [("c1", 4), ("b4", 3), ("b2", 1), ("lvl3", 8), ("layered", 4)])
+@pytest.mark.parametrize("name,chunk", [("c1", 4), ("b4", 3), ("b2", 1), ("lvl3", 8), ("layered", 4), ("nodec", 4)])
 def test_stage2_rank_partitions_vs_oracle(name, chunk):
     """stage2_chunk = c: stage 2 as run on ceil(nb / c) ranks, against the
     oracle's restatement; the solve stays within two iterations of the
