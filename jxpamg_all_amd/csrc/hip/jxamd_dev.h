@@ -26,7 +26,8 @@ int jxamd_event_elapsed(float *ms, void *e0, void *e1);
 /* events without timing (completion only), waiting on one, pinned host memory */
 int jxamd_event_create_sync(void **e);
 int jxamd_event_sync(void *e);
-int jxamd_host_alloc(void **p, size_t bytes);
+int jxamd_host_alloc(void **p, size_t bytes); /* mapped, coherent */
+int jxamd_host_dev_ptr(void **d, void *h);    /* its device address */
 int jxamd_host_free(void *p);
 int jxamd_last_error(void);
 
@@ -107,10 +108,11 @@ int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *str
 int jxamd_dot_grid(size_t n);
 /* one MGS step of the Arnoldi process (k_mgs_coop): plan (grid 0: not
  * available) and cooperative launch; hval[0..m-1] = projections, hval[m] =
- * squared norm before scaling; part >= (m + 1) * grid doubles */
+ * squared norm before scaling; part >= (m + 1) * grid doubles; hhost (device
+ * address of mapped host memory, or NULL) also receives hval[0..m] */
 int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm);
 int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, void *stream);
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, void *stream);
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream);
 int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream);

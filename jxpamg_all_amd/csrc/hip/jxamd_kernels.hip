@@ -1280,7 +1280,8 @@ __device__ __forceinline__ double block_sum(double v, double *red)
 template <int TM>
 __global__ void __launch_bounds__(MGS_BS) k_mgs_coop(size_t n, const double *__restrict__ V, size_t ld, int m,
                                                      double *__restrict__ w, double *hval, double *part,
-                                                     unsigned int *cnt, int *flag, int base, int *err)
+                                                     unsigned int *cnt, int *flag, int base, int *err,
+                                                     double *hhost)
 {
     __shared__ double red[MGS_BS / WAVE];
     __shared__ double hb;
@@ -1334,6 +1335,13 @@ __global__ void __launch_bounds__(MGS_BS) k_mgs_coop(size_t n, const double *__r
                 __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 st_flag_release(flag, target);
                 hb = sum;
+            }
+            if (k == m && hhost) {
+                // the whole column to pinned host memory (system-scope stores,
+                // visible to the host once the launch has completed)
+                for (int q = threadIdx.x; q <= m; q += blockDim.x)
+                    __hip_atomic_store(hhost + q, q == m ? sum : ld_coherent(hval + q), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
             }
         } else if (threadIdx.x == 0) {
             int spins = 0;
@@ -1438,7 +1446,11 @@ int jxamd_event_create_sync(void **e)
     return r;
 }
 int jxamd_event_sync(void *e) { return chk(hipEventSynchronize((hipEvent_t)e)); }
-int jxamd_host_alloc(void **p, size_t bytes) { return chk(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault)); }
+int jxamd_host_alloc(void **p, size_t bytes)
+{
+    return chk(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocMapped | hipHostMallocCoherent));
+}
+int jxamd_host_dev_ptr(void **d, void *h) { return chk(hipHostGetDevicePointer(d, h, 0)); }
 int jxamd_host_free(void *p) { return p ? chk(hipHostFree(p)) : 0; }
 int jxamd_event_create(void **e)
 {
@@ -1883,10 +1895,10 @@ int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm)
 }
 
 int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, void *stream)
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, void *stream)
 {
     if (grid < 1 || m < 0 || (size_t)grid * MGS_BS * (size_t)tm < n) return JXAMD_ERROR_ARG;
-    void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err};
+    void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err, &hhost};
     const void *fn;
     switch (tm) {
     case 4: fn = (const void *)k_mgs_coop<4>; break;

@@ -90,6 +90,7 @@ struct jxamd_solver_s {
     void *ev0, *ev1, *ev2, *ev3;
     void *evh[2];          /* Arnoldi columns copied to hpin[slot] (completion events) */
     double *hpin;          /* pinned host memory: 2 x (k_dim + 2) doubles */
+    double *hpin_dev;      /* its device address (the cooperative MGS step writes the column there) */
     int nb, b;
     size_t N;
     dbsr A;
@@ -1575,6 +1576,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_event_create_sync(&s->evh[0]);
     if (!e) e = jxamd_event_create_sync(&s->evh[1]);
     if (!e) e = jxamd_host_alloc((void **)&s->hpin, sizeof(double) * 2 * ((size_t)p->k_dim + 2));
+    if (!e) e = jxamd_host_dev_ptr((void **)&s->hpin_dev, s->hpin);
     if (!e) e = build_bsr(s, A);
     if (!e && p->stage2_solver_type == 5) e = build_bilu(s, A);
     for (int l = 0; l < s->L && !e; l++) {
@@ -2255,9 +2257,10 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
          * w = v_i read and written once */
         int pq = prof_begin(s);
         CK(jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1, s->mflag,
-                            s->mgs_base, s->err, s->stream));
+                            s->mgs_base, s->err, s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->stream));
         prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
         s->mgs_base += i + 1;
+        return jxamd_event_record(s->evh[slot], s->stream); /* the kernel wrote the column to hpin[slot] */
     } else {
         /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
         CK(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));

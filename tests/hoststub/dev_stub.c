@@ -28,6 +28,7 @@ int jxamd_event_create_sync(void **e) { *e = &token; return 0; }
 int jxamd_event_sync(void *e) { (void)e; return 0; }
 int jxamd_host_alloc(void **p, size_t bytes) { *p = calloc(1, bytes ? bytes : 1); return *p ? 0 : JXAMD_ERROR_MEMORY; }
 int jxamd_host_free(void *p) { free(p); return 0; }
+int jxamd_host_dev_ptr(void **d, void *h) { *d = h; return 0; }
 int jxamd_last_error(void) { return 0; }
 
 int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
@@ -96,8 +97,8 @@ int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *str
 int jxamd_dot_grid(size_t n) { (void)n; return 1; }
 int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm) { *grid = n > 0 ? 2 : 0; *tm = 4; return 0; }
 int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, void *stream)
-{ (void)grid; (void)tm; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)stream; return 0; }
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, void *stream)
+{ (void)grid; (void)tm; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)stream; return 0; }
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream)
 { (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }
