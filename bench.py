@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--cpu-iters", type=int, default=2, help="GMRES iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-kind", choices=("reference", "port"), default="reference",
+                    help="CPU baseline: the reference built from its sources (oracle/_ref/ref_driver, when built) "
+                         "or the oracle's restatement")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-learned", action="store_true", help="skip the mixed-precision + learned-smoother leg")
     ap.add_argument("--train-seeds", type=int, default=20,
@@ -62,6 +65,42 @@ def read_traffic(key):
             return json.load(f).get(key)
     except (OSError, ValueError):
         return None
+
+
+def reference_cpu_baseline(A, rhs, iters, n_apply):
+    """The reference's own solve phase (oracle/_ref/ref_driver: jxf_ library
+    compiled from the reference sources, FP64, one thread) on the same system
+    for `iters` GMRES iterations, scaled to n_apply CPR applications; None
+    when the driver is not built or fails."""
+    import re
+    import subprocess
+    import tempfile
+
+    import jxpamg_all_amd as jx
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.access(exe, os.X_OK):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        fa, fb, fo = (os.path.join(tmp, n) for n in ("A.bin", "b.rhs", "out.dump"))
+        jx.write_bsr_binary(fa, A)
+        jx.write_rhs_text(fb, rhs)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        try:
+            r = subprocess.run([exe, fa, fb, fo, "1e-6", str(iters), "3", "1", "0", "30"], capture_output=True,
+                               text=True, timeout=600, env=env)
+        except (OSError, subprocess.TimeoutExpired):
+            return None
+    m = re.search(r"ORACLE iters=(\d+) .* solve=([0-9.]+)s setup=([0-9.]+)s", r.stderr)
+    if r.returncode != 0 or not m:
+        return None
+    t_solve, applies = float(m.group(2)), int(m.group(1)) + 1
+    cpu = {"value": round(t_solve / applies * n_apply, 2), "unit": "s", "cores": 1, "kind": "reference",
+           "sample": f"{m.group(1)} GMRES iterations ({applies} CPR applications, {t_solve:.1f} s of "
+                     f"JXF_GMRESSolve) of the same system with the reference's FP64 jxf_ library built from its "
+                     f"sources (oracle/_ref/ref_driver, OMP_NUM_THREADS=1; its host setup {float(m.group(3)):.1f} s "
+                     f"not counted), scaled to the {n_apply} CPR applications of the GPU solve"}
+    log(f"cpu baseline {cpu}")
+    return cpu
 
 
 def main():
@@ -248,7 +287,9 @@ def main():
     assert s.error_flag() == 0
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.cpu_kind == "reference":
+        cpu = reference_cpu_baseline(A, rhs, args.cpu_iters, st.num_precond)
+    if not args.no_cpu and cpu is None:
         from oracle.cpu_ref import Oracle
         o = Oracle(Ad, H)
         p = o.params(tol=1e-6, max_iter=args.cpu_iters)  # one-thread reference semantics
