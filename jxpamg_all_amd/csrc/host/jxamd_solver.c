@@ -2256,12 +2256,20 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
         /* the whole MGS step in one launch: v_0..v_{i-1} read once,
          * w = v_i read and written once */
         int pq = prof_begin(s);
-        CK(jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1, s->mflag,
-                            s->mgs_base, s->err, s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->stream));
-        prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
-        s->mgs_base += i + 1;
-        return jxamd_event_record(s->evh[slot], s->stream); /* the kernel wrote the column to hpin[slot] */
-    } else {
+        const int e = jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1,
+                                       s->mflag, s->mgs_base, s->err,
+                                       s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->stream);
+        if (!e) {
+            prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
+            s->mgs_base += i + 1;
+            return jxamd_event_record(s->evh[slot], s->stream); /* the kernel wrote the column to hpin[slot] */
+        }
+        /* the cooperative launch was refused (the grid could not be made
+         * resident): nothing ran; use the launch-per-projection MGS from now on */
+        (void)jxamd_last_error(); /* the opened timing slot is reused by the next prof_begin */
+        s->mgs_grid = 0;
+    }
+    if (s->mgs_grid == 0) {
         /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
         CK(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
         for (int j = 1; j < i; j++) CK(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
