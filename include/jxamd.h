@@ -182,6 +182,14 @@ typedef struct {
      * launch per projection.  Same arithmetic (MGS, pgmres.c:1258-1290
      * order); the reductions group their partial sums differently. */
     int arnoldi_kernel;
+    /* Update of x at the end of a GMRES cycle: 0 x += Z y with the vectors
+     * z_j = M^{-1} v_j that the Arnoldi steps formed kept in a second basis
+     * (k_dim more vectors; no preconditioner application at the cycle's
+     * end; the flexible-GMRES update, exact for the computed z_j), 1 x +=
+     * M^{-1} (V y) as the reference (pgmres.c:1477-1503; one more
+     * application per cycle).  For a linear FP64 preconditioner the two agree
+     * to rounding. */
+    int solution_update;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread

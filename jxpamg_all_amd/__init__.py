@@ -57,7 +57,8 @@ class Params(ctypes.Structure):
                 ("threshold", ctypes.c_double), ("stage2_weight", ctypes.c_double),
                 ("stage2_omega", ctypes.c_double), ("relax_threads", ctypes.c_int), ("relax_chunk", ctypes.c_int),
                 ("precond_precision", ctypes.c_int), ("poly_degree", ctypes.c_int), ("stage2_chunk", ctypes.c_int),
-                ("stage2_precision", ctypes.c_int), ("arnoldi_kernel", ctypes.c_int)]
+                ("stage2_precision", ctypes.c_int), ("arnoldi_kernel", ctypes.c_int),
+                ("solution_update", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

@@ -115,6 +115,7 @@ struct jxamd_solver_s {
     double *bz;            /* BILU(0) intermediate vector, N */
     double *xtmp;          /* stage-2 ping-pong partner, N */
     double *V;             /* (k_dim+1) Krylov vectors, N each */
+    double *Z;             /* k_dim preconditioned vectors M^{-1} v_j (solution_update 0), N each */
     double *Rv, *Wv, *Bd, *Xd; /* GMRES work, host-solve staging */
     double *dsc;           /* device scalars: h[0..k+1], y[0..k], tmp */
     double *part;
@@ -1540,6 +1541,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->stage2_chunk < 0 || p->stage2_chunk > 8) return JXAMD_ERROR_ARG;
     if (p->stage2_precision < 0 || p->stage2_precision > 2) return JXAMD_ERROR_ARG;
     if (p->arnoldi_kernel < 0 || p->arnoldi_kernel > 1) return JXAMD_ERROR_ARG;
+    if (p->solution_update < 0 || p->solution_update > 1) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -1594,6 +1596,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     const size_t N = s->N;
     if (!e) e = jxamd_dev_malloc((void **)&s->xtmp, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->V, sizeof(double) * N * (size_t)(p->k_dim + 1));
+    if (!e && p->solution_update == 0) e = jxamd_dev_malloc((void **)&s->Z, sizeof(double) * N * (size_t)p->k_dim);
     if (!e) e = jxamd_dev_malloc((void **)&s->Rv, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->Wv, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->Bd, sizeof(double) * N);
@@ -1647,7 +1650,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     }
     jxamd_dev_free(s->cinv); jxamd_dev_free(s->up64);
     dtri_free(&s->Lf); dtri_free(&s->Ub); jxamd_dev_free(s->bz);
-    jxamd_dev_free(s->xtmp); jxamd_dev_free(s->V); jxamd_dev_free(s->Rv); jxamd_dev_free(s->Wv);
+    jxamd_dev_free(s->xtmp); jxamd_dev_free(s->V); jxamd_dev_free(s->Z); jxamd_dev_free(s->Rv); jxamd_dev_free(s->Wv);
     jxamd_dev_free(s->Bd); jxamd_dev_free(s->Xd);
     jxamd_dev_free(s->dsc); jxamd_dev_free(s->part); jxamd_dev_free(s->count);
     jxamd_dev_free(s->mpart); jxamd_dev_free(s->mflag);
@@ -2250,8 +2253,10 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
     const size_t N = s->N;
     double *V = s->V, *h = s->dsc;
 #define VEC(j) (V + (size_t)(j) * N)
-    CK(precond_timed(s, VEC(i - 1), s->Rv, timed));
-    CK(bsr_apply(s, s->Rv, NULL, VEC(i), 0));
+    /* z = M^{-1} v_{i-1}, kept as column i-1 of Z for the cycle's update */
+    double *z = s->Z ? s->Z + (size_t)(i - 1) * N : s->Rv;
+    CK(precond_timed(s, VEC(i - 1), z, timed));
+    CK(bsr_apply(s, z, NULL, VEC(i), 0));
     if (s->mgs_grid > 0) {
         /* the whole MGS step in one launch: v_0..v_{i-1} read once,
          * w = v_i read and written once */
@@ -2401,15 +2406,21 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             rs[k] = t / hh[k][k];
         }
         TRY(jxamd_h2d(ydev, rs, sizeof(double) * i, s->stream));
-        {
+        if (s->Z) {
+            /* x += Z y */
+            int pq = prof_begin(s);
+            TRY(jxamd_k_lincomb(N, i, s->Z, N, ydev, s->Wv, s->stream));
+            prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 1));
+            TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Wv, x, s->stream));
+        } else {
             int pq = prof_begin(s);
             TRY(jxamd_k_lincomb(N, i, V, N, ydev, s->Wv, s->stream));
             prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 1));
+            TRY(precond_timed(s, s->Wv, s->Rv, timed));
+            nprec++;
+            TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Rv, x, s->stream));
+            if (timed) TRY(accumulate_precond_time(s));
         }
-        TRY(precond_timed(s, s->Wv, s->Rv, timed));
-        nprec++;
-        TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Rv, x, s->stream));
-        if (timed) TRY(accumulate_precond_time(s));
         if (r_norm <= epsilon) {
             TRY(bsr_apply(s, x, bvec, s->Rv, 1));
             TRY(host_norm(s, s->Rv, &r_norm));

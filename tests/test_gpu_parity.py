@@ -186,6 +186,47 @@ def test_arnoldi_kernels_agree(name):
     assert relerr(x0, x1) < 1e-9
 
 
+@pytest.mark.parametrize("name", ["lvl3", "restart", "b4"])
+def test_solution_updates_agree(name):
+    """x += Z y with the Arnoldi steps' z_j = M^{-1} v_j kept, against the
+    reference's x += M^{-1}(V y): for the FP64 preconditioner (linear) the
+    same Arnoldi process and the same x to rounding, with one preconditioner
+    application fewer per cycle."""
+    d, Ad, bd, H, meta, s0 = _solver(name, solution_update=0)
+    x0, st0, h0 = s0.solve(bd)
+    s0.close()
+    d, Ad, bd, H, meta, s1 = _solver(name, solution_update=1)
+    x1, st1, h1 = s1.solve(bd)
+    s1.close()
+    print(f"[gpu] {name}: solution updates iterations {st0.iterations} / {st1.iterations}, applications "
+          f"{st0.num_precond} / {st1.num_precond}", flush=True)
+    assert st0.iterations == st1.iterations and st0.converged == st1.converged == 1
+    n = st0.iterations + 1
+    assert relerr(h0[:n], h1[:n]) < 1e-9
+    assert relerr(x0, x1) < 1e-9
+    assert st0.true_rel_res <= meta["rtol"] and st1.true_rel_res <= meta["rtol"]
+    assert st0.num_precond == st0.iterations
+    assert st1.num_precond == st1.iterations + st1.num_restarts + 1
+
+
+def test_solution_updates_mixed_precision():
+    """With an FP16 stage 1 (a preconditioner linear only to FP16 rounding)
+    both updates reach the FP64 true residual in the same number of
+    iterations to one; the kept-basis update is the flexible-GMRES one, for
+    which the Arnoldi relation holds exactly."""
+    its = []
+    for update in (0, 1):
+        d, Ad, bd, H, meta, s = _solver("lvl3", solution_update=update, precond_precision=2, poly_degree=2,
+                                        relax_chunk=4, stage2_chunk=4, stage2_precision=2)
+        x, st, h = s.solve(bd)
+        s.close()
+        print(f"[gpu] lvl3 FP16 stage 1, update {update}: {st.iterations} iterations, true "
+              f"{st.true_rel_res:.3e}", flush=True)
+        assert st.converged == 1 and st.true_rel_res <= meta["rtol"]
+        its.append(st.iterations)
+    assert abs(its[0] - its[1]) <= 1
+
+
 def test_nan_rhs_is_reported_as_error():
     d, Ad, bd, H, meta, s = _solver("tiny")
     b = bd.copy()
