@@ -48,6 +48,7 @@ int main(int argc, char **argv)
                 if (ci == 6) p.poly_degree = 1;
                 if (ci == 5 && s2 == 1) p.poly_degree = 4;
                 if (ci == 1 || ci == 3) p.stage2_chunk = ci == 1 ? 1 : 5;
+                p.solution_update = (prec + s2) % 2; /* both end-of-cycle updates */
                 jxamd_solver *s = NULL;
                 int rc = jxamd_solver_create(&s, &A, &H, &p);
                 jxamd_stats st;
