@@ -48,6 +48,9 @@ int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, con
                      const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream);
 int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
                       const void *dinv, const void *f, int fs, const void *uo, void *un, double omega, void *stream);
+/* the first polynomial step from a zero guess, un = omega D^{-1} f, one thread
+ * per row; xt F64 or F32 */
+int jxamd_k_poly_first(int xt, int n, const void *dinv, const void *f, int fs, void *un, double omega, void *stream);
 int jxamd_sweep_waves(int nsl);
 int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                      const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,

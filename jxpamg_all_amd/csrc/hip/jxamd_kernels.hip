@@ -1102,6 +1102,18 @@ __global__ void k_gather_cast(int n, const double *__restrict__ x, int stride, i
     if (i < n) y[i] = (float)x[(size_t)i * stride + off];
 }
 
+// first polynomial step from a zero guess: u = 0 + w D^{-1} (f - 0), one
+// thread per row (the same operations as k_poly_step with uo = NULL)
+template <typename X>
+__global__ void k_poly_first(int n, const X *__restrict__ dinv, const X *__restrict__ f, int fs, X *__restrict__ un,
+                             X omega)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const X r = f[(size_t)i * fs] - (X)0;
+    un[i] = (X)0 + omega * dinv[i] * r;
+}
+
 __global__ void k_widen(int n, const float *__restrict__ x, double *__restrict__ y)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1529,6 +1541,21 @@ int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, co
                  (k_poly_step<V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
                      n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs, (const X_ *)uo,
                      (X_ *)un, (X_)omega)));
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_poly_first(int xt, int n, const void *dinv, const void *f, int fs, void *un, double omega, void *stream)
+{
+    if (n == 0) return 0;
+    const int grid = (n + 255) / 256;
+    if (xt == JXAMD_T_F64)
+        k_poly_first<double><<<grid, 256, 0, (hipStream_t)stream>>>(n, (const double *)dinv, (const double *)f, fs,
+                                                                     (double *)un, omega);
+    else if (xt == JXAMD_T_F32)
+        k_poly_first<float><<<grid, 256, 0, (hipStream_t)stream>>>(n, (const float *)dinv, (const float *)f, fs,
+                                                                    (float *)un, (float)omega);
+    else
+        return JXAMD_ERROR_ARG;
     return chk(hipGetLastError());
 }
 

@@ -1726,7 +1726,9 @@ static int poly_apply(jxamd_solver *s, int l, const void *f, int fs, void *u, in
     for (int k = 0; k < m; k++) {
         void *dst = (k == m - 1 && m > 1) ? u : ((k % 2 == 0) ? s->T[l] : s->Vt[l]);
         int pq = prof_begin(s);
-        if (A->G > 1)
+        if (!src)
+            CK(jxamd_k_poly_first(A->xt, A->n, s->dinvl[l], f, fs, dst, s->poly_w[l][k], s->stream));
+        else if (A->G > 1)
             CK(jxamd_k_poly_step_grp(A->G, lvt(A), A->xt, A->n, A->sptr, A->col, A->val, A->rsc, s->dinvl[l], f, fs,
                                      src, dst, s->poly_w[l][k], s->stream));
         else
