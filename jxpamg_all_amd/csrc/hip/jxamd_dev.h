@@ -51,6 +51,12 @@ int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, co
 /* the first polynomial step from a zero guess, un = omega D^{-1} f, one thread
  * per row; xt F64 or F32 */
 int jxamd_k_poly_first(int xt, int n, const void *dinv, const void *f, int fs, void *un, double omega, void *stream);
+/* restriction y = R x (R in SELL-64 when G = 1, SELL-G otherwise) that also
+ * writes the next level's first polynomial step from a zero guess,
+ * u = omega D^{-1} y, as jxamd_k_poly_first would */
+int jxamd_k_restrict_first(int G, int vt, int xt, int n, const int *sptr, const int *col, const void *val,
+                           const float *rsc, const void *x, void *y, const void *dinv, void *u, double omega,
+                           void *stream);
 int jxamd_sweep_waves(int nsl);
 int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                      const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,

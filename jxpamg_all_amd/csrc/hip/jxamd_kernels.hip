@@ -305,11 +305,21 @@ __device__ __forceinline__ void csr_row_gather(int k, int ke, int i, int ns, int
 // x and rhs may be strided views (xs, rs) so the pressure block of a BSR
 // vector is read in place.
 // ---------------------------------------------------------------------------
+// first polynomial step from a zero guess on one row whose right-hand side
+// f is already in a register: u = 0 + w D^{-1} (f - 0)
+template <typename X>
+__device__ __forceinline__ void poly_first_row(int row, const X *__restrict__ dinv, X f, X *__restrict__ u, X w)
+{
+    const X r = f - (X)0;
+    u[row] = (X)0 + w * dinv[row] * r;
+}
+
 template <typename V, typename X, bool PK>
 __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                   const V *__restrict__ val, const float *__restrict__ rsc,
                                                   const X *__restrict__ x, int xs, const X *__restrict__ rhs, int rs,
-                                                  X *__restrict__ y, int mode)
+                                                  X *__restrict__ y, int mode, const X *__restrict__ d2,
+                                                  X *__restrict__ u2, X w2)
 {
     const int s = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
@@ -322,6 +332,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int *__restrict__
     if (mode == 0) y[row] = acc;
     else if (mode == 1) y[row] = rhs[(size_t)row * rs] - acc;
     else y[row] += acc;
+    if (u2) poly_first_row(row, d2, acc, u2, w2); // mode 0 only (launcher checks)
 }
 
 // One step of the polynomial smoother on a level (SELL-64, full rows):
@@ -967,7 +978,8 @@ template <int G, typename V, typename X, bool PK>
 __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                       const V *__restrict__ val, const float *__restrict__ rsc,
                                                       const X *__restrict__ x, int xs, const X *__restrict__ rhs,
-                                                      int rs, X *__restrict__ y, int mode)
+                                                      int rs, X *__restrict__ y, int mode, const X *__restrict__ d2,
+                                                      X *__restrict__ u2, X w2)
 {
     constexpr int R = WAVE / G;
     const int nsl = (n + R - 1) / R;
@@ -982,6 +994,7 @@ __global__ void __launch_bounds__(256) k_csr_spmv_grp(int n, const int *__restri
     if (mode == 0) y[row] = acc;
     else if (mode == 1) y[row] = rhs[(size_t)row * rs] - acc;
     else y[row] += acc;
+    if (u2) poly_first_row(row, d2, acc, u2, w2); // mode 0 only (launcher checks)
 }
 
 template <int G, typename V, typename X, bool PK>
@@ -1527,7 +1540,8 @@ int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, con
     if (grid == 0) return 0;
     DISPATCH_VXP(vt, xt,
                  (k_csr_spmv<V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
-                     n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y, mode)));
+                     n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs, (X_ *)y, mode,
+                     nullptr, nullptr, (X_)0)));
     return chk(hipGetLastError());
 }
 
@@ -1769,7 +1783,30 @@ int jxamd_k_csr_spmv_grp(int G, int vt, int xt, int n, const int *sptr, const in
     DISPATCH_G(G, DISPATCH_VXP(vt, xt,
                                (k_csr_spmv_grp<G_, V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
                                    n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, xs, (const X_ *)rhs, rs,
-                                   (X_ *)y, mode))));
+                                   (X_ *)y, mode, nullptr, nullptr, (X_)0))));
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_restrict_first(int G, int vt, int xt, int n, const int *sptr, const int *col, const void *val,
+                           const float *rsc, const void *x, void *y, const void *dinv, void *u, double omega,
+                           void *stream)
+{
+    if (n == 0) return 0;
+    if (!dinv || !u || (G != 1 && (G < 4 || G > 32))) return JXAMD_ERROR_ARG;
+    if (G == 1) {
+        const int nsl = (n + WAVE - 1) / WAVE;
+        const int grid = (nsl + 3) / 4;
+        DISPATCH_VXP(vt, xt,
+                     (k_csr_spmv<V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                         n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, 1, (const X_ *)nullptr, 1, (X_ *)y, 0,
+                         (const X_ *)dinv, (X_ *)u, (X_)omega)));
+    } else {
+        const int grid = grp_grid(n, G);
+        DISPATCH_G(G, DISPATCH_VXP(vt, xt,
+                                   (k_csr_spmv_grp<G_, V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
+                                       n, sptr, col, (const V_ *)val, rsc, (const X_ *)x, 1, (const X_ *)nullptr, 1,
+                                       (X_ *)y, 0, (const X_ *)dinv, (X_ *)u, (X_)omega))));
+    }
     return chk(hipGetLastError());
 }
 

@@ -1717,13 +1717,21 @@ static int hsgs_chunk(jxamd_solver *s, int l, const void *f, int fs, const void 
 }
 
 /* one application of the level's polynomial smoother; steps alternate
- * between T and Vt and the last one lands in u */
+ * between T and Vt and the last one lands in u.  uo_zero: 0 u holds the
+ * entry values, 1 they are zero, 2 they are zero and the first step is
+ * already in T[l] */
 static int poly_apply(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
     const dsell *A = &s->Ares[l];
     const int m = s->prm.poly_degree;
     const void *src = uo_zero ? NULL : u;
-    for (int k = 0; k < m; k++) {
+    int k0 = 0;
+    if (uo_zero == 2) {
+        /* the restriction already wrote the first step into T[l] */
+        src = s->T[l];
+        k0 = 1;
+    }
+    for (int k = k0; k < m; k++) {
         void *dst = (k == m - 1 && m > 1) ? u : ((k % 2 == 0) ? s->T[l] : s->Vt[l]);
         int pq = prof_begin(s);
         if (!src)
@@ -1739,6 +1747,23 @@ static int poly_apply(jxamd_solver *s, int l, const void *f, int fs, void *u, in
         src = dst;
     }
     if (m == 1) return jxamd_d2d(u, s->T[l], s->xsz * s->nlev[l], s->stream);
+    return 0;
+}
+
+static int csr_launch(jxamd_solver *s, const dsell *A, const void *x, const void *rhs, int rs, void *y, int mode);
+
+/* F_{l+1} = R_l Vt_l; with the polynomial smoother the same launch writes the
+ * first step of level l+1's pre-smoothing (from zero) into T[l+1] */
+static int restrict_launch(jxamd_solver *s, int l)
+{
+    const dsell *R = &s->R[l];
+    const int c = l + 1;
+    if (s->prm.poly_degree == 0 || c >= s->L - 1)
+        return csr_launch(s, R, s->Vt[l], NULL, 1, s->F[c], 0);
+    int pq = prof_begin(s);
+    CK(jxamd_k_restrict_first(R->G > 1 ? R->G : 1, lvt(R), R->xt, R->n, R->sptr, R->col, R->val, R->rsc, s->Vt[l],
+                              s->F[c], s->dinvl[c], s->T[c], s->poly_w[c][0], s->stream));
+    prof_end(s, pq, JXAMD_PROF_CSR_SPMV, R->abytes + 2.0 * (double)s->xsz * R->n);
     return 0;
 }
 
@@ -1791,11 +1816,13 @@ static int vcycle(jxamd_solver *s, const void *f0, int fs0, int u0_zero)
     for (int l = 0; l < L - 1; l++) {
         const void *f = l == 0 ? f0 : s->F[l];
         const int fs = l == 0 ? fs0 : 1;
-        const int zero = l == 0 ? u0_zero : 1;
+        /* coarse levels start from zero; with the polynomial smoother their
+         * first step came with the restriction (2) */
+        const int zero = l == 0 ? u0_zero : (s->prm.poly_degree > 0 ? 2 : 1);
         CK(relax_pair(s, l, f, fs, s->U[l], zero));
         /* Vtemp = F - A U; F_{l+1} = P^T Vtemp */
         CK(csr_launch(s, &s->Ares[l], s->U[l], f, fs, s->Vt[l], 1));
-        CK(csr_launch(s, &s->R[l], s->Vt[l], NULL, 1, s->F[l + 1], 0));
+        CK(restrict_launch(s, l));
     }
     int pc = prof_begin(s);
     CK(jxamd_k_dense_gemv(s->xt, s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));

@@ -42,6 +42,10 @@ int jxamd_k_poly_step(int vt, int xt, int n, const int *sptr, const int *col, co
 { (void)vt; (void)xt; (void)n; (void)sptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)f; (void)fs; (void)uo; (void)un; (void)omega; (void)stream; return 0; }
 int jxamd_k_poly_first(int xt, int n, const void *dinv, const void *f, int fs, void *un, double omega, void *stream)
 { (void)xt; (void)n; (void)dinv; (void)f; (void)fs; (void)un; (void)omega; (void)stream; return 0; }
+int jxamd_k_restrict_first(int G, int vt, int xt, int n, const int *sptr, const int *col, const void *val,
+                           const float *rsc, const void *x, void *y, const void *dinv, void *u, double omega,
+                           void *stream)
+{ (void)G; (void)vt; (void)xt; (void)n; (void)sptr; (void)col; (void)val; (void)rsc; (void)x; (void)y; (void)dinv; (void)u; (void)omega; (void)stream; return 0; }
 int jxamd_sweep_waves(int nsl) { return nsl < 2048 ? (nsl > 0 ? nsl : 1) : 2048; }
 int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                      const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
