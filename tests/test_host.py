@@ -209,6 +209,13 @@ def test_solver_rejects_bad_input_without_gpu_calls():
     hc2 = bad2.c_struct()
     rc = jx.lib().jxamd_solver_create(ctypes.byref(s), ctypes.byref(ac), ctypes.byref(hc2), ctypes.byref(p))
     assert rc == jx.ERROR_ARG
+    # out-of-range options are rejected before any device work too
+    good = jx.Hierarchy([(ia, ja, a, Ad.nb)], []).c_struct()
+    for field, value in [("solution_update", 2), ("arnoldi_kernel", -1), ("stage2_precision", 3),
+                         ("poly_degree", 9), ("precond_precision", 3)]:
+        pb = jx.default_params(**{field: value})
+        rc = jx.lib().jxamd_solver_create(ctypes.byref(s), ctypes.byref(ac), ctypes.byref(good), ctypes.byref(pb))
+        assert rc == jx.ERROR_ARG, field
 
 
 @pytest.mark.parametrize("name", ALL)
