@@ -323,6 +323,19 @@ int jxamd_solver_profile_read_level(jxamd_solver *s, int level, long *launches, 
 /* number of sync-free waits that hit the spin bound since creation (0 when healthy) */
 int jxamd_solver_error_flag(jxamd_solver *s, int *flag);
 
+/* ------------------------------------------------------------------------
+ * Standalone device operations on the reference BSR layout (device arrays
+ * d_ia[nb+1], d_ja[nnzb], d_a[nnzb*b*b]; no solver handle).
+ * ---------------------------------------------------------------------- */
+/* d_norms[0] = ||b - A x||_2, d_norms[1] = ||b||_2 (device memory, FP64),
+ * b = 1..4: the true-residual check GMRES makes on convergence
+ * (jxfpamg-0220/csrc/krylov/pgmres.c:1505-1511: CopyVector b -> r, Matvec
+ * alpha -1 beta 1, sqrt(InnerProd(r, r)); ||b|| as at pgmres.c:1062).
+ * Deterministic (fixed-order reduction).  Allocates a 33 KB work buffer per
+ * call and returns after the stream has finished the launch. */
+int jxamd_bsr_residual_norm(int nb, int b, const int *d_ia, const int *d_ja, const double *d_a, const double *d_x,
+                            const double *d_b, double *d_norms, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

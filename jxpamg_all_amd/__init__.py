@@ -126,6 +126,7 @@ def _declare(L):
         "jxamd_solver_vcycle_factor": [vp, i32, vp],
         "jxamd_solver_apply_mlp": [vp, vp, vp, vp, vp],
         "jxamd_bilu0_factor": [i32, i32, vp, vp, vp, vp, vp],
+        "jxamd_bsr_residual_norm": [i32, i32, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -233,6 +234,13 @@ def solver_interface(row, ia, ja, a, ser_x, ser_b, tol, solver_id, num_functions
                                        _p(a), _p(ser_x), _p(ser_b), float(tol), int(solver_id), int(num_functions),
                                        ctypes.byref(it)), "jxamd_solver_interface")
     return it.value
+
+
+def bsr_residual_norm(nb, b, d_ia, d_ja, d_a, d_x, d_b, d_norms, stream=None):
+    """d_norms[0:2] = (||b - A x||, ||b||) for device BSR arrays (raw device
+    addresses, reference layout); pgmres.c:1505-1511.  Synchronous."""
+    check(lib().jxamd_bsr_residual_norm(int(nb), int(b), d_ia, d_ja, d_a, d_x, d_b, d_norms, stream),
+          "jxamd_bsr_residual_norm")
 
 
 def bilu0_factor(A):
