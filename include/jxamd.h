@@ -335,6 +335,19 @@ int jxamd_solver_error_flag(jxamd_solver *s, int *flag);
  * call and returns after the stream has finished the launch. */
 int jxamd_bsr_residual_norm(int nb, int b, const int *d_ia, const int *d_ja, const double *d_a, const double *d_x,
                             const double *d_b, double *d_norms, void *stream);
+/* The same computation with no allocation and no synchronisation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its work buffer is a static device array of 33,024
+ * bytes per GPU (partial sums of at most 2048 workgroups plus a counter),
+ * held from the first call on that GPU until the process ends; the counter
+ * is cleared by every call.  The launch is enqueued on `stream` and the
+ * call returns at once.  Calls that overlap on one GPU (several host threads
+ * or streams) share the array and race.  nb*b above 2^31 - 1 is rejected
+ * with JXAMD_ERROR_ARG (every smaller system fits the buffer: the grid never
+ * exceeds 2048 workgroups). */
+int jxamd_bsr_residual_norm_noalloc(int nb, int b, const int *d_ia, const int *d_ja, const double *d_a,
+                                    const double *d_x, const double *d_b, double *d_norms, void *stream);
 
 #ifdef __cplusplus
 }

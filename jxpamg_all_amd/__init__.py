@@ -127,6 +127,7 @@ def _declare(L):
         "jxamd_solver_apply_mlp": [vp, vp, vp, vp, vp],
         "jxamd_bilu0_factor": [i32, i32, vp, vp, vp, vp, vp],
         "jxamd_bsr_residual_norm": [i32, i32, vp, vp, vp, vp, vp, vp, vp],
+        "jxamd_bsr_residual_norm_noalloc": [i32, i32, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -13,6 +13,8 @@
 // pgmres.c:1062).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 extern "C" {
 #include "jxamd_dev.h"
 }
@@ -148,6 +150,31 @@ int rn_launch(int nb, int b, const int *ia, const int *ja, const double *a, cons
     return hipGetLastError() == hipSuccess ? 0 : JXAMD_ERROR_DEVICE;
 }
 
+// Scratch of the allocation-free entry point.  In host code `scratch` names
+// a host-side stand-in, not device memory: only the address returned by
+// hipGetSymbolAddress (one per device) may be handed to kernels.
+static __device__ __align__(256) unsigned char scratch[RN_SCRATCH_BYTES];
+
+#define RN_MAX_DEVICES 64
+std::mutex rn_mutex;
+void *rn_scratch_dev[RN_MAX_DEVICES];
+
+// this device's copy of the scratch array, looked up once per device
+int rn_scratch(void **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return JXAMD_ERROR_DEVICE;
+    if (dev < 0 || dev >= RN_MAX_DEVICES) return JXAMD_ERROR_DEVICE;
+    std::lock_guard<std::mutex> g(rn_mutex);
+    if (!rn_scratch_dev[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return JXAMD_ERROR_DEVICE;
+        rn_scratch_dev[dev] = p;
+    }
+    *out = rn_scratch_dev[dev];
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -166,6 +193,18 @@ int jxamd_bsr_residual_norm(int nb, int b, const int *d_ia, const int *d_ja, con
     if (hipStreamSynchronize(st) != hipSuccess && !e) e = JXAMD_ERROR_DEVICE;
     if (hipFree(work) != hipSuccess && !e) e = JXAMD_ERROR_DEVICE;
     return e;
+}
+
+int jxamd_bsr_residual_norm_noalloc(int nb, int b, const int *d_ia, const int *d_ja, const double *d_a,
+                                    const double *d_x, const double *d_b, double *d_norms, void *stream)
+{
+    int e = rn_check(nb, b, d_ia, d_ja, d_a, d_x, d_b, d_norms);
+    if (e) return e;
+    void *work = nullptr;
+    e = rn_scratch(&work);
+    if (e) return e;
+    return rn_launch(nb, b, d_ia, d_ja, d_a, d_x, d_b, d_norms, (double *)work,
+                     (unsigned int *)((unsigned char *)work + RN_PART_BYTES), (hipStream_t)stream);
 }
 
 } // extern "C"
