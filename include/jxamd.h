@@ -320,8 +320,15 @@ int jxamd_solver_profile_read(jxamd_solver *s, int kind, long *launches, double 
 /* the smoother launches (JXAMD_PROF_GS_SWEEP) of one level: one kernel per
  * level, so this is a per-kernel figure */
 int jxamd_solver_profile_read_level(jxamd_solver *s, int level, long *launches, double *ms, double *bytes);
-/* number of sync-free waits that hit the spin bound since creation (0 when healthy) */
+/* nonzero when a sync-free wait hit its spin bound since creation or since
+ * the last solve that reported it (0 when healthy).  A solve during which a
+ * wait gave up returns JXAMD_ERROR_GENERIC and clears the solver's
+ * synchronisation state (arrival counters, flags, this flag). */
 int jxamd_solver_error_flag(jxamd_solver *s, int *flag);
+/* polls after which a waiting workgroup of the cooperative MGS step gives up
+ * (default 2^24).  No reference counterpart: failure-path testing forces the
+ * give-up from the host with a small limit, so every kernel still exits. */
+int jxamd_solver_set_spin_limit(jxamd_solver *s, int limit);
 
 /* ------------------------------------------------------------------------
  * Standalone device operations on the reference BSR layout (device arrays

@@ -107,6 +107,7 @@ def _declare(L):
         "jxamd_op_vcycle": [vp, vp, vp],
         "jxamd_op_bhgs": [vp, vp, vp, f64, f64],
         "jxamd_solver_error_flag": [vp, vp],
+        "jxamd_solver_set_spin_limit": [vp, i32],
         "jxamd_op_bsr_matvec_timed": [vp, vp, vp, i32, vp],
         "jxamd_amg_params_default": [vp],
         "jxamd_solver_profile": [vp, i32],
@@ -510,6 +511,9 @@ class Solver:
             if n.value:
                 out[l] = (n.value, ms.value, by.value)
         return out
+
+    def set_spin_limit(self, limit):
+        check(self._L.jxamd_solver_set_spin_limit(self._s, int(limit)), "jxamd_solver_set_spin_limit")
 
     def error_flag(self):
         f = ctypes.c_int()

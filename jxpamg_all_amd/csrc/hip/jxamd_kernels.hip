@@ -1299,14 +1299,17 @@ __device__ __forceinline__ double block_sum(double v, double *red)
 // deterministic: per-workgroup partials in part[k * gridDim.x + b], summed in
 // block order by the last workgroup to arrive (counter cnt), which publishes
 // h_k in hval[k] and raises *flag to base + k + 1; the others wait on the
-// flag (bounded: an overflow sets err[0] and the step goes on).
+// flag (bounded by spin_limit polls: an overflow sets err[0] and the step
+// goes on with whatever hval[k] holds; a waiter that gave up can then arrive
+// at the next reduction before the counter is reset, so after such a step
+// the host must clear cnt and flag before the next launch).
 #define MGS_BS 1024 // threads per workgroup of k_mgs_coop: few workgroups, few arrivals per reduction
 
 template <int TM>
 __global__ void __launch_bounds__(MGS_BS) k_mgs_coop(size_t n, const double *__restrict__ V, size_t ld, int m,
                                                      double *__restrict__ w, double *hval, double *part,
                                                      unsigned int *cnt, int *flag, int base, int *err,
-                                                     double *hhost)
+                                                     double *hhost, int spin_limit)
 {
     __shared__ double red[MGS_BS / WAVE];
     __shared__ double hb;
@@ -1371,7 +1374,7 @@ __global__ void __launch_bounds__(MGS_BS) k_mgs_coop(size_t n, const double *__r
         } else if (threadIdx.x == 0) {
             int spins = 0;
             while (!flag_ready(ld_flag(flag), target)) {
-                if (++spins > SPIN_LIMIT) {
+                if (++spins > spin_limit) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
@@ -1959,10 +1962,11 @@ int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm)
 }
 
 int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, void *stream)
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, int spin_limit,
+                     void *stream)
 {
-    if (grid < 1 || m < 0 || (size_t)grid * MGS_BS * (size_t)tm < n) return JXAMD_ERROR_ARG;
-    void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err, &hhost};
+    if (grid < 1 || m < 0 || (size_t)grid * MGS_BS * (size_t)tm < n || spin_limit < 0) return JXAMD_ERROR_ARG;
+    void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err, &hhost, &spin_limit};
     const void *fn;
     switch (tm) {
     case 4: fn = (const void *)k_mgs_coop<4>; break;

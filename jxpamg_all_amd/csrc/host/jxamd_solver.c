@@ -57,7 +57,7 @@ typedef struct {
     double *gval, *dinv, *xs;  /* xs: x^(k) of the K sweeps in slice order, then x^(-1) */
     void *val32, *dinv32;      /* FP32 (stage2_precision 1) or FP16 blocks (2; FP32 dinv) for the partitioned stage 2 */
     float *rsc16;              /* stage2_precision 2: power-of-two scale per block row */
-    double spmv_bytes, sweep_bytes, sweep_bytes32; /* algorithmic bytes: y = A x, one block-HGS sweep (FP64, narrow blocks) */
+    double spmv_bytes, sweep_bytes; /* algorithmic bytes: y = A x, one fused one-rank block-HGS sweep */
     int unit_diag;             /* every diagonal block is exactly the identity: the partitioned sweeps skip Dinv */
     double part_bytes, part_bytes32; /* algorithmic bytes of one partitioned sweep (FP64, narrow blocks) */
 } dbsr;
@@ -121,6 +121,7 @@ struct jxamd_solver_s {
     double *part;
     unsigned int *count;       /* [0] k_dot_mgs, [1] k_mgs_coop arrivals */
     int mgs_grid, mgs_tm, mgs_base; /* cooperative MGS step: grid (0: off), register tile, flag base */
+    int spin_limit;            /* polls before a waiting workgroup of the MGS step gives up */
     double *mpart;             /* its partial sums, (k_dim + 1) * mgs_grid */
     int *mflag;
     unsigned long long *ticket;
@@ -1049,11 +1050,9 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
         }
         d->nsl = h.nsl;
         d->spmv_bytes = (double)A->nnzb * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 16.0 * nb * b;
-        d->sweep_bytes = d->spmv_bytes + 16.0 * nb * b + 8.0 * nb * bb; /* + old x, rhs, Dinv */
-        d->sweep_bytes32 = s->prm.stage2_precision == 2
-                               ? (double)A->nnzb * (2.0 * bb + 4.0) + 4.0 * (nb + 1) + 32.0 * nb * b + 4.0 * nb * bb +
-                                     4.0 * nb
-                               : (double)A->nnzb * (4.0 * bb + 4.0) + 4.0 * (nb + 1) + 32.0 * nb * b + 4.0 * nb * bb;
+        /* one fused one-rank sweep (k_bhgs_multi): off-diagonal blocks and
+         * their columns, row pointers, x read once, rhs read, x written, Dinv */
+        d->sweep_bytes = (double)(A->nnzb - nb) * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 24.0 * nb * b + 8.0 * nb * bb;
         int rc = jxamd_dev_malloc((void **)&d->sptr, sizeof(int) * (h.nsl + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->col, sizeof(int) * (slots * SELLW + 1));
         if (!rc) rc = jxamd_dev_malloc((void **)&d->val, sizeof(double) * (slots * bb * SELLW + 1));
@@ -1117,9 +1116,18 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             for (int r = 0; r < b; r++)
                 for (int c2 = 0; c2 < b; c2++)
                     if (A->a[(size_t)A->ia[i] * bb + r * b + c2] != (r == c2 ? 1.0 : 0.0)) d->unit_diag = 0;
-        /* the partitioned sweeps then read no Dinv */
-        d->part_bytes = d->sweep_bytes - (d->unit_diag ? 8.0 * nb * bb : 0.0);
-        d->part_bytes32 = d->sweep_bytes32 - (d->unit_diag ? 4.0 * nb * bb : 0.0);
+        /* one partitioned sweep (k_bhgs_part) reads every column index of
+         * the SpMV layout, every stored block but the identity diagonal ones
+         * (skipped before their load), row pointers, x once, rhs, writes x,
+         * and reads Dinv unless every diagonal block is the identity */
+        {
+            const double nblk = (double)A->nnzb - (d->unit_diag ? (double)nb : 0.0);
+            const double common = 4.0 * A->nnzb + 4.0 * (nb + 1) + 24.0 * nb * b;
+            const double sv = s->prm.stage2_precision == 2 ? 2.0 : 4.0;
+            d->part_bytes = common + nblk * 8.0 * bb + (d->unit_diag ? 0.0 : 8.0 * nb * bb);
+            d->part_bytes32 = common + nblk * sv * bb + (d->unit_diag ? 0.0 : 4.0 * nb * bb) +
+                              (s->prm.stage2_precision == 2 ? 4.0 * nb : 0.0);
+        }
         /* columns as positions: ~pos for lower neighbours, the own position for padding */
         {
             int *inv = (int *)malloc(sizeof(int) * (size_t)nb);
@@ -1622,6 +1630,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     }
     s->tbase = 0;
     s->epoch = 0;
+    s->spin_limit = JXAMD_SPIN_LIMIT;
     if (e) {
         jxamd_solver_destroy(s);
         return e;
@@ -1672,6 +1681,28 @@ int jxamd_solver_error_flag(jxamd_solver *s, int *flag)
 {
     if (!s || !flag) return JXAMD_ERROR_ARG;
     CK(jxamd_d2h(flag, s->err, sizeof(int), s->stream));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_solver_set_spin_limit(jxamd_solver *s, int limit)
+{
+    if (!s || limit < 0) return JXAMD_ERROR_ARG;
+    s->spin_limit = limit;
+    return 0;
+}
+
+/* After a bounded wait gave up, the reductions' arrival counters and the MGS
+ * flag may hold stale counts; with the stream idle, clear them, the flag
+ * base and the error flag so the next launch starts from a clean state. */
+static int reset_sync_state(jxamd_solver *s)
+{
+    unsigned int zc[4] = {0, 0, 0, 0};
+    int zi[4] = {0, 0, 0, 0};
+    CK(jxamd_stream_sync(s->stream));
+    CK(jxamd_h2d(s->count, zc, sizeof(zc), s->stream));
+    CK(jxamd_h2d(s->err, zi, sizeof(zi), s->stream));
+    if (s->mflag) CK(jxamd_h2d(s->mflag, zi, sizeof(int), s->stream));
+    s->mgs_base = 0;
     return jxamd_stream_sync(s->stream);
 }
 
@@ -2292,7 +2323,7 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
         int pq = prof_begin(s);
         const int e = jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1,
                                        s->mflag, s->mgs_base, s->err,
-                                       s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->stream);
+                                       s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->spin_limit, s->stream);
         if (!e) {
             prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
             s->mgs_base += i + 1;
@@ -2330,10 +2361,16 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     double *c = (double *)calloc(k_dim, sizeof(double));
     double *sn = (double *)calloc(k_dim, sizeof(double));
     double *hcol = (double *)calloc(k_dim + 2, sizeof(double));
-    double **hh = (double **)malloc(sizeof(double *) * (k_dim + 1));
-    if (!rs || !c || !sn || !hcol || !hh) return JXAMD_ERROR_MEMORY;
-    for (int q = 0; q <= k_dim; q++) hh[q] = (double *)calloc(k_dim, sizeof(double));
+    double **hh = (double **)calloc(k_dim + 1, sizeof(double *));
     int err = 0, converged = 0, nprec = 0, nrestart = 0;
+    if (!rs || !c || !sn || !hcol || !hh) err = JXAMD_ERROR_MEMORY;
+    for (int q = 0; q <= k_dim && !err; q++)
+        if (!(hh[q] = (double *)calloc(k_dim, sizeof(double)))) err = JXAMD_ERROR_MEMORY;
+    if (err) {
+        for (int q = 0; hh && q <= k_dim; q++) free(hh[q]);
+        free(hh); free(rs); free(c); free(sn); free(hcol);
+        return err;
+    }
     int i = 0, j, k, iter = 0;
     double t, gamma, r_norm = 0.0, b_norm = 0.0, den_norm, epsilon, r_norm_0 = 0.0;
     double normup, normdown, conv_factor = 0.0;
@@ -2506,7 +2543,12 @@ done:
     }
     if (!err) {
         int ef = 0;
-        if (!jxamd_solver_error_flag(s, &ef) && ef) err = JXAMD_ERROR_GENERIC;
+        if (!jxamd_solver_error_flag(s, &ef) && ef) {
+            /* a bounded wait gave up during this solve: its result is not
+             * trusted, and the synchronisation state is reset for the next */
+            err = JXAMD_ERROR_GENERIC;
+            reset_sync_state(s);
+        }
     }
     for (int q = 0; q <= k_dim; q++) free(hh[q]);
     free(hh); free(rs); free(c); free(sn); free(hcol);

@@ -103,8 +103,9 @@ int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *str
 int jxamd_dot_grid(size_t n) { (void)n; return 1; }
 int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm) { *grid = n > 0 ? 2 : 0; *tm = 4; return 0; }
 int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, void *stream)
-{ (void)grid; (void)tm; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)stream; return 0; }
+                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, int spin_limit,
+                     void *stream)
+{ (void)grid; (void)tm; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)spin_limit; (void)stream; return 0; }
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream)
 { (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }
