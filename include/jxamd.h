@@ -81,6 +81,36 @@ int jxamd_bsr_decouple_abf(int nb, int b, const int *ia, const int *ja, double *
  * jxf_bsr_decoup.c:112-183, and its Gauss-Jordan above 4x4, :200-266, do not
  * return the inverse) and in the rhs for b > 1 */
 int jxamd_bsr_decouple_abf_exact(int nb, int b, const int *ia, const int *ja, double *a, double *rhs);
+/* Decoupling types, JXF_DecoupType (jxfpamg-0220/include/jxf_bsr_decoup.h:13-21) */
+enum {
+    JXAMD_DECOUP_NONE = 0,
+    JXAMD_DECOUP_ABF = 1,     /* D_i^{-1} x row i, jxf_bsr_decoup.c:364-527 */
+    JXAMD_DECOUP_ANL = 2,     /* analytical, :529-588 */
+    JXAMD_DECOUP_SEM = 3,     /* semi-analytical, :591-657 */
+    JXAMD_DECOUP_QI = 4,      /* quasi-IMPES, :659-718 */
+    JXAMD_DECOUP_TIMPES = 5,  /* True-IMPES (column sums), :721-1099 */
+    JXAMD_DECOUP_TIMPES2 = 6  /* True-IMPES, second form, :1102-1262 */
+};
+/* In-place decoupling of a BSR system, jxf_BSRMatrixDecouple(A, rhs, type,
+ * is_thermal) (jxfpamg-0220/csrc/cpr/jxf_bsr_decoup.c:1273-1348), bit for
+ * bit on the host, b = 1..10; rhs may be NULL (left alone) */
+int jxamd_bsr_decouple(int nb, int b, const int *ia, const int *ja, double *a, double *rhs, int type, int is_thermal);
+/* The same on device arrays (b = 2..4), bit for bit with the host routine:
+ * one HBM pass over the blocks (True-IMPES: one more pass for the column
+ * sums, which needs a structurally symmetric block pattern, else
+ * JXAMD_ERROR_ARG with nothing modified).  If d_p_a is not NULL it also
+ * receives the pressure matrix values of the decoupled blocks,
+ * d_p_a[k] = A_k[pidx][pidx] (the pattern is A's: ia, ja), as
+ * jxamd_bsr_extract_pressure_device would, in the same pass.  Synchronous on
+ * `stream`; allocates a work buffer of nb*b doubles for True-IMPES. */
+int jxamd_bsr_decouple_device(int nb, int b, const int *d_ia, const int *d_ja, double *d_a, double *d_rhs, int type,
+                              int is_thermal, int pidx, double *d_p_a, void *stream);
+/* pressure matrix on device arrays: d_p_ia = ia, d_p_ja = ja, d_p_a[k] =
+ * A_k[pidx][pidx] (jxf_BSRMatrixGetSubmatrix, seq_bsr_matrix.c:214-250, as
+ * called by the pressure extraction of par_bsr_matrix.c:327-392); d_p_ia /
+ * d_p_ja may be NULL (not written).  Asynchronous on `stream`. */
+int jxamd_bsr_extract_pressure_device(int nb, int b, int nnzb, const int *d_ia, const int *d_ja, const double *d_a,
+                                      int pidx, int *d_p_ia, int *d_p_ja, double *d_p_a, void *stream);
 /* pressure matrix A_p[i,j] = A_ij[p][p] on A's pattern, jxf_BSRMatrixGetSubmatrix
  * (jxfpamg-0220/csrc/mat_vec_data/seq_bsr_matrix.c:214-250); p_ia[nb+1], p_ja/p_a[nnzb] */
 int jxamd_bsr_extract_pressure(int nb, int b, const int *ia, const int *ja, const double *a, int pidx, int *p_ia,

@@ -89,6 +89,9 @@ def _declare(L):
         "jxamd_gen_reservoir": [vp, vp, vp, vp, vp],
         "jxamd_bsr_decouple_abf": [i32, i32, vp, vp, vp, vp],
         "jxamd_bsr_decouple_abf_exact": [i32, i32, vp, vp, vp, vp],
+        "jxamd_bsr_decouple": [i32, i32, vp, vp, vp, vp, i32, i32],
+        "jxamd_bsr_decouple_device": [i32, i32, vp, vp, vp, vp, i32, i32, i32, vp, vp],
+        "jxamd_bsr_extract_pressure_device": [i32, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp],
         "jxamd_bsr_extract_pressure": [i32, i32, vp, vp, vp, i32, vp, vp, vp],
         "jxamd_bsr_write_binary": [ctypes.c_char_p, i32, i32, i32, i32, vp, vp, vp],
         "jxamd_bsr_read_binary_sizes": [ctypes.c_char_p, vp, vp, vp, vp],
@@ -263,6 +266,27 @@ def decouple_abf(A, rhs=None, exact=False):
     fn = "jxamd_bsr_decouple_abf_exact" if exact else "jxamd_bsr_decouple_abf"
     check(getattr(lib(), fn)(A.nb, A.b, _p(A.ia), _p(A.ja), _p(a), None if r is None else _p(r)), fn)
     return BSRMatrix(A.nb, A.b, A.ia, A.ja, a), r
+
+
+DECOUP_NONE, DECOUP_ABF, DECOUP_ANL, DECOUP_SEM, DECOUP_QI, DECOUP_TIMPES, DECOUP_TIMPES2 = range(7)
+
+
+def decouple(A, rhs=None, dtype=DECOUP_ABF, is_thermal=0):
+    """jxf_BSRMatrixDecouple(A, rhs, dtype, is_thermal) on a copy, bit for bit
+    with the reference (jxf_bsr_decoup.c:1273-1348); returns (A_dec, rhs_dec)."""
+    a = A.a.copy()
+    r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).copy()
+    check(lib().jxamd_bsr_decouple(A.nb, A.b, _p(A.ia), _p(A.ja), _p(a), None if r is None else _p(r), int(dtype),
+                                   int(is_thermal)), "jxamd_bsr_decouple")
+    return BSRMatrix(A.nb, A.b, A.ia, A.ja, a), r
+
+
+def decouple_device(nb, b, d_ia, d_ja, d_a, d_rhs, dtype=DECOUP_ABF, is_thermal=0, pidx=0, d_p_a=None, stream=None):
+    """In-place decoupling of device BSR arrays (raw device addresses); with
+    d_p_a, also the decoupled pressure values.  Returns the error flag (0 ok,
+    4 for a pattern True-IMPES cannot transpose)."""
+    return lib().jxamd_bsr_decouple_device(int(nb), int(b), d_ia, d_ja, d_a, d_rhs, int(dtype), int(is_thermal),
+                                           int(pidx), d_p_a, stream)
 
 
 def extract_pressure(A, pidx=0):

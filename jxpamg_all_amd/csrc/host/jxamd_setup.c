@@ -266,6 +266,194 @@ static int decouple(int nb, int b, const int *ia, double *a, double *rhs, int ex
     return JXAMD_SUCCESS;
 }
 
+/* jxf_smat_inv_nc (jxf_bsr_decoup.c:186-266): the closed forms up to 4x4,
+ * Gauss-Jordan above */
+static void inv_nc(double *a, int n)
+{
+    if (n == 1) a[0] = fabs(a[0]) > 1e-15 ? 1.0 / a[0] : 1.0;
+    else if (n == 2) inv2(a);
+    else if (n == 3) inv3(a);
+    else if (n == 4) inv4(a);
+    else inv_gj(a, n);
+}
+
+/* blk <- M blk, jxf_smat_mul (jxf_bsr_decoup.c:286-296): each entry summed
+ * from zero in column order */
+static void mul_generic(const double *M, double *blk, int b)
+{
+    double t[100];
+    for (int r = 0; r < b; r++)
+        for (int c = 0; c < b; c++) {
+            double acc = 0.0;
+            for (int q = 0; q < b; q++) acc += M[r * b + q] * blk[q * b + c];
+            t[r * b + c] = acc;
+        }
+    memcpy(blk, t, sizeof(double) * b * b);
+}
+
+/* v <- M v in place with aliased input and output, jxf_smat_vec_mul
+ * (jxf_bsr_decoup.c:335-343) called as (M, svec, svec): each component is
+ * cleared before it is accumulated, so it reads its own (now zero) value and
+ * the already updated earlier components */
+static void vec_inplace(const double *M, double *v, int b)
+{
+    for (int r = 0; r < b; r++) {
+        v[r] = 0.0;
+        for (int q = 0; q < b; q++) v[r] += M[r * b + q] * v[q];
+    }
+}
+
+/* ANL, SEM and QI (jxf_bsr_decoup.c:529-588, 591-657, 659-718): a matrix M
+ * built from the row's diagonal block multiplies every block of the row,
+ * the diagonal one included, and the rhs (in place) */
+static int decouple_rows(int nb, int b, const int *ia, double *a, double *rhs, int type, int is_thermal)
+{
+    const int bb = b * b;
+    double M[100], D[100];
+    for (int i = 0; i < nb; i++) {
+        if (ia[i + 1] <= ia[i]) return JXAMD_ERROR_ARG;
+        memcpy(D, a + (size_t)ia[i] * bb, sizeof(double) * bb);
+        if (type == JXAMD_DECOUP_SEM) {
+            memcpy(M, D, sizeof(double) * bb);
+            inv_nc(M, b);
+            if (is_thermal) {
+                M[0] = 1.0;
+                M[b - 1] = 0.0;
+                M[(b - 1) * b] = 0.0;
+                M[bb - 1] = 1.0;
+                for (int l = 0; l < b - 2; l++) {
+                    M[1 + l] = -D[1 + l];
+                    M[(b - 1) * b + 1 + l] = -D[(b - 1) * b + 1 + l];
+                }
+            } else {
+                M[0] = 1.0;
+                for (int l = 0; l < b - 1; l++) M[1 + l] = -D[1 + l];
+            }
+        } else {
+            for (int r = 0; r < b; r++)
+                for (int c = 0; c < b; c++) M[r * b + c] = r == c ? 1.0 : 0.0;
+            const int nl = is_thermal ? b - 2 : b - 1;
+            for (int l = 0; l < nl; l++) {
+                if (type == JXAMD_DECOUP_ANL) {
+                    M[1 + l] = -D[1 + l];
+                    if (is_thermal) M[(b - 1) * b + 1 + l] = -D[(b - 1) * b + 1 + l];
+                } else { /* QI: scaled by the diagonal entry, no guard */
+                    M[1 + l] = -D[1 + l] / D[(l + 1) * b + l + 1];
+                    if (is_thermal) M[(b - 1) * b + 1 + l] = -D[(b - 1) * b + 1 + l] / D[(l + 1) * b + l + 1];
+                }
+            }
+        }
+        for (int k = ia[i]; k < ia[i + 1]; k++) mul_generic(M, a + (size_t)k * bb, b);
+        if (rhs) vec_inplace(M, rhs + (size_t)i * b, b);
+    }
+    return JXAMD_SUCCESS;
+}
+
+/* True-IMPES and its second form (jxf_bsr_decoup.c:721-808, 911-997,
+ * 999-1099, 1102-1262): column sums Ac_c = sum_i A_ic (rows in ascending
+ * order, from zero), then row i's pressure equation is recombined with
+ * coefficients from Ac_i = Ac_ii.  The arithmetic and its quirks follow the
+ * reference per block size: 2x2 uses -a12/a22 (first form, zero when Ac_ii
+ * or a22 is near singular) or -a12/a22 with only the determinant guarded
+ * (second form); 3x3 keeps the first two entries of Ac_ii^{-1}'s first row
+ * for the matrix and all three for the rhs (first form), or adds c0 x row 1
+ * + c1 x row 2 to row 0 (second form); other sizes replace row 0 by
+ * (first row of Ac_ii^{-1}) x block (first form) or add the rest of that row
+ * times rows 1.. (second form, which leaves the rhs unchanged). */
+static int decouple_timpes(int nb, int b, const int *ia, const int *ja, double *a, double *rhs, int second)
+{
+    const int bb = b * b;
+    double *Ac = (double *)calloc((size_t)nb * bb + 1, sizeof(double));
+    double *cf = (double *)calloc((size_t)nb * b + 1, sizeof(double));
+    if (!Ac || !cf) { free(Ac); free(cf); return JXAMD_ERROR_MEMORY; }
+    for (int i = 0; i < nb; i++)
+        for (int k = ia[i]; k < ia[i + 1]; k++) {
+            if (ja[k] < 0 || ja[k] >= nb) { free(Ac); free(cf); return JXAMD_ERROR_ARG; }
+            const double *blk = a + (size_t)k * bb;
+            double *ac = Ac + (size_t)ja[k] * bb;
+            for (int e = 0; e < bb; e++) ac[e] += blk[e];
+        }
+    for (int i = 0; i < nb; i++) {
+        double binv[100];
+        const double *ac = Ac + (size_t)i * bb;
+        double *c = cf + (size_t)i * b;
+        if (b == 2) {
+            const double a11 = ac[0], a12 = ac[1], a21 = ac[2], a22 = ac[3];
+            const double det = a11 * a22 - a12 * a21;
+            if (fabs(det) < 1e-15) c[0] = 0.0;
+            else if (second) c[0] = -a12 / a22;
+            else c[0] = fabs(a22) > 1e-15 ? -a12 / a22 : 0.0;
+        } else {
+            memcpy(binv, ac, sizeof(double) * bb);
+            if (b == 3) inv3(binv);
+            else inv_nc(binv, b);
+            if (!second) for (int k = 0; k < b; k++) c[k] = binv[k];
+            else if (b == 3) { c[0] = binv[0]; c[1] = binv[1]; }
+            else for (int k = 0; k < b - 1; k++) c[k] = binv[k + 1];
+        }
+    }
+    for (int i = 0; i < nb; i++) {
+        const double *c = cf + (size_t)i * b;
+        for (int k = ia[i]; k < ia[i + 1]; k++) {
+            double *blk = a + (size_t)k * bb;
+            if (b == 2) {
+                blk[0] = blk[0] + c[0] * blk[2];
+                blk[1] = blk[1] + c[0] * blk[3];
+            } else if (b == 3 && !second) {
+                const double n0 = c[0] * blk[0] + c[1] * blk[3];
+                const double n1 = c[0] * blk[1] + c[1] * blk[4];
+                const double n2 = c[0] * blk[2] + c[1] * blk[5];
+                blk[0] = n0; blk[1] = n1; blk[2] = n2;
+            } else if (b == 3) {
+                blk[0] = blk[0] + c[0] * blk[3] + c[1] * blk[6];
+                blk[1] = blk[1] + c[0] * blk[4] + c[1] * blk[7];
+                blk[2] = blk[2] + c[0] * blk[5] + c[1] * blk[8];
+            } else if (!second) {
+                double n[10];
+                for (int col = 0; col < b; col++) {
+                    n[col] = 0.0;
+                    for (int q = 0; q < b; q++) n[col] += c[q] * blk[q * b + col];
+                }
+                for (int col = 0; col < b; col++) blk[col] = n[col];
+            } else {
+                for (int col = 0; col < b; col++) {
+                    double sum = blk[col];
+                    for (int q = 0; q < b - 1; q++) sum += c[q] * blk[(q + 1) * b + col];
+                    blk[col] = sum;
+                }
+            }
+        }
+        if (!rhs) continue;
+        double *r = rhs + (size_t)i * b;
+        if (b == 2) r[0] = r[0] + c[0] * r[1];
+        else if (b == 3 && !second) r[0] = c[0] * r[0] + c[1] * r[1] + c[2] * r[2];
+        else if (b == 3) r[0] = r[0] + c[0] * r[1] + c[1] * r[2];
+        else if (!second) {
+            double n0 = 0.0;
+            for (int q = 0; q < b; q++) n0 += c[q] * r[q];
+            r[0] = n0;
+        }
+    }
+    free(Ac);
+    free(cf);
+    return JXAMD_SUCCESS;
+}
+
+int jxamd_bsr_decouple(int nb, int b, const int *ia, const int *ja, double *a, double *rhs, int type, int is_thermal)
+{
+    if (nb < 0 || b < 1 || b > 10 || !ia || !ja || !a) return JXAMD_ERROR_ARG;
+    switch (type) {
+    case JXAMD_DECOUP_NONE: return JXAMD_SUCCESS;
+    case JXAMD_DECOUP_ABF: return decouple(nb, b, ia, a, rhs, 0);
+    case JXAMD_DECOUP_ANL:
+    case JXAMD_DECOUP_SEM:
+    case JXAMD_DECOUP_QI: return decouple_rows(nb, b, ia, a, rhs, type, is_thermal ? 1 : 0);
+    case JXAMD_DECOUP_TIMPES: return decouple_timpes(nb, b, ia, ja, a, rhs, 0);
+    case JXAMD_DECOUP_TIMPES2: return decouple_timpes(nb, b, ia, ja, a, rhs, 1);
+    default: return JXAMD_ERROR_ARG;
+    }
+}
+
 int jxamd_bsr_extract_pressure(int nb, int b, const int *ia, const int *ja, const double *a, int pidx, int *p_ia,
                                int *p_ja, double *p_a)
 {

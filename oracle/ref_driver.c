@@ -12,9 +12,12 @@
  * and intermediate objects are dumped as golden vectors.  Nothing in the
  * product links or runs this program.
  *
- * usage: ref_driver A.bin b.txt out.dump rtol max_iter stage2_maxit decouple(0/1) dump_level [k_dim]
+ * usage: ref_driver A.bin b.txt out.dump rtol max_iter stage2_maxit decouple dump_level [k_dim [is_thermal]]
+ *   decouple: the JXF_DecoupType value (0 none, 1 ABF as the reference
+ *   driver, 2 ANL, 3 SEM, 4 QI, 5 True-IMPES, 6 True-IMPES 2).
  *   dump_level 0: scalars + residual history only; 1: + hierarchy and
- *   single-operator vectors; 2: + decoupled system.
+ *   single-operator vectors; 2: + decoupled system; 3: the decoupled system
+ *   only (no solve).
  */
 #include <math.h>
 #include <stdio.h>
@@ -95,6 +98,7 @@ int main(int argc, char **argv)
     int decouple = atoi(argv[7]);
     int dump_level = atoi(argv[8]);
     int k_dim = argc > 9 ? atoi(argv[9]) : 30;
+    int is_thermal = argc > 10 ? atoi(argv[10]) : 0;
     g_out = fopen(argv[3], "wb");
     if (!g_out) { fprintf(stderr, "cannot open %s\n", argv[3]); return 1; }
 
@@ -117,14 +121,20 @@ int main(int argc, char **argv)
     jxf_SeqVectorInitialize(x_Ser);
     for (int i = 0; i < N; i++) jxf_VectorData(x_Ser)[i] = 0.0;
 
-    /* 2. ABF decoupling on the serial system (src/test_bsr_cprgmres.c:193-212) */
+    /* 2. decoupling on the serial system (src/test_bsr_cprgmres.c:193-212 uses ABF) */
     double t0 = MPI_Wtime();
-    if (decouple) jxf_BSRMatrixDecouple(A_bsr, b_Ser, JXF_DECOUP_ABF, 0);
+    if (decouple) jxf_BSRMatrixDecouple(A_bsr, b_Ser, (JXF_DecoupType)decouple, is_thermal);
     double t_decouple = MPI_Wtime() - t0;
     if (dump_level >= 2) {
         int nnzb = jxf_BSRMatrixNumNonzeros(A_bsr);
         dump_d("dec_a", (long)nnzb * b * b, jxf_BSRMatrixData(A_bsr));
         dump_d("dec_b", N, jxf_VectorData(b_Ser));
+    }
+    if (dump_level == 3) {
+        dump_scalar_d("t_decouple", t_decouple);
+        fclose(g_out);
+        MPI_Finalize();
+        return 0;
     }
 
     /* 3. distribute (src/test_bsr_cprgmres.c:248-258) */

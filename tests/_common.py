@@ -12,11 +12,34 @@ def system(name):
     A, rhs = jx.generate(**gp)
     meta = fixture_meta(d)
     if int(meta["decouple"]):
-        Ad, bd = jx.decouple_abf(A, rhs)
+        Ad, bd = jx.decouple(A, rhs, int(meta["decouple"]))
     else:
         Ad, bd = A, rhs.copy()
     H = jx.Hierarchy.from_fixture(d) if "A0_ia" in d else None
     return d, A, rhs, Ad, bd, H, meta
+
+
+def decoup_cases():
+    """(key, block size, decoupling type, is_thermal) of every run in decoup.npz"""
+    d = load_fixture("decoup")
+    out = []
+    for k in sorted(d):
+        if k.startswith("b") and k.endswith("_a") and "_t" in k:
+            b, t, th = k[:-2].split("_")
+            out.append((k[:-2], int(b[1:]), int(t[1:]), int(th[2:])))
+    return out
+
+
+def decoup_system(d, b):
+    """the undecoupled system of block size b in decoup.npz"""
+    keys = [str(k) for k in d[f"b{b}_gen_keys"]]
+    gp = {k: (float(v) if k in ("sigma_lnk", "corr_len", "channel_ratio", "kz_ratio", "well_boost") else int(v))
+          for k, v in zip(keys, d[f"b{b}_gen_values"])}
+    return jx.generate(**gp)
+
+
+def bits(x):
+    return np.ascontiguousarray(x, dtype=np.float64).view(np.uint64)
 
 
 def relerr(x, y):

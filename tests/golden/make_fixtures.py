@@ -39,7 +39,24 @@ CASES = {
     "c2": (dict(jx.CONFIGS["C2"]), 1e-6, 100, 3, 1, 0, 30),
     # lvl3 with OMP_NUM_THREADS=4: the threaded relax-6 (par_relax_6.c) golden vectors
     "lvl3_t4": (dict(jx.CONFIGS["C1"], nx=20, ny=20, nz=10, seed=3), 1e-6, 100, 3, 1, 1, 30),
+    # True-IMPES decoupling (type 5) before the solve, as the reference's hybrid drivers use it
+    "timpes": (dict(jx.CONFIGS["C1"], nx=10, ny=10, nz=6, seed=8), 1e-6, 100, 3, 5, 2, 30),
+    # the full 60x220x85 system decoupled by ABF and by True-IMPES (SHA-256 of
+    # the decoupled system only; the reference's True-IMPES solve of this
+    # system makes no progress: 100 iterations, relative residual 0.9997)
+    "c2_dec_abf": (dict(jx.CONFIGS["C2"]), 1e-6, 100, 3, 1, 3, 30),
+    "c2_dec_timpes": (dict(jx.CONFIGS["C2"]), 1e-6, 100, 3, 5, 3, 30),
 }
+# decoupling-only fixture (decoup.npz): every type on small systems of each
+# block size, the thermal variants of ANL/SEM/QI, and True-IMPES on a block
+# pattern that is not structurally symmetric
+DECOUP_GRIDS = {
+    2: dict(jx.CONFIGS["C1"], nx=4, ny=4, nz=3, block_size=2, seed=31),
+    3: dict(jx.CONFIGS["C1"], nx=4, ny=4, nz=3, block_size=3, seed=32),
+    4: dict(jx.CONFIGS["C1"], nx=4, ny=3, nz=3, block_size=4, field=jx.FIELD_SMOOTH, sigma_lnk=2.0, corr_len=2.0,
+            seed=33),
+}
+DECOUP_RUNS = [(b, t, th) for b in (2, 3, 4) for t in range(1, 7) for th in ((0, 1) if t in (2, 3, 4) else (0,))]
 # OpenMP threads of the reference run (1 unless listed)
 THREADS = {"lvl3_t4": 4}
 # operator vectors not kept for these cases (the hierarchy and the solve are)
@@ -74,6 +91,8 @@ def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference first: make -C oracle ref")
     for name, (gp, rtol, max_iter, s2, dec, lvl, kdim) in CASES.items():
+        if name == "decoup":
+            continue
         if only and name not in only:
             continue
         A, rhs = jx.generate(**gp)
@@ -93,6 +112,8 @@ def main():
             rec = {k: v for k, v in rec.items() if not k.startswith(OPERATOR_KEYS)}
         if "dec_a" in rec and A.nnzb * A.b * A.b > 50000:
             rec["dec_a_sha256"] = np.frombuffer(sha(rec.pop("dec_a")).encode(), dtype=np.uint8)
+        if "dec_b" in rec and A.nb * A.b > 50000:
+            rec["dec_b_sha256"] = np.frombuffer(sha(rec.pop("dec_b")).encode(), dtype=np.uint8)
         rec["gen_" + "keys"] = np.array(list(gp.keys()))
         rec["gen_values"] = np.array([float(v) for v in gp.values()])
         rec["meta_keys"] = np.array(list(meta.keys()))
@@ -100,8 +121,54 @@ def main():
         rec["input_a_sha256"] = np.frombuffer(sha(A.a).encode(), dtype=np.uint8)
         rec["input_rhs_sha256"] = np.frombuffer(sha(rhs).encode(), dtype=np.uint8)
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **rec)
-        print(name, r.stderr.strip().splitlines()[-1])
+        print(name, (r.stderr.strip().splitlines() or ["decoupled"])[-1])
+
+
+def asymmetric(A, rng):
+    """A with about a fifth of its off-diagonal blocks dropped (the diagonal
+    block stays first), so the block pattern is not structurally symmetric"""
+    ia, ja, a, bb = [0], [], [], A.b * A.b
+    for i in range(A.nb):
+        for k in range(A.ia[i], A.ia[i + 1]):
+            if k == A.ia[i] or rng.random() > 0.2:
+                ja.append(A.ja[k])
+                a.append(A.a[k * bb:(k + 1) * bb])
+        ia.append(len(ja))
+    return jx.BSRMatrix(A.nb, A.b, np.array(ia), np.array(ja), np.concatenate(a))
+
+
+def run_decouple(A, rhs, dtype, thermal):
+    tmp = tempfile.mkdtemp()
+    jx.write_bsr_binary(os.path.join(tmp, "A.bin"), A)
+    jx.write_rhs_text(os.path.join(tmp, "b.rhs"), rhs)
+    dump = os.path.join(tmp, "o.dump")
+    subprocess.run([DRIVER, os.path.join(tmp, "A.bin"), os.path.join(tmp, "b.rhs"), dump, "1e-6", "1", "1",
+                    str(dtype), "3", "30", str(thermal)], capture_output=True, text=True,
+                   env=dict(os.environ, OMP_NUM_THREADS="1"), check=True)
+    d = read_dump(dump)
+    return d["dec_a"], d["dec_b"]
+
+
+def decoup_fixture():
+    rec = {}
+    for b, gp in DECOUP_GRIDS.items():
+        rec[f"b{b}_gen_keys"] = np.array(list(gp.keys()))
+        rec[f"b{b}_gen_values"] = np.array([float(v) for v in gp.values()])
+    for b, t, th in DECOUP_RUNS:
+        A, rhs = jx.generate(**DECOUP_GRIDS[b])
+        rec[f"b{b}_t{t}_th{th}_a"], rec[f"b{b}_t{t}_th{th}_b"] = run_decouple(A, rhs, t, th)
+    # True-IMPES on a pattern that is not structurally symmetric (host path only)
+    A, rhs = jx.generate(**DECOUP_GRIDS[3])
+    As = asymmetric(A, np.random.default_rng(7))
+    rec["asym_ia"], rec["asym_ja"], rec["asym_a"] = As.ia, As.ja, As.a
+    for t in (5, 6):
+        rec[f"asym_t{t}_a"], rec[f"asym_t{t}_b"] = run_decouple(As, rhs, t, 0)
+    np.savez_compressed(os.path.join(OUT, "decoup.npz"), **rec)
+    print("decoup", len(DECOUP_RUNS) + 2, "runs")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["decoup"]:
+        decoup_fixture()
+    else:
+        main()

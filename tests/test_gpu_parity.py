@@ -49,7 +49,7 @@ def test_bsr_matvec(name):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["c1", "tiny", "b4", "b2", "lvl3", "nodec"])
+@pytest.mark.parametrize("name", ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "timpes"])
 def test_smoothers_and_cycle_vs_reference(name):
     d, Ad, bd, H, meta, s = _solver(name)
     u = dev(d["hsgs_u_in"])
@@ -71,7 +71,7 @@ def test_smoothers_and_cycle_vs_reference(name):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "restart", "layered"])
+@pytest.mark.parametrize("name", ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "restart", "layered", "timpes"])
 def test_full_solve_vs_reference(name):
     d, Ad, bd, H, meta, s = _solver(name)
     x, st, hist = s.solve(bd)

@@ -9,10 +9,10 @@ import numpy as np
 import pytest
 
 import jxpamg_all_amd as jx
-from tests._common import system
+from tests._common import bits, decoup_cases, decoup_system, system
 from tests.conftest import ROOT, fixture_gen_params, load_fixture
 
-ALL = ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "restart", "layered"]
+ALL = ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "restart", "layered", "timpes"]
 
 
 def _sha(a):
@@ -48,11 +48,13 @@ def test_generator_structure(b):
     assert np.all(np.abs(rhs) < 1.0)
 
 
-@pytest.mark.parametrize("name", ["tiny", "b4", "b2"])
+@pytest.mark.parametrize("name", ["tiny", "b4", "b2", "timpes"])
 def test_abf_decoupling_bit_exact(name):
     d, A, rhs, Ad, bd, H, meta = system(name)
     assert np.array_equal(Ad.a, d["dec_a"])
     assert np.array_equal(bd, d["dec_b"])
+    if name == "timpes":  # True-IMPES keeps the diagonal blocks' other rows
+        return
     # diagonal blocks become the identity
     for i in range(Ad.nb):
         assert np.array_equal(Ad.a[Ad.ia[i] * Ad.b ** 2:(Ad.ia[i] + 1) * Ad.b ** 2], np.eye(Ad.b).ravel())
@@ -144,6 +146,63 @@ def test_abf_exact_matches_reference_matrix_for_3x3():
     b = A.to_scipy() @ x
     _, bx = jx.decouple_abf(A, b, exact=True)
     assert np.allclose(Ae.to_scipy() @ x, bx, rtol=1e-10, atol=1e-10 * np.abs(bx).max())
+
+
+@pytest.mark.parametrize("key,b,dtype,thermal", decoup_cases())
+def test_every_decoupling_type_bit_exact(key, b, dtype, thermal):
+    """jxf_BSRMatrixDecouple of every type (ABF, ANL, SEM, QI, True-IMPES and
+    its second form; thermal variants) against the reference build, bit for
+    bit including signed zeros"""
+    d = load_fixture("decoup")
+    A, rhs = decoup_system(d, b)
+    Ad, bd = jx.decouple(A, rhs, dtype, thermal)
+    assert np.array_equal(bits(Ad.a), bits(d[key + "_a"]))
+    assert np.array_equal(bits(bd), bits(d[key + "_b"]))
+    if dtype == jx.DECOUP_ABF:
+        Aa, ba = jx.decouple_abf(A, rhs)
+        assert np.array_equal(bits(Aa.a), bits(Ad.a)) and np.array_equal(bits(ba), bits(bd))
+
+
+@pytest.mark.parametrize("dtype", [jx.DECOUP_TIMPES, jx.DECOUP_TIMPES2])
+def test_true_impes_on_unsymmetric_pattern_bit_exact(dtype):
+    d = load_fixture("decoup")
+    _, rhs = decoup_system(d, 3)
+    As = jx.BSRMatrix(len(d["asym_ia"]) - 1, 3, d["asym_ia"], d["asym_ja"], d["asym_a"])
+    Ad, bd = jx.decouple(As, rhs, dtype)
+    assert np.array_equal(bits(Ad.a), bits(d[f"asym_t{dtype}_a"]))
+    assert np.array_equal(bits(bd), bits(d[f"asym_t{dtype}_b"]))
+
+
+def test_true_impes_column_sums_closed_form():
+    """True-IMPES 3x3 recombines row 0 of every block of row i with the first
+    two entries of the first row of (sum over rows of column block i)^{-1};
+    restated here with numpy on a tiny system (a dropped column term or a
+    transposed sum fails it)"""
+    A, rhs = jx.generate(nx=3, ny=2, nz=2, block_size=3, seed=12)
+    Ad, bd = jx.decouple(A, rhs, jx.DECOUP_TIMPES)
+    blocks = A.a.reshape(-1, 3, 3)
+    Ac = np.zeros((A.nb, 3, 3))
+    for i in range(A.nb):
+        for k in range(A.ia[i], A.ia[i + 1]):
+            Ac[A.ja[k]] += blocks[k]
+    for i in range(A.nb):
+        w = np.linalg.inv(Ac[i])[0]
+        for k in range(A.ia[i], A.ia[i + 1]):
+            got = Ad.a.reshape(-1, 3, 3)[k]
+            assert np.allclose(got[0], w[0] * blocks[k][0] + w[1] * blocks[k][1], rtol=1e-12, atol=1e-14)
+            assert np.array_equal(got[1:], blocks[k][1:])
+        assert np.isclose(bd[3 * i], w @ rhs[3 * i:3 * i + 3], rtol=1e-12)
+        assert np.array_equal(bd[3 * i + 1:3 * i + 3], rhs[3 * i + 1:3 * i + 3])
+
+
+@pytest.mark.parametrize("name,dtype", [("c2_dec_abf", 1), ("c2_dec_timpes", 5)])
+def test_decoupled_c2_hash(name, dtype):
+    """the full 60x220x85 system decoupled on the host = the reference's (SHA-256)"""
+    d = load_fixture(name)
+    A, rhs = jx.generate(**fixture_gen_params(d))
+    Ad, bd = jx.decouple(A, rhs, dtype)
+    assert _sha(Ad.a) == bytes(d["dec_a_sha256"]).decode()
+    assert _sha(bd) == bytes(d["dec_b_sha256"]).decode()
 
 
 def test_pressure_extraction_matches_reference():
@@ -256,7 +315,7 @@ def test_amg_setup_rejects_diagonal_not_first():
 @pytest.mark.parametrize("name", ["c1", "b4", "b2", "nodec"])
 def test_bilu0_factor_matches_oracle(name):
     from oracle.cpu_ref import Oracle
-    from tests._common import system
+    from tests._common import bits, decoup_cases, decoup_system, system
     d, A, rhs, Ad, bd, H, meta = system(name)
     lu, dinv = jx.bilu0_factor(Ad)
     lu_o, dinv_o = Oracle(Ad).bilu0()

@@ -14,7 +14,7 @@ import scipy.sparse.linalg as spla
 from oracle.cpu_ref import Oracle
 from tests._common import relerr, system
 
-OP_CASES = ["c1", "tiny", "b4", "b2", "lvl3", "nodec"]
+OP_CASES = ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "timpes"]
 
 
 @pytest.mark.parametrize("name", OP_CASES)
@@ -31,7 +31,7 @@ def test_operators_bit_exact_vs_reference(name):
     assert np.array_equal(o.cpr(d["cpr_r"], p), d["cpr_x"])
 
 
-@pytest.mark.parametrize("name", ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "restart", "layered"])
+@pytest.mark.parametrize("name", ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "restart", "layered", "timpes"])
 def test_gmres_matches_reference(name):
     d, A, rhs, Ad, bd, H, meta = system(name)
     if H is None:
