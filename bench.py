@@ -11,9 +11,12 @@ setup, upload) is reported separately.
 
 Prints one JSON line (rank 0).  Also measured live on the solver's stream:
 per-kernel-class device time and algorithmic bytes (roofline of the
-dominant kernel) and the BSR SpMV bandwidth.  The CPU baseline is the oracle
-(a bit-exact single-thread restatement of the reference) on a bounded
-sample of the same solve.
+dominant kernel), the BSR SpMV bandwidth (3x3 here, 4x4 on the 100^3
+system), the reference-exact configuration (one-thread relax 6, one-rank
+stage 2), BILU(0) stage 2, and the decoupling kernels.  The CPU baseline is
+the reference's own whole solve of the same system (its library built from
+its sources, one thread and all usable cores), or the oracle's
+restatement where that build is absent.
 """
 import argparse
 import json
@@ -34,7 +37,6 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="C2")
-    ap.add_argument("--cpu-iters", type=int, default=2, help="GMRES iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-kind", choices=("reference", "port"), default="reference",
                     help="CPU baseline: the reference built from its sources (oracle/_ref/ref_driver, when built) "
@@ -49,6 +51,10 @@ def parse():
                     help="block rows per stage-2 partition (0: one-rank sweeps); default: the package's value")
     ap.add_argument("--relax-chunk", type=int, default=None,
                     help="rows per relax-6 chunk (0: one-thread sweeps); default: the package's large-workload value")
+    ap.add_argument("--arnoldi-kernel", type=int, default=0,
+                    help="0: one cooperative launch per MGS step, 1: one launch per projection")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the subsystem legs (reference-exact, BILU(0), 4x4 SpMV, decoupling)")
     return ap.parse_args()
 
 
@@ -67,11 +73,29 @@ def read_traffic(key):
         return None
 
 
-def reference_cpu_baseline(A, rhs, iters, n_apply):
-    """The reference's own solve phase (oracle/_ref/ref_driver: jxf_ library
-    compiled from the reference sources, FP64, one thread) on the same system
-    for `iters` GMRES iterations, scaled to n_apply CPR applications; None
-    when the driver is not built or fails."""
+def host_cpu():
+    """(CPU model, logical CPUs, CPUs this process may run on)"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def reference_cpu_solve(A, rhs, threads):
+    """The reference's own whole solve phase (oracle/_ref/ref_driver: its jxf_
+    library compiled from the reference sources, FP64) on the same raw
+    system, ABF-decoupled by the reference itself, CPR-GMRES(30) to rtol 1e-6
+    with OMP_NUM_THREADS=threads; None when the build is absent or fails."""
     import re
     import subprocess
     import tempfile
@@ -84,24 +108,92 @@ def reference_cpu_baseline(A, rhs, iters, n_apply):
         fa, fb, fo = (os.path.join(tmp, n) for n in ("A.bin", "b.rhs", "out.dump"))
         jx.write_bsr_binary(fa, A)
         jx.write_rhs_text(fb, rhs)
-        env = dict(os.environ, OMP_NUM_THREADS="1")
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         try:
-            r = subprocess.run([exe, fa, fb, fo, "1e-6", str(iters), "3", "1", "0", "30"], capture_output=True,
-                               text=True, timeout=600, env=env)
+            r = subprocess.run([exe, fa, fb, fo, "1e-6", "100", "3", "1", "0", "30"], capture_output=True,
+                               text=True, timeout=900, env=env)
         except (OSError, subprocess.TimeoutExpired):
             return None
-    m = re.search(r"ORACLE iters=(\d+) .* solve=([0-9.]+)s setup=([0-9.]+)s", r.stderr)
+    m = re.search(r"ORACLE iters=(\d+) rel=\S+ true_rel=(\S+) solve=([0-9.]+)s setup=([0-9.]+)s", r.stderr)
     if r.returncode != 0 or not m:
         return None
-    t_solve, applies = float(m.group(2)), int(m.group(1)) + 1
-    cpu = {"value": round(t_solve / applies * n_apply, 2), "unit": "s", "cores": 1, "kind": "reference",
-           "sample": f"{m.group(1)} GMRES iterations ({applies} CPR applications, {t_solve:.1f} s of "
-                     f"JXF_GMRESSolve) of the same system with the reference's FP64 jxf_ library built from its "
-                     f"sources (oracle/_ref/ref_driver, OMP_NUM_THREADS=1; its host setup {float(m.group(3)):.1f} s "
-                     f"not counted), scaled to the {n_apply} CPR applications the reference makes on the full "
-                     f"solve"}
-    log(f"cpu baseline {cpu}")
-    return cpu
+    out = {"s": float(m.group(3)), "iterations": int(m.group(1)), "true_rel_res": float(m.group(2)),
+           "threads": threads, "host_setup_s": float(m.group(4))}
+    log(f"reference CPU solve, {threads} thread(s): {out}")
+    return out
+
+
+def cpu_baseline(A, rhs, Ad, bd, H, kind):
+    """Whole C2 solves on the host: the reference build with one thread and
+    with every usable core (capped at 16); the oracle restatement (one
+    thread) when the reference build is absent."""
+    model, ncpu, usable = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(usable, 16)
+    sample = "the whole solve phase (JXF_GMRESSolve from x0 = 0 to rtol 1e-6; host setup not counted)"
+    if kind == "reference":
+        one = reference_cpu_solve(A, rhs, 1)
+        if one is not None:
+            allc = reference_cpu_solve(A, rhs, threads) if threads > 1 else None
+            best = allc if allc is not None and allc["s"] < one["s"] else one
+            return {"value": round(best["s"], 2), "unit": "s", "cores": best["threads"], "kind": "reference",
+                    "sample": f"{sample} of the same 60x220x85 system with the reference's FP64 jxf_ library built "
+                              f"from its sources (oracle/_ref/ref_driver), OMP_NUM_THREADS={best['threads']}",
+                    "cpu_model": model, "nproc": ncpu, "usable_cpus": usable,
+                    "one_thread": one, "all_threads": allc}
+    import numpy as np
+    from oracle.cpu_ref import Oracle
+    o = Oracle(Ad, H)
+    t0 = time.time()
+    xo, sto, histo, rc = o.gmres(bd, o.params(tol=1e-6, max_iter=100))
+    t_cpu = time.time() - t0
+    true = float(np.linalg.norm(bd - Ad.to_scipy() @ xo) / np.linalg.norm(bd))
+    log(f"oracle CPU solve: {t_cpu:.1f} s, {sto.iterations} iterations")
+    return {"value": round(t_cpu, 2), "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"{sample} of the same system with the oracle (single-thread restatement of the reference, "
+                      f"pinned to it bit for bit): {sto.iterations} iterations, true rel. residual {true:.2e}",
+            "cpu_model": model, "nproc": ncpu, "usable_cpus": usable}
+
+
+def decouple_legs(A, rhs):
+    """ABF and True-IMPES decoupling of the raw system on the GPU (in place,
+    with the pressure matrix values in the same pass), wall time of the
+    synchronous entry point over the algorithmic bytes: matrix values read
+    and written once, row pointers, rhs read and written, pressure values
+    written; True-IMPES adds its column-sum pass (values, column indices,
+    row pointers read once, nb*b coefficients written and read)."""
+    import numpy as np
+    import torch
+
+    import jxpamg_all_amd as jx
+    ia = torch.tensor(A.ia, dtype=torch.int32, device="cuda")
+    ja = torch.tensor(A.ja, dtype=torch.int32, device="cuda")
+    a0 = torch.tensor(A.a, dtype=torch.float64, device="cuda")
+    r0 = torch.tensor(rhs, dtype=torch.float64, device="cuda")
+    a, r = a0.clone(), r0.clone()
+    pa = torch.empty(A.nnzb, dtype=torch.float64, device="cuda")
+    nnzb, nb, N, bb = A.nnzb, A.nb, A.n, A.b * A.b
+    apply_bytes = 2 * 8 * nnzb * bb + 4 * (nb + 1) + 16 * N + 8 * nnzb
+    coef_bytes = 8 * nnzb * bb + 4 * nnzb + 4 * (nb + 1) + 16 * N
+    out = {}
+    for name, dtype, by in (("abf", jx.DECOUP_ABF, apply_bytes), ("true_impes", jx.DECOUP_TIMPES,
+                                                                  apply_bytes + coef_bytes)):
+        ts = []
+        for rep in range(4):
+            a.copy_(a0)
+            r.copy_(r0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rc = jx.decouple_device(nb, A.b, ia.data_ptr(), ja.data_ptr(), a.data_ptr(), r.data_ptr(), dtype, 0, 0,
+                                    pa.data_ptr())
+            torch.cuda.synchronize()
+            if rep:
+                ts.append(time.perf_counter() - t0)
+            assert rc == 0, f"device decoupling {name} failed: {rc}"
+        t = float(np.median(ts))
+        out[name] = {"ms": round(t * 1e3, 3), "algorithmic_bytes": int(by), "GBs": round(by / t / 1e9, 1),
+                     "frac": round(by / t / 1e9 / PEAK_HBM_GBS, 4)}
+        log(f"decoupling on the GPU, {name}: {out[name]}")
+    return out
 
 
 def main():
@@ -141,7 +233,8 @@ def main():
         f"gen+decouple {t_gen:.1f}s amg {t_amg:.1f}s")
     chunk = jx.BENCH_RELAX_CHUNK if args.relax_chunk is None else args.relax_chunk
     s2chunk = jx.BENCH_STAGE2_CHUNK if args.stage2_chunk is None else args.stage2_chunk
-    params = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_chunk=s2chunk)
+    params = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_chunk=s2chunk,
+                               arnoldi_kernel=args.arnoldi_kernel)
     t0 = time.time()
     s = jx.Solver(Ad, H, params)
     t_create = time.time() - t0
@@ -166,17 +259,19 @@ def main():
         log(f"step {k}: {times[-1] * 1e3:.1f} ms iters {st.iterations} true_rel {st.true_rel_res:.3e}")
     sec = float(np.mean(times))
 
-    def timed_solves(solver, n):
+    def timed_solves(solver, n, name=""):
         ts, stx = [], None
         x_d.zero_()
         solver.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
-        for _ in range(n):
+        for k in range(n):
             x_d.zero_()
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             stx = solver.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t1)
+            if name:
+                log(f"{name} solve {k}: {ts[-1] * 1e3:.1f} ms")
         return float(np.mean(ts)), stx
 
     mixed = None
@@ -224,13 +319,49 @@ def main():
             sm.close()
         legs["training_s"] = round(t_train, 2) if net is not None else None
         mixed = legs
-    # the plain path with the one-rank (sequential-order) stage 2, for reference
-    pe = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_chunk=0)
-    se = jx.Solver(Ad, H, pe)
-    t_e, st_e = timed_solves(se, args.steps)
-    se.close()
-    exact_stage2 = {"s": round(t_e, 4), "iterations": st_e.iterations, "true_rel_res": st_e.true_rel_res}
-    log(f"plain FP64, one-rank stage 2: {t_e * 1e3:.1f} ms iters {st_e.iterations}")
+    def profiled(solver):
+        solver.profile(True)
+        x_d.zero_()
+        solver.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
+        pr, lv = solver.profile_read(), solver.profile_read_levels()
+        solver.profile(False)
+        return pr, lv
+
+    def gbs(v):
+        return round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 else None
+
+    extra = None
+    if not args.no_extra:
+        extra = {}
+        # the reference's own one-thread semantics: sequential relax-6 sweeps
+        # (k_gs_sweep) and the one-rank stage 2 (k_bhgs_multi)
+        pe = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=0, stage2_chunk=0)
+        se = jx.Solver(Ad, H, pe)
+        log("reference-exact solver created")
+        t_e, st_e = timed_solves(se, 1, "reference-exact")
+        pr, lv = profiled(se)
+        log("reference-exact profiled solve done")
+        se.close()
+        extra["reference_exact"] = {
+            "s": round(t_e, 4), "iterations": st_e.iterations, "true_rel_res": st_e.true_rel_res,
+            "k_gs_sweep_per_level": {f"level {l}": {"launches": v[0], "ms": round(v[1], 3), "GBs": gbs(v)}
+                                     for l, v in lv.items()},
+            "k_bhgs_multi": {"launches": pr["bhgs_sweep"][0], "ms": round(pr["bhgs_sweep"][1], 3),
+                             "GBs": gbs(pr["bhgs_sweep"])}}
+        log(f"reference-exact (one-thread relax 6, one-rank stage 2): {t_e * 1e3:.1f} ms iters {st_e.iterations}")
+        # stage 2 as BILU(0) corrections with level-scheduled block triangular solves (k_btri)
+        pb = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_solver_type=5)
+        sb = jx.Solver(Ad, H, pb)
+        t_b, st_b = timed_solves(sb, args.steps, "BILU(0) stage 2")
+        pr, _ = profiled(sb)
+        sb.close()
+        extra["bilu0_stage2"] = {"s": round(t_b, 4), "iterations": st_b.iterations, "true_rel_res": st_b.true_rel_res,
+                                 "k_btri": {"launches": pr["bilu_solve"][0], "ms": round(pr["bilu_solve"][1], 3),
+                                            "GBs": gbs(pr["bilu_solve"]),
+                                            "frac": round((gbs(pr["bilu_solve"]) or 0) / PEAK_HBM_GBS, 4)}}
+        log(f"BILU(0) stage 2: {t_b * 1e3:.1f} ms iters {st_b.iterations} k_btri {extra['bilu0_stage2']['k_btri']}")
+        extra["decoupling"] = decouple_legs(A, rhs)
+    iters = st.iterations
     iters = st.iterations
     assert st.converged == 1 and st.true_rel_res <= 1e-6, "solve did not reach rtol 1e-6"
 
@@ -286,27 +417,27 @@ def main():
             "frac": round(spmv_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": read_traffic("k_bsr_spmv<3, 8>")}
     log(f"BSR SpMV {spmv}")
     assert s.error_flag() == 0
+    if extra is not None:
+        # 4x4 BSR SpMV on the 100^3 system (exact ABF decoupling, its own hierarchy)
+        A4, r4 = jx.generate(**jx.CONFIGS["C4"])
+        A4d, _ = jx.decouple_abf(A4, r4, exact=True)
+        del A4, r4
+        H4 = jx.amg_setup(*jx.extract_pressure(A4d))
+        s4 = jx.Solver(A4d, H4, jx.default_params(tol=1e-8, max_iter=100, k_dim=30, relax_chunk=chunk,
+                                                  stage2_chunk=s2chunk))
+        x4 = torch.ones(A4d.n, dtype=torch.float64, device="cuda")
+        y4 = torch.empty_like(x4)
+        s4.bsr_matvec_timed(x4.data_ptr(), y4.data_ptr(), 5)
+        ms4 = s4.bsr_matvec_timed(x4.data_ptr(), y4.data_ptr(), args.spmv_reps)
+        by4 = A4d.nnzb * (8 * 16 + 4) + 4 * (A4d.nb + 1) + 16 * A4d.n
+        extra["spmv_4x4_100cubed"] = {"kernel": "k_bsr_spmv<4, 8>", "mean_launch_us": round(ms4 * 1e3, 2),
+                                      "algorithmic_bytes": by4, "achieved_GBs": round(by4 / (ms4 * 1e-3) / 1e9, 1),
+                                      "frac": round(by4 / (ms4 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        log(f"BSR SpMV 4x4 {extra['spmv_4x4_100cubed']}")
+        s4.close()
+        del A4d, H4, x4, y4
 
-    # CPR applications the reference makes on this solve: one per iteration
-    # and one for the update of x at the end of each cycle
-    ref_applies = st.iterations + st.num_restarts + 1
-    cpu = None
-    if not args.no_cpu and args.cpu_kind == "reference":
-        cpu = reference_cpu_baseline(A, rhs, args.cpu_iters, ref_applies)
-    if not args.no_cpu and cpu is None:
-        from oracle.cpu_ref import Oracle
-        o = Oracle(Ad, H)
-        p = o.params(tol=1e-6, max_iter=args.cpu_iters)  # one-thread reference semantics
-        t0 = time.time()
-        xo, sto, histo, rc = o.gmres(bd, p)
-        t_cpu = time.time() - t0
-        applies = sto.num_precond
-        est = t_cpu / applies * ref_applies
-        cpu = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_iters} GMRES iterations ({applies} CPR applications, {t_cpu:.1f} s) of the "
-                         f"same solve with the oracle (bit-exact single-thread restatement of the reference), "
-                         f"scaled to the {ref_applies} CPR applications of the full solve"}
-        log(f"cpu baseline {cpu}")
+    cpu = None if args.no_cpu else cpu_baseline(A, rhs, Ad, bd, H, args.cpu_kind)
 
     out = {"metric": METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
@@ -320,7 +451,7 @@ def main():
                                        "upload": round(t_create, 2)}},
            "roofline": roofline, "spmv": spmv, "kernel_classes": breakdown, "smoother_levels": smoother_levels,
            "cpu_baseline": cpu,
-           "mixed_precision_learned": mixed, "plain_fp64_one_rank_stage2": exact_stage2}
+           "mixed_precision_learned": mixed, "subsystems": extra}
     s.close()
     print(json.dumps(out), flush=True)
 

@@ -220,6 +220,10 @@ typedef struct {
      * application per cycle).  For a linear FP64 preconditioner the two agree
      * to rounding. */
     int solution_update;
+    /* GMRES stops when one step's residual estimate grows by more than this
+     * factor, JXF_GMRESSetConvFacThreshold (pgmres.c:201, default 1e4 at
+     * :449, test at :1325-1333) */
+    double convfac_threshold;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread
@@ -237,6 +241,9 @@ typedef struct {
     double t_solve_ms;           /* device time of the whole solve (HIP events) */
     double t_precond_ms;         /* device time inside CPR applications, measured when print_level > 0
                                   * or profiling is on (each application then ends with a sync); else 0 */
+    int num_false_conv;          /* residual estimates at the tolerance whose true residual was not
+                                  * ("false convergence", pgmres.c:1200-1241, 1583-1608) */
+    int diverged;                /* stopped by the convergence-factor test (pgmres.c:1325-1333) */
 } jxamd_stats;
 
 typedef struct jxamd_solver_s jxamd_solver;
@@ -254,6 +261,20 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *d_b, double *d_x, d
                               jxamd_stats *st);
 /* x = M^{-1} r, device vectors.  Replaces JXF_CPRPrecond (jxf_cpr.c:424-610). */
 int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x);
+/* The CPR application as a preconditioner callback of the reference's GMRES:
+ * the shape of JXF_PtrToSolverFcn, int (*)(void *precond_data, void *A,
+ * void *b, void *x) (registered by JXF_GMRESSetPrecond, pgmres.c:267-281;
+ * called at pgmres.c:1258, 1400, 1444, 1500, 1538).  precond_data points to a
+ * jxamd_precond_ctx; b and x are the caller's vector objects, whose host
+ * arrays (nb*b doubles) vector_data returns (for the reference:
+ * jxf_VectorData(jxf_ParVectorLocalVector(v))).  A is ignored, as
+ * JXF_CPRPrecond ignores its matrix argument (jxf_cpr.c:427).  x = M^{-1} b
+ * through HBM (two PCIe copies); returns 0 or an error flag. */
+typedef struct {
+    jxamd_solver *solver;
+    double *(*vector_data)(void *vec);
+} jxamd_precond_ctx;
+int jxamd_cpr_precond_callback(void *precond_data, void *A, void *b, void *x);
 
 /* The CSR-array solve entry with the arguments of JXFPAMG_Solver_Interface
  * (src/jxpamg_interface.c:118-257) minus the MPI communicator (one GPU, one

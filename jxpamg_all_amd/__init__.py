@@ -58,14 +58,15 @@ class Params(ctypes.Structure):
                 ("stage2_omega", ctypes.c_double), ("relax_threads", ctypes.c_int), ("relax_chunk", ctypes.c_int),
                 ("precond_precision", ctypes.c_int), ("poly_degree", ctypes.c_int), ("stage2_chunk", ctypes.c_int),
                 ("stage2_precision", ctypes.c_int), ("arnoldi_kernel", ctypes.c_int),
-                ("solution_update", ctypes.c_int)]
+                ("solution_update", ctypes.c_int), ("convfac_threshold", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("num_precond", ctypes.c_int),
                 ("num_restarts", ctypes.c_int), ("final_rel_res", ctypes.c_double),
                 ("true_rel_res", ctypes.c_double), ("b_norm", ctypes.c_double),
-                ("t_solve_ms", ctypes.c_double), ("t_precond_ms", ctypes.c_double)]
+                ("t_solve_ms", ctypes.c_double), ("t_precond_ms", ctypes.c_double),
+                ("num_false_conv", ctypes.c_int), ("diverged", ctypes.c_int)]
 
 
 _lib = None
@@ -104,6 +105,7 @@ def _declare(L):
         "jxamd_solver_solve": [vp, vp, vp, vp, vp],
         "jxamd_solver_solve_device": [vp, vp, vp, vp, vp],
         "jxamd_cpr_precond_device": [vp, vp, vp],
+        "jxamd_cpr_precond_callback": [vp, vp, vp, vp],
         "jxamd_op_bsr_matvec": [vp, f64, vp, f64, vp],
         "jxamd_op_hsgs": [vp, i32, vp, vp],
         "jxamd_op_coarse_solve": [vp, vp, vp],

@@ -195,6 +195,7 @@ void jxamd_params_default(jxamd_params *p)
     p->threshold = 1e-15;
     p->stage2_weight = 0.7;
     p->stage2_omega = 0.9;
+    p->convfac_threshold = 1.0e4;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -913,6 +914,65 @@ static int dense_inverse(double *M, int n)
     return 0;
 }
 
+/* The stage-2 block solve v -> D^{-1} v as the reference computes it,
+ * jxf_BSRMatrixBlockInvMatvec_Stable (seq_bsr_matrix.c:1054-1097), written
+ * out as a b x b matrix: the routine is linear in v for a fixed block, so
+ * column j of M is its image of the unit vector e_j.  b = 2: closed form,
+ * or v / (a_ii + 1e-12) componentwise when the block is ill-conditioned
+ * (:1067-1078).  b >= 3 (_MixedPrec, :948-1051): partial pivoting LU; when a
+ * pivot column is below 1e-15 the elimination stops there (the diagonal
+ * estimate formed at :985-991 is overwritten by the substitutions that
+ * follow), and the substitutions then run on the partly eliminated block,
+ * giving 0 for every component whose pivot is below 1e-15. */
+static void block_solve_matrix(const double *D, int b, double *M)
+{
+    for (int col = 0; col < b; col++) {
+        double m[JXAMD_MAX_BLOCK * JXAMD_MAX_BLOCK], v[JXAMD_MAX_BLOCK], o[JXAMD_MAX_BLOCK];
+        for (int i = 0; i < b; i++) v[i] = i == col ? 1.0 : 0.0;
+        if (b == 2) {
+            const double a11 = D[0], a12 = D[1], a21 = D[2], a22 = D[3];
+            const double det = a11 * a22 - a12 * a21;
+            if (fabs(det) < 1e-12 * (fabs(a11 * a22) + fabs(a12 * a21))) {
+                o[0] = v[0] / (a11 + 1e-12);
+                o[1] = v[1] / (a22 + 1e-12);
+            } else {
+                const double inv_det = 1.0 / det;
+                o[0] = (a22 * v[0] - a12 * v[1]) * inv_det;
+                o[1] = (-a21 * v[0] + a11 * v[1]) * inv_det;
+            }
+        } else {
+            memcpy(m, D, sizeof(double) * b * b);
+            for (int i = 0; i < b; i++) {
+                int mr = i;
+                double mv = fabs(m[i * b + i]);
+                for (int k = i + 1; k < b; k++)
+                    if (fabs(m[k * b + i]) > mv) { mv = fabs(m[k * b + i]); mr = k; }
+                if (mv < 1e-15) break;
+                if (mr != i) {
+                    for (int j = 0; j < b; j++) { double t = m[i * b + j]; m[i * b + j] = m[mr * b + j]; m[mr * b + j] = t; }
+                    double t = v[i]; v[i] = v[mr]; v[mr] = t;
+                }
+                const double piv = m[i * b + i];
+                for (int k = i + 1; k < b; k++) {
+                    const double f = m[k * b + i] / piv;
+                    m[k * b + i] = f;
+                    for (int j = i + 1; j < b; j++) m[k * b + j] -= f * m[i * b + j];
+                }
+            }
+            for (int i = 0; i < b; i++) {
+                o[i] = v[i];
+                for (int j = 0; j < i; j++) o[i] -= m[i * b + j] * o[j];
+            }
+            for (int i = b - 1; i >= 0; i--) {
+                for (int j = i + 1; j < b; j++) o[i] -= m[i * b + j] * o[j];
+                const double dg = m[i * b + i];
+                o[i] = fabs(dg) > 1e-15 ? o[i] / dg : 0.0;
+            }
+        }
+        for (int r = 0; r < b; r++) M[r * b + col] = o[r];
+    }
+}
+
 /* transpose of a CSR matrix (rows of R sorted by the source row, which keeps
  * the accumulation order of jxf_CSRMatrixMatvecT, seq_csr_matvec.c:211-330) */
 static int csr_transpose(int n, int m, const int *ia, const int *ja, const double *a, int **tia, int **tja,
@@ -1098,19 +1158,7 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
                 }
             }
         }
-        for (int i = 0; i < nb; i++) {
-            double M[JXAMD_MAX_BLOCK * JXAMD_MAX_BLOCK];
-            memcpy(M, A->a + (size_t)A->ia[i] * bb, sizeof(double) * bb);
-            if (dense_inverse(M, b)) {
-                /* singular diagonal block: diagonal approximation (cf. seq_bsr_matrix.c:985-991) */
-                for (int r = 0; r < bb; r++) M[r] = 0.0;
-                for (int r = 0; r < b; r++) {
-                    double dg = A->a[(size_t)A->ia[i] * bb + r * b + r];
-                    M[r * b + r] = fabs(dg) > 1e-12 ? 1.0 / dg : 0.0;
-                }
-            }
-            memcpy(dinv + (size_t)i * bb, M, sizeof(double) * bb);
-        }
+        for (int i = 0; i < nb; i++) block_solve_matrix(A->a + (size_t)A->ia[i] * bb, b, dinv + (size_t)i * bb);
         d->unit_diag = 1;
         for (int i = 0; i < nb && d->unit_diag; i++)
             for (int r = 0; r < b; r++)
@@ -1550,6 +1598,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->stage2_precision < 0 || p->stage2_precision > 2) return JXAMD_ERROR_ARG;
     if (p->arnoldi_kernel < 0 || p->arnoldi_kernel > 1) return JXAMD_ERROR_ARG;
     if (p->solution_update < 0 || p->solution_update > 1) return JXAMD_ERROR_ARG;
+    if (!(p->convfac_threshold > 0.0)) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -1587,6 +1636,9 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_event_create_sync(&s->evh[1]);
     if (!e) e = jxamd_host_alloc((void **)&s->hpin, sizeof(double) * 2 * ((size_t)p->k_dim + 2));
     if (!e) e = jxamd_host_dev_ptr((void **)&s->hpin_dev, s->hpin);
+    if (!e && getenv("JXAMD_DEBUG"))
+        fprintf(stderr, "[jxamd] solver %p: pinned column buffer host %p device %p\n", (void *)s, (void *)s->hpin,
+                (void *)s->hpin_dev);
     if (!e) e = build_bsr(s, A);
     if (!e && p->stage2_solver_type == 5) e = build_bilu(s, A);
     for (int l = 0; l < s->L && !e; l++) {
@@ -1643,6 +1695,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
 {
     if (!s) return 0;
     if (s->stream) jxamd_stream_sync(s->stream);
+    if (getenv("JXAMD_DEBUG")) fprintf(stderr, "[jxamd] solver %p destroyed\n", (void *)s);
     dbsr *d = &s->A;
     jxamd_dev_free(d->sptr); jxamd_dev_free(d->col); jxamd_dev_free(d->val);
     jxamd_dev_free(d->gperm); jxamd_dev_free(d->gsptr); jxamd_dev_free(d->gcol);
@@ -1970,6 +2023,21 @@ int jxamd_cpr_precond_device(jxamd_solver *s, const double *d_r, double *d_x)
 {
     if (!s || !d_r || !d_x) return JXAMD_ERROR_ARG;
     CK(cpr_apply(s, d_r, d_x));
+    return jxamd_stream_sync(s->stream);
+}
+
+int jxamd_cpr_precond_callback(void *precond_data, void *A, void *b, void *x)
+{
+    (void)A;
+    jxamd_precond_ctx *c = (jxamd_precond_ctx *)precond_data;
+    if (!c || !c->solver || !c->vector_data || !b || !x) return JXAMD_ERROR_ARG;
+    jxamd_solver *s = c->solver;
+    const double *hb = c->vector_data(b);
+    double *hx = c->vector_data(x);
+    if (!hb || !hx) return JXAMD_ERROR_ARG;
+    CK(jxamd_h2d(s->Bd, hb, sizeof(double) * s->N, s->stream));
+    CK(cpr_apply(s, s->Bd, s->Xd));
+    CK(jxamd_d2h(hx, s->Xd, sizeof(double) * s->N, s->stream));
     return jxamd_stream_sync(s->stream);
 }
 
@@ -2353,7 +2421,8 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     if (!s || !bvec || !x) return JXAMD_ERROR_ARG;
     const int k_dim = s->prm.k_dim, max_iter = s->prm.max_iter;
     const size_t N = s->N;
-    const double r_tol = s->prm.tol, a_tol = s->prm.a_tol, epsmac = 1.e-16, convfac_threshold = 1.0e4;
+    const double r_tol = s->prm.tol, a_tol = s->prm.a_tol, epsmac = 1.e-16;
+    const double convfac_threshold = s->prm.convfac_threshold;
     double *V = s->V;
 #define VEC(j) (V + (size_t)(j) * N)
     double *ydev = s->dsc + k_dim + 4; /* back-solved coefficients, k_dim+1 */
@@ -2362,7 +2431,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     double *sn = (double *)calloc(k_dim, sizeof(double));
     double *hcol = (double *)calloc(k_dim + 2, sizeof(double));
     double **hh = (double **)calloc(k_dim + 1, sizeof(double *));
-    int err = 0, converged = 0, nprec = 0, nrestart = 0;
+    int err = 0, converged = 0, nprec = 0, nrestart = 0, nfalse = 0, diverged = 0;
     if (!rs || !c || !sn || !hcol || !hh) err = JXAMD_ERROR_MEMORY;
     for (int q = 0; q <= k_dim && !err; q++)
         if (!(hh[q] = (double *)calloc(k_dim, sizeof(double)))) err = JXAMD_ERROR_MEMORY;
@@ -2405,6 +2474,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             TRY(bsr_apply(s, x, bvec, s->Rv, 1));
             TRY(host_norm(s, s->Rv, &r_norm));
             if (r_norm <= epsilon) { converged = 1; true_norm = r_norm; break; }
+            nfalse++;
             if (s->prm.is_check_restarted == 0) break;
         }
         TRY(jxamd_k_scale(N, NULL, 1.0 / r_norm, VEC(0), s->stream));
@@ -2459,7 +2529,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             normdown = normup;
             if (res_history) res_history[iter] = r_norm;
             if (s->prm.print_level > 0) printf("% 5d    %e    %e\n", iter, r_norm, b_norm > 0 ? r_norm / b_norm : r_norm);
-            if (iter > 1 && conv_factor > convfac_threshold) break;
+            if (iter > 1 && conv_factor > convfac_threshold) { diverged = 1; break; }
             if (r_norm <= epsilon) break;
         }
         if (ahead) nprec--; /* a step enqueued ahead in vain is not an application of the method */
@@ -2491,6 +2561,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             TRY(bsr_apply(s, x, bvec, s->Rv, 1));
             TRY(host_norm(s, s->Rv, &r_norm));
             if (r_norm <= epsilon) { converged = 1; true_norm = r_norm; break; }
+            nfalse++;
             if (s->prm.is_check_restarted == 1) {
                 /* false convergence 2: restart from the true residual */
                 TRY(jxamd_k_copy(N, s->Rv, VEC(0), s->stream));
@@ -2523,6 +2594,8 @@ done:
         st->converged = converged;
         st->num_precond = nprec;
         st->num_restarts = nrestart;
+        st->num_false_conv = nfalse;
+        st->diverged = diverged;
         st->b_norm = b_norm;
         st->final_rel_res = b_norm > 0.0 ? r_norm / b_norm : (r_norm_0 > 0.0 ? r_norm / r_norm_0 : 0.0);
         float ms = 0.f;

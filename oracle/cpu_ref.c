@@ -693,7 +693,8 @@ int cref_gmres_solve(const cref_bsr *A, const cref_hier *H, const cref_params *p
 {
     const int k_dim = p->k_dim, max_iter = p->max_iter;
     const long n = (long)A->nb * A->b;
-    const double r_tol = p->tol, a_tol = p->a_tol, epsmac = 1.e-16, convfac_threshold = 1.0e4;
+    const double r_tol = p->tol, a_tol = p->a_tol, epsmac = 1.e-16;
+    const double convfac_threshold = p->convfac_threshold > 0.0 ? p->convfac_threshold : 1.0e4;
     double **P = (double **)malloc(sizeof(double *) * (k_dim + 1));
     for (int i = 0; i <= k_dim; i++) P[i] = (double *)malloc(sizeof(double) * n);
     double *R = (double *)malloc(sizeof(double) * n);
@@ -703,7 +704,7 @@ int cref_gmres_solve(const cref_bsr *A, const cref_hier *H, const cref_params *p
     double *s = (double *)calloc(k_dim, sizeof(double));
     double **hh = (double **)malloc(sizeof(double *) * (k_dim + 1));
     for (int i = 0; i <= k_dim; i++) hh[i] = (double *)calloc(k_dim, sizeof(double));
-    int err = 0, converged = 0, nprec = 0;
+    int err = 0, converged = 0, nprec = 0, nfalse = 0;
     int i = 0, j, k, iter = 0;
     double t, gamma, r_norm, b_norm, den_norm, epsilon, r_norm_0;
     double normup, normdown, conv_factor = 0.0;
@@ -731,6 +732,7 @@ int cref_gmres_solve(const cref_bsr *A, const cref_hier *H, const cref_params *p
                 converged = 1;
                 break;
             }
+            nfalse++;
             if (p->is_check_restarted == 0) break;
         }
         t = 1.0 / r_norm;
@@ -803,6 +805,7 @@ int cref_gmres_solve(const cref_bsr *A, const cref_hier *H, const cref_params *p
                 converged = 1;
                 break;
             }
+            nfalse++;
             if (p->is_check_restarted == 1) {
                 memcpy(P[0], R, sizeof(double) * n);
                 i = 0;
@@ -835,6 +838,7 @@ done:
         st->num_precond = nprec;
         st->b_norm = b_norm;
         st->final_rel_res = b_norm > 0.0 ? r_norm / b_norm : r_norm / r_norm_0;
+        st->num_false_conv = nfalse;
     }
     for (int q = 0; q <= k_dim; q++) {
         free(P[q]);

@@ -49,11 +49,15 @@ typedef struct {
     /* stage2_solver_type 2 with stage2_parts > 1: the sweeps of a run on
      * stage2_parts ranks (contiguous block-row partitions) */
     int stage2_parts;
+    /* GMRES stops when one step's residual estimate grows by more than this
+     * factor (pgmres.c:1325-1333; JXF_GMRESSetConvFacThreshold, default 1e4) */
+    double convfac_threshold;
 } cref_params;
 
 typedef struct {
     int iterations, converged, num_precond;
     double final_rel_res, b_norm;
+    int num_false_conv; /* estimates at tolerance whose true residual was not (pgmres.c:1583-1608) */
 } cref_stats;
 
 void cref_bsr_matvec(const cref_bsr *A, double alpha, const double *x, double beta, double *y);

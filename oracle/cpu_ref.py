@@ -34,12 +34,12 @@ class Params(ctypes.Structure):
                 ("tol", ctypes.c_double), ("a_tol", ctypes.c_double), ("pressure_index", ctypes.c_int),
                 ("stage1_maxit", ctypes.c_int), ("stage2_maxit", ctypes.c_int), ("stage2_weight", ctypes.c_double),
                 ("stage2_omega", ctypes.c_double), ("stage2_solver_type", ctypes.c_int), ("bilu_lu", ctypes.c_void_p),
-                ("bilu_dinv", ctypes.c_void_p), ("stage2_parts", ctypes.c_int)]
+                ("bilu_dinv", ctypes.c_void_p), ("stage2_parts", ctypes.c_int), ("convfac_threshold", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("num_precond", ctypes.c_int),
-                ("final_rel_res", ctypes.c_double), ("b_norm", ctypes.c_double)]
+                ("final_rel_res", ctypes.c_double), ("b_norm", ctypes.c_double), ("num_false_conv", ctypes.c_int)]
 
 
 _lib = None
@@ -102,13 +102,13 @@ class Oracle:
 
     def params(self, k_dim=30, max_iter=100, is_check_restarted=1, tol=1e-6, a_tol=0.0, pressure_index=0,
                stage1_maxit=1, stage2_maxit=3, stage2_weight=0.7, stage2_omega=0.9, stage2_solver_type=2,
-               stage2_parts=0):
+               stage2_parts=0, convfac_threshold=1.0e4):
         lu = dinv = None
         if stage2_solver_type == 5:
             lu, dinv = self.bilu0()
             lu, dinv = _p(lu).value, _p(dinv).value
         return Params(k_dim, max_iter, is_check_restarted, tol, a_tol, pressure_index, stage1_maxit, stage2_maxit,
-                      stage2_weight, stage2_omega, stage2_solver_type, lu, dinv, stage2_parts)
+                      stage2_weight, stage2_omega, stage2_solver_type, lu, dinv, stage2_parts, convfac_threshold)
 
     def bilu0(self):
         """Exact block ILU(0) factors (lu on A's pattern, dinv = U_ii^{-1}), cached."""
