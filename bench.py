@@ -283,8 +283,10 @@ def main():
         # FP32.  Then the mixed + learned configuration: FP16 stage 1 (row
         # scaled, FP32 vectors) with a degree-2 polynomial smoother whose roots
         # the network predicts (trained here, on the GPU, on data the plain
-        # solver labels itself; default Chebyshev roots where they do not win),
-        # one stage-2 sweep with FP16 blocks (row-scaled)
+        # solver labels itself), judged level by level against relax 6 (the
+        # plain smoother, also the fallback), one stage-2 sweep with FP16
+        # blocks (row-scaled); the same with the default Chebyshev roots is
+        # the method without the network
         from jxpamg_all_amd import learned as lrn
         net, t_train = None, 0.0
         if args.train_seeds > 0:
@@ -297,7 +299,7 @@ def main():
         legs = {}
         stage2_ref = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
         stage2_one = dict(stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=2)
-        mixed_kw = dict(precond_precision=2, poly_degree=2, **stage2_one)
+        mixed_kw = dict(precond_precision=2, poly_degree=2, relax_chunk=chunk, **stage2_one)
         runs = [("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_ref)),
                 ("mixed_default_poly", mixed_kw)]
         if net is not None:
@@ -306,15 +308,19 @@ def main():
             pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, **kw)
             sm = jx.Solver(Ad, H, pm)
             used = None
-            if name == "mixed_learned":
-                used, f_l, f_d = sm.apply_mlp(net.P.data_ptr())
-                log(f"learned steps on {used} level(s): V-cycle factor {f_l:.4f} vs default {f_d:.4f}")
+            if name in ("mixed_default_poly", "mixed_learned"):
+                # judged level by level against relax 6, the fallback
+                used, f_j, f_p = sm.apply_mlp(net.P.data_ptr()) if name == "mixed_learned" else sm.judge_poly()
+                log(f"{name}: polynomial on {used} level(s), V-cycle factor {f_j:.4f} (relax 6 everywhere "
+                    f"{f_p:.4f}), relax-6 levels {[l for l, v in enumerate(sm.level_smoothers()[:-1]) if v]}")
             t_m, st_m = timed_solves(sm, args.steps)
             assert st_m.converged == 1 and st_m.true_rel_res <= 1e-6, f"{name} did not reach rtol 1e-6"
             legs[name] = {"s": round(t_m, 4), "iterations": st_m.iterations, "true_rel_res": st_m.true_rel_res,
                           "speedup_vs_plain_fp64": round(sec / t_m, 3)}
             if used is not None:
-                legs[name]["levels_on_learned_steps"] = used
+                legs[name]["polynomial_levels"] = used
+                legs[name]["vcycle_factor"] = round(f_j, 4)
+                legs[name]["vcycle_factor_relax6"] = round(f_p, 4)
             log(f"{name}: {t_m * 1e3:.1f} ms iters {st_m.iterations} true_rel {st_m.true_rel_res:.3e}")
             sm.close()
         legs["training_s"] = round(t_train, 2) if net is not None else None

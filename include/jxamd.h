@@ -322,14 +322,25 @@ int jxamd_solver_poly_features(jxamd_solver *s, int level, float *feat);
 /* V-cycle contraction factor: ||e_k|| / ||e_0|| to the power 1/ncycles for
  * ncycles V-cycles on f = 0 from a fixed pseudo-random e_0 */
 int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor);
-/* Predict every level's polynomial roots tau_k (w_k = 1 / (tau_k lambda_max),
- * the first poly_degree outputs; poly_degree <= 4) with the MLP (d_params on
- * the device) and, level by level, keep them only
- * where they lower the measured V-cycle contraction factor; elsewhere the
- * default Chebyshev steps stay (the fallback).  *used = number of levels on
- * learned steps; final and default factors reported if non-NULL. */
+/* The learned smoother and its judge.  The plain smoother of the reference
+ * (relax 6 as configured by relax_chunk / relax_threads) is both the
+ * fallback and the judge: with the polynomial on every smoothing level,
+ * each level in turn falls back to relax 6 whenever that gives a lower
+ * measured V-cycle contraction factor (jxamd_solver_vcycle_factor, 4
+ * cycles), i.e. whenever its polynomial's per-cycle reduction is worse;
+ * levels whose predicted roots are out of range start on relax 6.
+ * apply_mlp: polynomial roots tau_k predicted by the MLP (d_params on the
+ * device; w_k = 1 / (tau_k lambda_max), the first poly_degree <= 4 outputs).
+ * judge_poly: the polynomial steps in use (the default Chebyshev roots, or
+ * what set_poly installed), i.e. the same method without the network.
+ * *used = levels left on the polynomial; the final factor and that of
+ * relax 6 on every level are reported if non-NULL. */
 int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, double *factor_learned,
-                           double *factor_default);
+                           double *factor_plain);
+int jxamd_solver_judge_poly(jxamd_solver *s, int *used, double *factor, double *factor_plain);
+/* a level's smoother when poly_degree > 0: plain = 1 relax 6, 0 the polynomial */
+int jxamd_solver_set_level_smoother(jxamd_solver *s, int level, int plain);
+int jxamd_solver_get_level_smoother(jxamd_solver *s, int level, int *plain);
 
 /* Single operators on device vectors (parity tests, benchmarks). */
 /* y = alpha*A*x + beta*y, jxf_BSRMatrixMatvec (seq_bsr_matvec.c:21-134) */

@@ -131,6 +131,9 @@ def _declare(L):
         "jxamd_solver_poly_features": [vp, i32, vp],
         "jxamd_solver_vcycle_factor": [vp, i32, vp],
         "jxamd_solver_apply_mlp": [vp, vp, vp, vp, vp],
+        "jxamd_solver_judge_poly": [vp, vp, vp, vp],
+        "jxamd_solver_set_level_smoother": [vp, i32, i32],
+        "jxamd_solver_get_level_smoother": [vp, i32, vp],
         "jxamd_bilu0_factor": [i32, i32, vp, vp, vp, vp, vp],
         "jxamd_bsr_residual_norm": [i32, i32, vp, vp, vp, vp, vp, vp, vp],
         "jxamd_bsr_residual_norm_noalloc": [i32, i32, vp, vp, vp, vp, vp, vp, vp],
@@ -485,12 +488,35 @@ class Solver:
         return v.value
 
     def apply_mlp(self, d_params):
-        """Learned polynomial roots, kept on each level only where they lower the
-        V-cycle contraction: (levels on learned steps, final factor, default factor)."""
-        used, fl, fd = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        """Learned polynomial roots, judged against relax 6: each level keeps them
+        only where they lower the V-cycle contraction, else falls back to relax 6.
+        Returns (levels on learned steps, final factor, factor of relax 6 everywhere)."""
+        used, fl, fp = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         check(self._L.jxamd_solver_apply_mlp(self._s, ctypes.c_void_p(d_params), ctypes.byref(used), ctypes.byref(fl),
-                                             ctypes.byref(fd)), "jxamd_solver_apply_mlp")
-        return used.value, fl.value, fd.value
+                                             ctypes.byref(fp)), "jxamd_solver_apply_mlp")
+        return used.value, fl.value, fp.value
+
+    def judge_poly(self):
+        """The polynomial steps in use (default Chebyshev roots unless set),
+        judged against relax 6 level by level as apply_mlp does:
+        (levels on the polynomial, final factor, factor of relax 6 everywhere)."""
+        used, f, fp = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(self._L.jxamd_solver_judge_poly(self._s, ctypes.byref(used), ctypes.byref(f), ctypes.byref(fp)),
+              "jxamd_solver_judge_poly")
+        return used.value, f.value, fp.value
+
+    def set_level_smoother(self, level, plain):
+        check(self._L.jxamd_solver_set_level_smoother(self._s, int(level), int(plain)),
+              "jxamd_solver_set_level_smoother")
+
+    def level_smoothers(self):
+        """[1 if level l smooths with relax 6 else 0 for each level]"""
+        out = []
+        for l in range(self.num_levels):
+            v = ctypes.c_int()
+            check(self._L.jxamd_solver_get_level_smoother(self._s, l, ctypes.byref(v)), "get_level_smoother")
+            out.append(v.value)
+        return out
 
     def bilu_solve(self, d_r, d_z):
         check(self._L.jxamd_op_bilu_solve(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_z)), "jxamd_op_bilu_solve")

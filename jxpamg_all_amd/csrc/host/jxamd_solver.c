@@ -109,6 +109,7 @@ struct jxamd_solver_s {
     double *up64;          /* FP64 copy of the stage-1 result when xt != F64 */
     dtri Lf, Ub;           /* stage-2 BILU(0) factors (stage2_solver_type 5) */
     void *dinvl[JXAMD_MAX_LEVELS];          /* polynomial smoother: 1/a_ii (0 if a_ii = 0), type xt */
+    int plain[JXAMD_MAX_LEVELS];            /* with poly_degree > 0: 1 = this level smooths with relax 6 */
     double poly_w[JXAMD_MAX_LEVELS][8];     /* its step sizes */
     double lam[JXAMD_MAX_LEVELS];           /* lambda_max(D^{-1} A_l) estimate */
     float feat[JXAMD_MAX_LEVELS][JXAMD_MLP_NFEAT]; /* level features of the learned step sizes */
@@ -1506,10 +1507,9 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     }
     /* CSR SpMV bytes: nnz (value + index) + row pointer + x + rhs + y */
     s->Ares[l].abytes = level_bytes(&s->Ares[l], A->ia[n], n, 3.0);
-    if (s->prm.poly_degree > 0 && (l < s->L - 1 || s->L == 1)) {
-        CK(build_poly(s, l, H));
-        return build_transfer(s, l, H);
-    }
+    /* the polynomial smoother, and relax 6 beside it (the plain smoother a
+     * level falls back to) */
+    if (s->prm.poly_degree > 0 && (l < s->L - 1 || s->L == 1)) CK(build_poly(s, l, H));
     int T = 0;
     if (s->prm.relax_threads > 1) T = s->prm.relax_threads;
     else if (s->prm.relax_chunk > 0) T = (n + s->prm.relax_chunk - 1) / s->prm.relax_chunk;
@@ -1800,6 +1800,9 @@ static int hsgs_chunk(jxamd_solver *s, int l, const void *f, int fs, const void 
     return 0;
 }
 
+/* the level smooths with its polynomial (poly_degree > 0 and no fallback to relax 6) */
+static int level_poly(const jxamd_solver *s, int l) { return s->prm.poly_degree > 0 && !s->plain[l]; }
+
 /* one application of the level's polynomial smoother; steps alternate
  * between T and Vt and the last one lands in u.  uo_zero: 0 u holds the
  * entry values, 1 they are zero, 2 they are zero and the first step is
@@ -1842,7 +1845,7 @@ static int restrict_launch(jxamd_solver *s, int l)
 {
     const dsell *R = &s->R[l];
     const int c = l + 1;
-    if (s->prm.poly_degree == 0 || c >= s->L - 1)
+    if (!level_poly(s, c) || c >= s->L - 1)
         return csr_launch(s, R, s->Vt[l], NULL, 1, s->F[c], 0);
     int pq = prof_begin(s);
     CK(jxamd_k_restrict_first(R->G > 1 ? R->G : 1, lvt(R), R->xt, R->n, R->sptr, R->col, R->val, R->rsc, s->Vt[l],
@@ -1855,7 +1858,7 @@ static int restrict_launch(jxamd_solver *s, int l)
  * or one polynomial smoother application; the result stays in u */
 static int relax_pair(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
-    if (s->prm.poly_degree > 0) return poly_apply(s, l, f, fs, u, uo_zero);
+    if (level_poly(s, l)) return poly_apply(s, l, f, fs, u, uo_zero);
     if (s->nchunks[l] > 1) {
         CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
         return hsgs_chunk(s, l, f, fs, s->T[l], u);
@@ -1867,7 +1870,7 @@ static int relax_pair(jxamd_solver *s, int l, const void *f, int fs, void *u, in
 /* a single relax-6 call (or polynomial smoother application); the result stays in u */
 static int relax_once(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
-    if (s->prm.poly_degree > 0) return poly_apply(s, l, f, fs, u, uo_zero);
+    if (level_poly(s, l)) return poly_apply(s, l, f, fs, u, uo_zero);
     if (s->nchunks[l] > 1) {
         CK(hsgs_chunk(s, l, f, fs, uo_zero ? NULL : u, s->T[l]));
         return jxamd_d2d(u, s->T[l], s->xsz * s->nlev[l], s->stream);
@@ -1902,7 +1905,7 @@ static int vcycle(jxamd_solver *s, const void *f0, int fs0, int u0_zero)
         const int fs = l == 0 ? fs0 : 1;
         /* coarse levels start from zero; with the polynomial smoother their
          * first step came with the restriction (2) */
-        const int zero = l == 0 ? u0_zero : (s->prm.poly_degree > 0 ? 2 : 1);
+        const int zero = l == 0 ? u0_zero : (level_poly(s, l) ? 2 : 1);
         CK(relax_pair(s, l, f, fs, s->U[l], zero));
         /* Vtemp = F - A U; F_{l+1} = P^T Vtemp */
         CK(csr_launch(s, &s->Ares[l], s->U[l], f, fs, s->Vt[l], 1));
@@ -2205,8 +2208,79 @@ int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor)
     return 0;
 }
 
+/* The polynomial smoother with the step sizes w[l] on every smoothing level
+ * (levels with ok[l] = 0 start on relax 6), judged level by level against
+ * the plain smoother: a level falls back to relax 6 whenever that gives a
+ * lower measured V-cycle contraction factor, i.e. whenever its polynomial's
+ * per-cycle reduction is worse.  The plain smoother is both the fallback and
+ * the judge. */
+static int judge_levels(jxamd_solver *s, double (*w)[8], const int *ok, int *used, double *factor, double *fplain)
+{
+    const int m = s->prm.poly_degree;
+    const int Ls = s->L > 1 ? s->L - 1 : 1;
+    for (int l = 0; l < Ls; l++) s->plain[l] = 1;
+    double fp = 0.0;
+    CK(jxamd_solver_vcycle_factor(s, 4, &fp));
+    if (fplain) *fplain = fp;
+    for (int l = 0; l < Ls; l++) {
+        if (!ok[l]) continue;
+        for (int k = 0; k < m; k++) s->poly_w[l][k] = w[l][k];
+        s->plain[l] = 0;
+    }
+    double best = 0.0;
+    CK(jxamd_solver_vcycle_factor(s, 4, &best));
+    int kept = 0;
+    for (int l = 0; l < Ls; l++) {
+        if (s->plain[l]) continue;
+        s->plain[l] = 1;
+        double f = 0.0;
+        CK(jxamd_solver_vcycle_factor(s, 4, &f));
+        if (f < best) {
+            best = f; /* the polynomial is worse here: fall back to relax 6 */
+        } else {
+            s->plain[l] = 0;
+            kept++;
+        }
+    }
+    if (best > fp) {
+        /* worse than relax 6 on every level: fall back everywhere */
+        for (int l = 0; l < Ls; l++) s->plain[l] = 1;
+        best = fp;
+        kept = 0;
+    }
+    if (used) *used = kept;
+    if (factor) *factor = best;
+    return 0;
+}
+
+int jxamd_solver_judge_poly(jxamd_solver *s, int *used, double *factor, double *factor_plain)
+{
+    if (!s || s->prm.poly_degree < 1) return JXAMD_ERROR_ARG;
+    double w[JXAMD_MAX_LEVELS][8];
+    int ok[JXAMD_MAX_LEVELS];
+    for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
+        ok[l] = 1;
+        for (int k = 0; k < 8; k++) w[l][k] = s->poly_w[l][k];
+    }
+    return judge_levels(s, w, ok, used, factor, factor_plain);
+}
+
+int jxamd_solver_set_level_smoother(jxamd_solver *s, int level, int plain)
+{
+    if (!s || s->prm.poly_degree < 1 || level < 0 || level >= s->L) return JXAMD_ERROR_ARG;
+    s->plain[level] = plain ? 1 : 0;
+    return 0;
+}
+
+int jxamd_solver_get_level_smoother(jxamd_solver *s, int level, int *plain)
+{
+    if (!s || !plain || level < 0 || level >= s->L) return JXAMD_ERROR_ARG;
+    *plain = s->prm.poly_degree < 1 || s->plain[level];
+    return 0;
+}
+
 int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, double *factor_learned,
-                           double *factor_default)
+                           double *factor_plain)
 {
     if (!s || !d_params || s->prm.poly_degree < 1 || s->prm.poly_degree > JXAMD_MLP_NOUT) return JXAMD_ERROR_ARG;
     const int m = s->prm.poly_degree; /* the network's first m outputs */
@@ -2223,36 +2297,18 @@ int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, do
     jxamd_dev_free(dy);
     CK(e);
     /* the network predicts the polynomial's roots in units of lambda_max:
-     * w_k = 1 / (tau_k lambda_max); level by level, learned steps replace the
-     * defaults only where they lower the measured V-cycle contraction */
-    double best = 0.0;
-    CK(jxamd_solver_vcycle_factor(s, 4, &best));
-    const double fdef = best;
-    int kept = 0;
+     * w_k = 1 / (tau_k lambda_max) */
+    double w[JXAMD_MAX_LEVELS][8];
+    int ok[JXAMD_MAX_LEVELS];
     for (int l = 0; l < Ls; l++) {
-        double keep[8], w[8];
-        int ok = 1;
+        ok[l] = 1;
         for (int k = 0; k < m; k++) {
             const double tau = (double)hy[l * JXAMD_MLP_NOUT + k];
-            if (!(tau > 5e-3) || !(tau < 4.0)) ok = 0; /* also rejects NaN */
-            w[k] = 1.0 / (tau * s->lam[l]);
-            keep[k] = s->poly_w[l][k];
-        }
-        if (!ok) continue;
-        for (int k = 0; k < m; k++) s->poly_w[l][k] = w[k];
-        double f = 0.0;
-        CK(jxamd_solver_vcycle_factor(s, 4, &f));
-        if (f < best) {
-            best = f;
-            kept++;
-        } else {
-            for (int k = 0; k < m; k++) s->poly_w[l][k] = keep[k];
+            if (!(tau > 5e-3) || !(tau < 4.0)) ok[l] = 0; /* also rejects NaN */
+            w[l][k] = 1.0 / (tau * s->lam[l]);
         }
     }
-    if (used) *used = kept;
-    if (factor_learned) *factor_learned = best;
-    if (factor_default) *factor_default = fdef;
-    return 0;
+    return judge_levels(s, w, ok, used, factor_learned, factor_plain);
 }
 
 int jxamd_op_bilu_solve(jxamd_solver *s, const double *d_r, double *d_z)

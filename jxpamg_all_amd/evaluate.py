@@ -1,7 +1,7 @@
 """Evaluation of the mixed-precision and learned configurations on one GPU.
 
     python -m jxpamg_all_amd.evaluate C4      # 100^3, 4x4 blocks (exact decoupling), FP16 stage 1, rtol 1e-8
-    python -m jxpamg_all_amd.evaluate C5      # 120x440x85, mixed + learned against FP64, 5 unseen seeds
+    python -m jxpamg_all_amd.evaluate C5      # 120x440x85, FP32 preconditioner + learned smoother, 5 unseen seeds
     python -m jxpamg_all_amd.evaluate C3      # the 16x16x8 system with an FP32 hierarchy
 
 Each configuration is solved with the plain FP64 GPU path and with the
@@ -30,15 +30,24 @@ def _setup(cfg, exact=False):
     return Ad, bd, H, time.time() - t0
 
 
-def _solve(Ad, bd, H, params, reps=2, mlp=None):
+def _solve(Ad, bd, H, params, reps=2, mlp=None, judge=None):
+    """Time `reps` solves (after one untimed).  judge: None (the smoothers as
+    configured), "plain" (relax 6 on every level), "default" (default
+    Chebyshev roots judged level by level against relax 6) or "learned"
+    (the network's roots, mlp, judged the same way)."""
     import torch
 
     from . import Solver
     s = Solver(Ad, H, params)
-    used = None
-    if mlp is not None:
-        used, f_l, f_d = s.apply_mlp(mlp.P.data_ptr())
-        used = {"levels": used, "factor_learned": round(f_l, 4), "factor_default": round(f_d, 4)}
+    info = None
+    if judge == "plain":
+        for l in range(s.num_levels):
+            s.set_level_smoother(l, 1)
+        info = {"polynomial_levels": 0, "factor": round(s.vcycle_factor(), 4)}
+    elif judge in ("default", "learned"):
+        used, f, fp = s.apply_mlp(mlp.P.data_ptr()) if judge == "learned" else s.judge_poly()
+        info = {"polynomial_levels": used, "factor": round(f, 4), "factor_relax6": round(fp, 4),
+                "relax6_levels": [l for l, v in enumerate(s.level_smoothers()[:-1]) if v]}
     b = torch.tensor(bd, dtype=torch.float64, device="cuda")
     x = torch.zeros_like(b)
     hist = np.zeros(params.max_iter + 1)
@@ -54,8 +63,8 @@ def _solve(Ad, bd, H, params, reps=2, mlp=None):
     s.close()
     out = {"s": round(float(np.mean(ts)), 4), "iterations": st.iterations, "converged": st.converged,
            "true_rel_res": st.true_rel_res}
-    if used is not None:
-        out["learned"] = used
+    if info is not None:
+        out["smoother"] = info
     return out, x.cpu().numpy()
 
 
@@ -99,35 +108,53 @@ def evaluate_c4():
 # level matrices, FP32 vectors), degree-2 polynomial smoother, one stage-2
 # sweep (w 1.0, omega 0.8) with FP16 blocks; GMRES FP64
 MIXED = dict(precond_precision=2, poly_degree=2, stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8,
-             stage2_chunk=4, stage2_precision=2)
+             stage2_chunk=4, stage2_precision=2, relax_chunk=4)
+# the 120x440x85 learned-smoother configuration: FP32 preconditioner (stage-1
+# level matrices, vectors and accumulation; FP32 stage-2 blocks), degree-2
+# polynomial smoother judged against relax 6 (chunks of 4 rows) level by level
+C5_MIXED = dict(precond_precision=1, poly_degree=2, stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8,
+                stage2_chunk=4, stage2_precision=1, relax_chunk=4)
+
+
+def train_c5_network(train_seeds=20, precision=1, m=2):
+    """The learned smoother's network, trained on the GPU on levels of
+    train_seeds small systems labelled by the plain solver."""
+    from . import learned as lrn
+    t0 = time.time()
+    Xtr, Ttr, info = lrn.make_dataset(range(1001, 1001 + train_seeds), precision=precision, m=m)
+    net = lrn.MLP(seed=0)
+    losses = net.fit(Xtr, Ttr, steps=3000, lr=3e-3)
+    log(f"trained on {len(Xtr)} levels of {train_seeds} systems, loss {losses[0]:.4f} -> {losses[-1]:.5f}")
+    return net, {"samples": len(Xtr), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                 "seconds": round(time.time() - t0, 2)}
+
+
+def evaluate_c5_seed(seed, net, reps=2):
+    """One unseen 120x440x85 system: plain FP64 against the FP32
+    preconditioner with relax 6 everywhere, with the default roots and with
+    the learned roots (both judged against relax 6)."""
+    from . import CONFIGS, default_params
+    log(f"C5 seed {seed}: setup")
+    Ad, bd, H, t = _setup(dict(CONFIGS["C5"], seed=seed))
+    r64, x64 = _solve(Ad, bd, H, default_params(tol=1e-6, relax_chunk=4, stage2_chunk=4), reps=reps)
+    log(f"C5 seed {seed}: fp64 {r64}")
+    pm = default_params(tol=1e-6, **C5_MIXED)
+    rec = {"seed": seed, "setup_s": round(t, 2), "fp64": r64}
+    for name in ("plain", "default", "learned"):
+        r, x = _solve(Ad, bd, H, pm, reps=reps, mlp=net, judge=name)
+        r["error_vs_fp64"] = _rel(x, x64)
+        r["speedup_vs_fp64"] = round(r64["s"] / r["s"], 3)
+        rec[f"fp32_{name}"] = r
+        log(f"C5 seed {seed}: fp32 {name} {r}")
+    return rec
 
 
 def evaluate_c5(seeds=(2001, 2002, 2003, 2004, 2005), train_seeds=20):
-    from . import CONFIGS, default_params
-    from . import learned as lrn
-    t0 = time.time()
-    Xtr, Ttr, info = lrn.make_dataset(range(1001, 1001 + train_seeds), precision=2, m=2)
-    net = lrn.MLP(seed=0)
-    losses = net.fit(Xtr, Ttr, steps=3000, lr=3e-3)
-    log(f"C5: trained on {len(Xtr)} levels of {train_seeds} systems, loss {losses[0]:.4f} -> {losses[-1]:.5f}")
-    out = {"config": "C5 120x440x85 3x3: plain FP64 against FP16 stage 1 + degree-2 polynomial (default and "
-                     "learned roots) + one FP16-block stage-2 sweep, 5 unseen seeds",
-           "training": {"samples": len(Xtr), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
-                        "seconds": round(time.time() - t0, 2)}, "seeds": []}
-    for seed in seeds:
-        log(f"C5 seed {seed}: setup")
-        Ad, bd, H, t = _setup(dict(CONFIGS["C5"], seed=seed))
-        r64, x64 = _solve(Ad, bd, H, default_params(tol=1e-6, relax_chunk=4, stage2_chunk=4), reps=2)
-        pm = default_params(tol=1e-6, **MIXED)
-        rdef, xdef = _solve(Ad, bd, H, pm, reps=2)
-        rlrn, xlrn = _solve(Ad, bd, H, pm, reps=2, mlp=net)
-        for r, x in ((rdef, xdef), (rlrn, xlrn)):
-            r["error_vs_fp64"] = _rel(x, x64)
-            r["speedup_vs_fp64"] = round(r64["s"] / r["s"], 3)
-        rec = {"seed": seed, "setup_s": round(t, 2), "fp64": r64, "mixed_default_poly": rdef,
-               "mixed_learned_poly": rlrn}
-        log(f"C5 seed {seed}: {json.dumps(rec)}")
-        out["seeds"].append(rec)
+    net, training = train_c5_network(train_seeds)
+    out = {"config": "C5 120x440x85 3x3: plain FP64 against the FP32 preconditioner with relax 6 (plain), and "
+                     "with the degree-2 polynomial smoother on default and on learned roots, each judged level by "
+                     "level against relax 6; one FP32-block stage-2 sweep; 5 unseen seeds",
+           "training": training, "seeds": [evaluate_c5_seed(sd, net) for sd in seeds]}
     return out
 
 

@@ -9,9 +9,9 @@ the layered smooth-over-channel fields with wells of the large cases, a few
 grid sizes so that coarse levels appear too) the roots of each level are
 chosen, level by level, from Chebyshev candidates on [lo, hi] by the V-cycle
 contraction factor the solver measures on the GPU; the network learns
-features -> the best roots.  No reference counterpart; the solver keeps
-learned steps on a level only when they contract the V-cycle better than
-its default Chebyshev steps.
+features -> the best roots.  No reference counterpart; the solver judges
+the predicted roots level by level against the plain smoother (relax 6),
+which is also the fallback (jxamd_solver_apply_mlp).
 """
 import ctypes
 import math
