@@ -308,6 +308,14 @@ int jxamd_solver_num_levels(jxamd_solver *s, int *num_levels);
 #define JXAMD_MLP_NFEAT  16
 #define JXAMD_MLP_NOUT   4
 #define JXAMD_MLP_NPARAM 5508
+/* The training labels are roots of degree-m Chebyshev polynomials on
+ * [lo, hi] x lambda_max with lo in [JXAMD_MLP_LO_MIN, JXAMD_MLP_LO_MAX] and
+ * hi in [JXAMD_MLP_HI_MIN, JXAMD_MLP_HI_MAX]; a predicted root is clamped
+ * into the span of that family for its degree (its trust range). */
+#define JXAMD_MLP_LO_MIN 0.03
+#define JXAMD_MLP_LO_MAX 0.45
+#define JXAMD_MLP_HI_MIN 0.9
+#define JXAMD_MLP_HI_MAX 1.4
 /* y[n][4] = MLP(x[n][16]), device pointers */
 int jxamd_mlp_forward(const float *d_params, const float *d_x, int n, float *d_y, void *stream);
 /* steps Adam steps (lr, betas 0.9/0.999, eps 1e-8; update = 0 only computes
@@ -324,13 +332,16 @@ int jxamd_solver_poly_features(jxamd_solver *s, int level, float *feat);
 int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor);
 /* The learned smoother and its judge.  The plain smoother of the reference
  * (relax 6 as configured by relax_chunk / relax_threads) is both the
- * fallback and the judge: with the polynomial on every smoothing level,
- * each level in turn falls back to relax 6 whenever that gives a lower
+ * fallback and the judge: from relax 6 on every smoothing level, each
+ * level in turn switches to its polynomial smoother unless that makes the
  * measured V-cycle contraction factor (jxamd_solver_vcycle_factor, 4
- * cycles), i.e. whenever its polynomial's per-cycle reduction is worse;
- * levels whose predicted roots are out of range start on relax 6.
+ * cycles) worse, and stays on relax 6 otherwise (a second pass retries the
+ * levels left on relax 6); levels whose predicted roots are out of range
+ * stay on relax 6.  The result never contracts worse than relax 6.
  * apply_mlp: polynomial roots tau_k predicted by the MLP (d_params on the
- * device; w_k = 1 / (tau_k lambda_max), the first poly_degree <= 4 outputs).
+ * device; w_k = 1 / (tau_k lambda_max), the first poly_degree <= 4 outputs,
+ * each clamped into the span of the training labels' roots for that degree,
+ * JXAMD_MLP_LO_* / JXAMD_MLP_HI_*).
  * judge_poly: the polynomial steps in use (the default Chebyshev roots, or
  * what set_poly installed), i.e. the same method without the network.
  * *used = levels left on the polynomial; the final factor and that of

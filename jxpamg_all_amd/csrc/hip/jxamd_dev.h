@@ -31,7 +31,8 @@ int jxamd_host_dev_ptr(void **d, void *h);    /* its device address */
 int jxamd_host_free(void *p);
 int jxamd_last_error(void);
 
-int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
+/* unit: the layout leaves out identity diagonal blocks (y_i starts from x_i) */
+int jxamd_k_bsr_spmv(int b, int nb, int unit, const int *sptr, const int *col, const double *val, const double *x,
                      const double *rhs, double *y, int mode, double alpha, double beta, void *stream);
 /* storage / vector type codes of the V-cycle (matrix values vt, vectors xt):
  * (F64, F64), (F32, F32), (F16 with a float power-of-two scale per stored
@@ -76,8 +77,8 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
  * blocks vt = F64 (dinv F64), F32 (dinv F32) or F16 scaled by rsc per block
  * row (dinv F32) */
 int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const double *xp,
-                      int pidx, double *xn, double w, double omega, void *stream);
+                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const void *xp,
+                      int xp_f32, int pidx, double *xn, double w, double omega, void *stream);
 /* BILU(0) block triangular solve (vt: F64 or F32 factor storage) */
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                  const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
@@ -107,6 +108,10 @@ int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *i
 int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream);
 int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream);
 int jxamd_k_widen(int n, const float *x, double *y, void *stream);
+/* y = (float) x[i*stride + off] and un = omega dinv y (a first polynomial
+ * step from zero) in one launch */
+int jxamd_k_cast_first(int n, const double *x, int stride, int off, float *y, const float *dinv, float *un,
+                       double omega, void *stream);
 int jxamd_k_fill_f32(int n, float *y, float v, void *stream);
 int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream);
 int jxamd_k_prolong(int nb, int b, int pidx, const double *xp, double *x, void *stream);

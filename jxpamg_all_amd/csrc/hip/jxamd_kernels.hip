@@ -105,7 +105,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks)
 // BSR SpMV, SELL-64, one lane per block row.
 //   mode 0: y = A x;  mode 1: y = rhs - A x;  mode 2: y = alpha*A x + beta*y
 // ---------------------------------------------------------------------------
-template <int B, int SU>
+template <int B, int SU, bool UNIT>
 __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict__ sptr, const int *__restrict__ col,
                                                   const double *__restrict__ val, const double *__restrict__ x,
                                                   const double *__restrict__ rhs, double *__restrict__ y, int mode,
@@ -120,8 +120,10 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nb, const int *__restrict_
     if (s >= nsl) return;
     const int k0 = sptr[s], k1 = sptr[s + 1];
     double acc[B];
+    // UNIT: the identity diagonal block is not stored; the row starts from
+    // its product, x_i, exactly what the first slot would have added
 #pragma unroll
-    for (int r = 0; r < B; r++) acc[r] = 0.0;
+    for (int r = 0; r < B; r++) acc[r] = (UNIT && row < nb) ? x[(size_t)row * B + r] : 0.0;
     // matrix entries are streamed once: non-temporal loads keep them from
     // displacing the reused x entries in the caches; SU slots per batch
     for (int k = k0; k < k1; k += SU) {
@@ -573,17 +575,18 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
 // over the SpMV layout: SELL-64, all blocks, natural order); the in-partition
 // lower blocks go to LDS and one lane per partition runs the short forward
 // recurrence (phase B).  x_old: xo (B values per row) or, when xo ==
-// nullptr, the prolongated pressure xp (slot pidx, zeros elsewhere).
+// nullptr, the prolongated pressure xp (slot pidx, zeros elsewhere; XP
+// double, or float for a stage-1 result in FP32, widened as it is read).
 // Blocks are stored as V (double, float, or __half scaled per block row:
 // A_ij = rsc[i] * H_ij), inverted diagonal blocks as DV (dinv == nullptr:
 // every diagonal block is the identity, as after ABF decoupling, and none
 // is read); arithmetic in double.  PU slots per batch.
-template <int B, int LMAX, typename V, typename DV, int PU>
+template <int B, int LMAX, typename V, typename DV, int PU, typename XP>
 __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, const int *__restrict__ sptr,
                                                    const int *__restrict__ col, const V *__restrict__ val,
                                                    const float *__restrict__ rsc, const DV *__restrict__ dinv,
                                                    const double *__restrict__ rhs,
-                                                   const double *__restrict__ xo, const double *__restrict__ xp,
+                                                   const double *__restrict__ xo, const XP *__restrict__ xp,
                                                    int pidx, double *__restrict__ xn, double w, double omega)
 {
     constexpr int NL = LMAX * (LMAX - 1) / 2; // strictly lower pairs of a partition
@@ -633,7 +636,7 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
                 const int j = jj[u];
 #pragma unroll
                 for (int q = 0; q < B; q++)
-                    xj[u][q] = j == i ? 0.0 : (xo ? xo[(size_t)j * B + q] : (q == pidx ? xp[j] : 0.0));
+                    xj[u][q] = j == i ? 0.0 : (xo ? xo[(size_t)j * B + q] : (q == pidx ? (double)xp[j] : 0.0));
             }
 #pragma unroll
             for (int u = 0; u < PU; u++) {
@@ -663,7 +666,7 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
 #pragma unroll
         for (int q = 0; q < B; q++) {
             h[r * B + q] = omega * res[q] + res0[q] + (1.0 - omega) * res2[q];
-            xold[r * B + q] = xo ? xo[(size_t)i * B + q] : (q == pidx ? xp[i] : 0.0);
+            xold[r * B + q] = xo ? xo[(size_t)i * B + q] : (q == pidx ? (double)xp[i] : 0.0);
         }
     }
     __syncthreads();
@@ -1127,6 +1130,20 @@ __global__ void k_poly_first(int n, const X *__restrict__ dinv, const X *__restr
     un[i] = (X)0 + omega * dinv[i] * r;
 }
 
+// the cast of the restricted residual (k_gather_cast) and the first
+// polynomial step of level 0 from a zero guess (k_poly_first) in one pass:
+// y = (float) x[i*stride + off], un = 0 + omega dinv (y - 0)
+__global__ void k_cast_first(int n, const double *__restrict__ x, int stride, int off, float *__restrict__ y,
+                             const float *__restrict__ dinv, float *__restrict__ un, float omega)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float f = (float)x[(size_t)i * stride + off];
+    y[i] = f;
+    const float r = f - 0.0f;
+    un[i] = 0.0f + omega * dinv[i] * r;
+}
+
 __global__ void k_widen(int n, const float *__restrict__ x, double *__restrict__ y)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1495,7 +1512,7 @@ int jxamd_event_elapsed(float *ms, void *e0, void *e1)
 }
 int jxamd_last_error(void) { return chk(hipGetLastError()); }
 
-int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
+int jxamd_k_bsr_spmv(int b, int nb, int unit, const int *sptr, const int *col, const double *val, const double *x,
                      const double *rhs, double *y, int mode, double alpha, double beta, void *stream)
 {
     const int nsl = (nb + WAVE - 1) / WAVE;
@@ -1505,10 +1522,16 @@ int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const doubl
     switch (b) {
     // eight slots per batch: a 7-point row in one batch (3x3 blocks on the
     // 60x220x85 system: 117.8 us with one slot per step, 114.8 us with eight)
-    case 2: k_bsr_spmv<2, 8><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
-    case 3: k_bsr_spmv<3, 8><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
-    case 4: k_bsr_spmv<4, 8><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta); break;
+#define SPMV_B(BS)                                                                                                     \
+    do {                                                                                                               \
+        if (unit) k_bsr_spmv<BS, 8, true><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta);   \
+        else k_bsr_spmv<BS, 8, false><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta);        \
+    } while (0)
+    case 2: SPMV_B(2); break;
+    case 3: SPMV_B(3); break;
+    case 4: SPMV_B(4); break;
     default: return JXAMD_ERROR_ARG;
+#undef SPMV_B
     }
     return chk(hipGetLastError());
 }
@@ -1637,8 +1660,8 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
 }
 
 int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const double *xp,
-                      int pidx, double *xn, double w, double omega, void *stream)
+                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const void *xp,
+                      int xp_f32, int pidx, double *xn, double w, double omega, void *stream)
 {
     if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32 && vt != JXAMD_T_F16) return JXAMD_ERROR_ARG;
     if (nb == 0) return 0;
@@ -1649,22 +1672,27 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
     // 60x220x85 solve against one slot per step)
 #define LAUNCH_PART(BS, LM)                                                                                            \
     do {                                                                                                               \
+        if (xp_f32) LAUNCH_PART_X(BS, LM, float);                                                                     \
+        else LAUNCH_PART_X(BS, LM, double);                                                                            \
+    } while (0)
+#define LAUNCH_PART_X(BS, LM, XT)                                                                                      \
+    do {                                                                                                               \
         const int cpb = 256 / LM;                                                                                      \
         const int grid = (nparts + cpb - 1) / cpb;                                                                     \
         const size_t lb = (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1);                                          \
         const size_t lds = sizeof(double) * 2 * 256 * BS + (vt == JXAMD_T_F64 ? sizeof(double) : sizeof(float)) * lb; \
         if (vt == JXAMD_T_F64)                                                                                        \
-            k_bhgs_part<BS, LM, double, double, 4><<<grid, 256, lds, st>>>(                                            \
-                nb, nparts, cpb, sptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo, xp, pidx, xn, w,  \
-                omega);                                                                                                \
+            k_bhgs_part<BS, LM, double, double, 4, XT><<<grid, 256, lds, st>>>(                                        \
+                nb, nparts, cpb, sptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo, (const XT *)xp,   \
+                pidx, xn, w, omega);                                                                                   \
         else if (vt == JXAMD_T_F32)                                                                                    \
-            k_bhgs_part<BS, LM, float, float, 4><<<grid, 256, lds, st>>>(                                              \
-                nb, nparts, cpb, sptr, col, (const float *)val, rsc, (const float *)dinv, rhs, xo, xp, pidx, xn, w,    \
-                omega);                                                                                                \
+            k_bhgs_part<BS, LM, float, float, 4, XT><<<grid, 256, lds, st>>>(                                          \
+                nb, nparts, cpb, sptr, col, (const float *)val, rsc, (const float *)dinv, rhs, xo, (const XT *)xp,     \
+                pidx, xn, w, omega);                                                                                   \
         else                                                                                                           \
-            k_bhgs_part<BS, LM, __half, float, 4><<<grid, 256, lds, st>>>(                                             \
-                nb, nparts, cpb, sptr, col, (const __half *)val, rsc, (const float *)dinv, rhs, xo, xp, pidx, xn, w,   \
-                omega);                                                                                                \
+            k_bhgs_part<BS, LM, __half, float, 4, XT><<<grid, 256, lds, st>>>(                                         \
+                nb, nparts, cpb, sptr, col, (const __half *)val, rsc, (const float *)dinv, rhs, xo, (const XT *)xp,    \
+                pidx, xn, w, omega);                                                                                   \
     } while (0)
 #define PART_B(BS)                                                                                                     \
     do {                                                                                                               \
@@ -1680,6 +1708,7 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
     }
 #undef PART_B
 #undef LAUNCH_PART
+#undef LAUNCH_PART_X
     return chk(hipGetLastError());
 }
 
@@ -1874,6 +1903,13 @@ int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, v
     return chk(hipGetLastError());
 }
 
+int jxamd_k_cast_first(int n, const double *x, int stride, int off, float *y, const float *dinv, float *un,
+                       double omega, void *stream)
+{
+    if (n == 0) return 0;
+    k_cast_first<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, x, stride, off, y, dinv, un, (float)omega);
+    return chk(hipGetLastError());
+}
 int jxamd_k_widen(int n, const float *x, double *y, void *stream)
 {
     if (n == 0) return 0;

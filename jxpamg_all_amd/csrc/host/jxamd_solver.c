@@ -1084,14 +1084,23 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
     dbsr *d = &s->A;
     d->nb = nb;
     d->b = b;
-    /* natural-order SELL over all blocks (SpMV) */
+    /* every diagonal block exactly the identity (any ABF-decoupled system):
+     * the SpMV layout then leaves them out (y_i starts from x_i, which is
+     * A_ii x_i to the bit) and the partitioned sweeps skip Dinv */
+    d->unit_diag = 1;
+    for (int i = 0; i < nb && d->unit_diag; i++)
+        for (int r = 0; r < b; r++)
+            for (int c2 = 0; c2 < b; c2++)
+                if (A->a[(size_t)A->ia[i] * bb + r * b + c2] != (r == c2 ? 1.0 : 0.0)) d->unit_diag = 0;
+    const int skip = d->unit_diag;
+    /* natural-order SELL over all blocks but the identity diagonal ones (SpMV) */
     {
         hsell h;
         jxamd_csr pat = {nb, nb, A->nnzb, A->ia, A->ja, NULL};
         /* reuse the scalar builder for the pattern, then place block values */
         double *dummy = (double *)calloc((size_t)(A->nnzb > 0 ? A->nnzb : 1), sizeof(double));
         if (!dummy) return JXAMD_ERROR_MEMORY;
-        int e = hsell_build(&h, nb, pat.ia, pat.ja, dummy, NULL, 0, 1); /* padding: own row, zero block */
+        int e = hsell_build(&h, nb, pat.ia, pat.ja, dummy, NULL, skip, 1); /* padding: own row, zero block */
         free(dummy);
         if (e) { hsell_free(&h); return e; }
         size_t slots = (size_t)(h.nsl ? h.sptr[h.nsl] : 0);
@@ -1102,15 +1111,18 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             for (int l = 0; l < SELLW; l++) {
                 int i = sl * SELLW + l;
                 if (i >= nb) break;
-                int len = A->ia[i + 1] - A->ia[i];
+                int len = A->ia[i + 1] - A->ia[i] - skip;
                 for (int k = 0; k < w && k < len; k++) {
-                    const double *blk = A->a + (size_t)(A->ia[i] + k) * bb;
+                    const double *blk = A->a + (size_t)(A->ia[i] + skip + k) * bb;
                     for (int e2 = 0; e2 < bb; e2++) v[((size_t)(k0 + k) * bb + e2) * SELLW + l] = blk[e2];
                 }
             }
         }
         d->nsl = h.nsl;
-        d->spmv_bytes = (double)A->nnzb * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 16.0 * nb * b;
+        /* bytes the SpMV reads and writes: the stored blocks (without the
+         * identity diagonal ones) with their column indices, row pointers,
+         * x once, y once */
+        d->spmv_bytes = (double)(A->nnzb - (skip ? nb : 0)) * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 16.0 * nb * b;
         /* one fused one-rank sweep (k_bhgs_multi): off-diagonal blocks and
          * their columns, row pointers, x read once, rhs read, x written, Dinv */
         d->sweep_bytes = (double)(A->nnzb - nb) * (8.0 * bb + 4.0) + 4.0 * (nb + 1) + 24.0 * nb * b + 8.0 * nb * bb;
@@ -1160,18 +1172,13 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
             }
         }
         for (int i = 0; i < nb; i++) block_solve_matrix(A->a + (size_t)A->ia[i] * bb, b, dinv + (size_t)i * bb);
-        d->unit_diag = 1;
-        for (int i = 0; i < nb && d->unit_diag; i++)
-            for (int r = 0; r < b; r++)
-                for (int c2 = 0; c2 < b; c2++)
-                    if (A->a[(size_t)A->ia[i] * bb + r * b + c2] != (r == c2 ? 1.0 : 0.0)) d->unit_diag = 0;
-        /* one partitioned sweep (k_bhgs_part) reads every column index of
-         * the SpMV layout, every stored block but the identity diagonal ones
-         * (skipped before their load), row pointers, x once, rhs, writes x,
-         * and reads Dinv unless every diagonal block is the identity */
+        /* one partitioned sweep (k_bhgs_part) reads the SpMV layout (every
+         * stored block but the identity diagonal ones, with their column
+         * indices), row pointers, x once, rhs, writes x, and reads Dinv
+         * unless every diagonal block is the identity */
         {
             const double nblk = (double)A->nnzb - (d->unit_diag ? (double)nb : 0.0);
-            const double common = 4.0 * A->nnzb + 4.0 * (nb + 1) + 24.0 * nb * b;
+            const double common = 4.0 * nblk + 4.0 * (nb + 1) + 24.0 * nb * b;
             const double sv = s->prm.stage2_precision == 2 ? 2.0 : 4.0;
             d->part_bytes = common + nblk * 8.0 * bb + (d->unit_diag ? 0.0 : 8.0 * nb * bb);
             d->part_bytes32 = common + nblk * sv * bb + (d->unit_diag ? 0.0 : 4.0 * nb * bb) +
@@ -1969,25 +1976,37 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
      * into the V-cycle's precision */
     const void *f0 = r + pi;
     int fs0 = b;
+    int zero = 1;
     if (s->xt != JXAMD_T_F64 && s->prm.stage1_maxit > 0) {
-        CK(jxamd_k_gather_cast(s->nlev[0], r, b, pi, (float *)s->F[0], s->stream));
+        if (level_poly(s, 0) && s->L > 1) {
+            /* the cast and level 0's first polynomial step from zero in one launch */
+            int pq = prof_begin(s);
+            CK(jxamd_k_cast_first(s->nlev[0], r, b, pi, (float *)s->F[0], (const float *)s->dinvl[0],
+                                  (float *)s->T[0], s->poly_w[0][0], s->stream));
+            prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, 0, (8.0 + 4.0 * 3.0) * s->nlev[0]);
+            zero = 2;
+        } else {
+            CK(jxamd_k_gather_cast(s->nlev[0], r, b, pi, (float *)s->F[0], s->stream));
+        }
         f0 = s->F[0];
         fs0 = 1;
     }
-    int zero = 1;
     for (int it = 0; it < s->prm.stage1_maxit; it++) {
         CK(vcycle(s, f0, fs0, zero));
         zero = 0;
     }
-    double *xp = (double *)s->U[0];
-    if (s->xt != JXAMD_T_F64) {
-        xp = s->up64;
-        if (s->prm.stage1_maxit > 0) CK(jxamd_k_widen(s->nlev[0], (const float *)s->U[0], xp, s->stream));
-    }
-    if (s->prm.stage1_maxit == 0) CK(jxamd_k_fill(s->nlev[0], xp, 0.0, s->stream));
     /* prolongation (type 0) + stage-2 sweeps; the first sweep reads the
      * prolongated vector on the fly and the last one lands in x */
     const int m = s->prm.stage2_maxit;
+    /* the partitioned sweeps read an FP32 stage-1 result directly */
+    const int xp_f32 = s->xt != JXAMD_T_F64 && s->prm.stage1_maxit > 0 && m > 0 && s->prm.stage2_chunk > 0 &&
+                       s->prm.stage2_solver_type == 2;
+    double *xp = (double *)s->U[0];
+    if (s->xt != JXAMD_T_F64) {
+        xp = s->up64;
+        if (s->prm.stage1_maxit > 0 && !xp_f32) CK(jxamd_k_widen(s->nlev[0], (const float *)s->U[0], xp, s->stream));
+    }
+    if (s->prm.stage1_maxit == 0) CK(jxamd_k_fill(s->nlev[0], xp, 0.0, s->stream));
     if (m == 0 || s->prm.stage2_solver_type == 5) CK(jxamd_k_prolong(s->nb, b, pi, xp, x, s->stream));
     if (m == 0) return 0;
     if (s->prm.stage2_chunk > 0 && s->prm.stage2_solver_type == 2) {
@@ -2002,8 +2021,9 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
             const int nar = s->prm.stage2_precision; /* 0 FP64, 1 FP32, 2 FP16 blocks */
             CK(jxamd_k_bhgs_part(b, stage2_vt(nar), s->nb, parts, d->sptr, d->col,
                                  nar ? d->val32 : (const void *)d->val, d->rsc16,
-                                 d->unit_diag ? NULL : (nar ? d->dinv32 : (const void *)d->dinv), r, src, xp, pi, dst,
-                                 s->prm.stage2_weight, s->prm.stage2_omega, s->stream));
+                                 d->unit_diag ? NULL : (nar ? d->dinv32 : (const void *)d->dinv), r, src,
+                                 xp_f32 ? s->U[0] : (const void *)xp, xp_f32, pi, dst, s->prm.stage2_weight,
+                                 s->prm.stage2_omega, s->stream));
             prof_end(s, pq, JXAMD_PROF_BHGS_SWEEP, nar ? d->part_bytes32 : d->part_bytes);
             src = dst;
             dst = (dst == x) ? s->xtmp : x;
@@ -2048,7 +2068,7 @@ int jxamd_op_bsr_matvec(jxamd_solver *s, double alpha, const double *d_x, double
 {
     if (!s) return JXAMD_ERROR_ARG;
     dbsr *d = &s->A;
-    CK(jxamd_k_bsr_spmv(d->b, d->nb, d->sptr, d->col, d->val, d_x, NULL, d_y, 2, alpha, beta, s->stream));
+    CK(jxamd_k_bsr_spmv(d->b, d->nb, d->unit_diag, d->sptr, d->col, d->val, d_x, NULL, d_y, 2, alpha, beta, s->stream));
     return jxamd_stream_sync(s->stream);
 }
 
@@ -2058,7 +2078,7 @@ int jxamd_op_bsr_matvec_timed(jxamd_solver *s, const double *d_x, double *d_y, i
     dbsr *d = &s->A;
     CK(jxamd_event_record(s->ev0, s->stream));
     for (int r = 0; r < reps; r++)
-        CK(jxamd_k_bsr_spmv(d->b, d->nb, d->sptr, d->col, d->val, d_x, NULL, d_y, 0, 1.0, 0.0, s->stream));
+        CK(jxamd_k_bsr_spmv(d->b, d->nb, d->unit_diag, d->sptr, d->col, d->val, d_x, NULL, d_y, 0, 1.0, 0.0, s->stream));
     CK(jxamd_event_record(s->ev1, s->stream));
     CK(jxamd_stream_sync(s->stream));
     float ms = 0.f;
@@ -2208,45 +2228,43 @@ int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor)
     return 0;
 }
 
-/* The polynomial smoother with the step sizes w[l] on every smoothing level
- * (levels with ok[l] = 0 start on relax 6), judged level by level against
- * the plain smoother: a level falls back to relax 6 whenever that gives a
- * lower measured V-cycle contraction factor, i.e. whenever its polynomial's
- * per-cycle reduction is worse.  The plain smoother is both the fallback and
- * the judge. */
+/* The polynomial smoother with the step sizes w[l] (levels with ok[l] = 0
+ * excluded), judged level by level against the plain smoother, which is
+ * also the fallback: from relax 6 on every smoothing level, each level in
+ * turn switches to its polynomial unless that makes the measured V-cycle
+ * contraction factor worse (a second pass retries the levels still on
+ * relax 6 in the final context).  The result never contracts worse than
+ * relax 6 everywhere, and a level with a poor polynomial (a predicted root
+ * far outside the useful range) stays on relax 6 without disturbing the
+ * judgement of the others. */
 static int judge_levels(jxamd_solver *s, double (*w)[8], const int *ok, int *used, double *factor, double *fplain)
 {
     const int m = s->prm.poly_degree;
     const int Ls = s->L > 1 ? s->L - 1 : 1;
     for (int l = 0; l < Ls; l++) s->plain[l] = 1;
-    double fp = 0.0;
-    CK(jxamd_solver_vcycle_factor(s, 4, &fp));
-    if (fplain) *fplain = fp;
-    for (int l = 0; l < Ls; l++) {
-        if (!ok[l]) continue;
-        for (int k = 0; k < m; k++) s->poly_w[l][k] = w[l][k];
-        s->plain[l] = 0;
-    }
     double best = 0.0;
     CK(jxamd_solver_vcycle_factor(s, 4, &best));
+    if (fplain) *fplain = best;
     int kept = 0;
-    for (int l = 0; l < Ls; l++) {
-        if (s->plain[l]) continue;
-        s->plain[l] = 1;
-        double f = 0.0;
-        CK(jxamd_solver_vcycle_factor(s, 4, &f));
-        if (f < best) {
-            best = f; /* the polynomial is worse here: fall back to relax 6 */
-        } else {
+    for (int pass = 0; pass < 2; pass++) {
+        for (int l = 0; l < Ls; l++) {
+            if (!ok[l] || !s->plain[l]) continue;
+            double keep[8];
+            for (int k = 0; k < m; k++) {
+                keep[k] = s->poly_w[l][k];
+                s->poly_w[l][k] = w[l][k];
+            }
             s->plain[l] = 0;
-            kept++;
+            double f = 0.0;
+            CK(jxamd_solver_vcycle_factor(s, 4, &f));
+            if (f <= best) {
+                best = f;
+                kept++;
+            } else {
+                s->plain[l] = 1; /* worse per cycle: stay on relax 6 */
+                for (int k = 0; k < m; k++) s->poly_w[l][k] = keep[k];
+            }
         }
-    }
-    if (best > fp) {
-        /* worse than relax 6 on every level: fall back everywhere */
-        for (int l = 0; l < Ls; l++) s->plain[l] = 1;
-        best = fp;
-        kept = 0;
     }
     if (used) *used = kept;
     if (factor) *factor = best;
@@ -2300,11 +2318,18 @@ int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, do
      * w_k = 1 / (tau_k lambda_max) */
     double w[JXAMD_MAX_LEVELS][8];
     int ok[JXAMD_MAX_LEVELS];
+    /* the span of the roots of the label family for degree m: the smallest
+     * root of a Chebyshev polynomial on [lo, hi] is lo (1 + c) / 2 + hi (1 - c) / 2,
+     * the largest lo (1 - c) / 2 + hi (1 + c) / 2, c = cos(pi / (2m)) */
+    const double cm = cos(M_PI / (2.0 * m));
+    const double tmin = JXAMD_MLP_LO_MIN * (1.0 + cm) / 2.0 + JXAMD_MLP_HI_MIN * (1.0 - cm) / 2.0;
+    const double tmax = JXAMD_MLP_LO_MAX * (1.0 - cm) / 2.0 + JXAMD_MLP_HI_MAX * (1.0 + cm) / 2.0;
     for (int l = 0; l < Ls; l++) {
         ok[l] = 1;
         for (int k = 0; k < m; k++) {
-            const double tau = (double)hy[l * JXAMD_MLP_NOUT + k];
-            if (!(tau > 5e-3) || !(tau < 4.0)) ok[l] = 0; /* also rejects NaN */
+            double tau = (double)hy[l * JXAMD_MLP_NOUT + k];
+            if (tau != tau) ok[l] = 0; /* NaN: this level stays on relax 6 */
+            tau = tau < tmin ? tmin : (tau > tmax ? tmax : tau);
             w[l][k] = 1.0 / (tau * s->lam[l]);
         }
     }
@@ -2326,7 +2351,7 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
         const int nar = s->prm.stage2_precision;
         CK(jxamd_k_bhgs_part(s->b, stage2_vt(nar), s->nb, parts, s->A.sptr, s->A.col,
                              nar ? s->A.val32 : (const void *)s->A.val, s->A.rsc16,
-                             s->A.unit_diag ? NULL : (nar ? s->A.dinv32 : (const void *)s->A.dinv), d_b, d_x, NULL, 0,
+                             s->A.unit_diag ? NULL : (nar ? s->A.dinv32 : (const void *)s->A.dinv), d_b, d_x, NULL, 0, 0,
                              s->xtmp, w, omega, s->stream));
     } else {
         CK(bhgs_run(s, 1, d_b, d_x, NULL, s->xtmp, w, omega));
@@ -2352,7 +2377,7 @@ static int bsr_apply(jxamd_solver *s, const double *x, const double *rhs, double
 {
     dbsr *A = &s->A;
     int pq = prof_begin(s);
-    int e = jxamd_k_bsr_spmv(A->b, A->nb, A->sptr, A->col, A->val, x, rhs, y, mode, 0.0, 0.0, s->stream);
+    int e = jxamd_k_bsr_spmv(A->b, A->nb, A->unit_diag, A->sptr, A->col, A->val, x, rhs, y, mode, 0.0, 0.0, s->stream);
     prof_end(s, pq, JXAMD_PROF_BSR_SPMV, A->spmv_bytes + (mode == 1 ? 8.0 * s->N : 0.0));
     return e;
 }

@@ -31,9 +31,9 @@ int jxamd_host_free(void *p) { free(p); return 0; }
 int jxamd_host_dev_ptr(void **d, void *h) { *d = h; return 0; }
 int jxamd_last_error(void) { return 0; }
 
-int jxamd_k_bsr_spmv(int b, int nb, const int *sptr, const int *col, const double *val, const double *x,
+int jxamd_k_bsr_spmv(int b, int nb, int unit, const int *sptr, const int *col, const double *val, const double *x,
                      const double *rhs, double *y, int mode, double alpha, double beta, void *stream)
-{ (void)b; (void)nb; (void)sptr; (void)col; (void)val; (void)x; (void)rhs; (void)y; (void)mode; (void)alpha; (void)beta; (void)stream; return 0; }
+{ (void)b; (void)nb; (void)unit; (void)sptr; (void)col; (void)val; (void)x; (void)rhs; (void)y; (void)mode; (void)alpha; (void)beta; (void)stream; return 0; }
 int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
                      const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream)
 { (void)vt; (void)xt; (void)n; (void)sptr; (void)col; (void)val; (void)rsc; (void)x; (void)xs; (void)rhs; (void)rs; (void)y; (void)mode; (void)stream; return 0; }
@@ -89,6 +89,9 @@ int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, voi
 int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream)
 { (void)n; (void)x; (void)stride; (void)off; (void)y; (void)stream; return 0; }
 int jxamd_k_widen(int n, const float *x, double *y, void *stream) { (void)n; (void)x; (void)y; (void)stream; return 0; }
+int jxamd_k_cast_first(int n, const double *x, int stride, int off, float *y, const float *dinv, float *un,
+                       double omega, void *stream)
+{ (void)n; (void)x; (void)stride; (void)off; (void)y; (void)dinv; (void)un; (void)omega; (void)stream; return 0; }
 int jxamd_k_fill_f32(int n, float *y, float v, void *stream) { (void)n; (void)y; (void)v; (void)stream; return 0; }
 int jxamd_k_gather_stride(int n, const double *x, int stride, int off, double *y, void *stream)
 { (void)n; (void)x; (void)stride; (void)off; (void)y; (void)stream; return 0; }
@@ -136,6 +139,6 @@ int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *i
 { (void)G; (void)vt; (void)xt; (void)n; (void)nchunks; (void)ia; (void)ja; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
 
 int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const double *xp,
-                      int pidx, double *xn, double w, double omega, void *stream)
-{ (void)b; (void)vt; (void)nb; (void)nparts; (void)sptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }
+                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const void *xp,
+                      int xp_f32, int pidx, double *xn, double w, double omega, void *stream)
+{ (void)b; (void)vt; (void)nb; (void)nparts; (void)sptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)xp_f32; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }

@@ -330,3 +330,14 @@ def test_bilu0_factor_rejects_unordered_rows():
     B = jx.BSRMatrix(A.nb, A.b, A.ia, ja, A.a)
     with pytest.raises(RuntimeError):
         jx.bilu0_factor(B)
+
+
+def test_learned_root_trust_range_matches_the_label_family():
+    """the clamp range of predicted roots (jxamd.h) is the candidate family
+    the training labels are drawn from (learned.py LO, HI)"""
+    learned = pytest.importorskip("jxpamg_all_amd.learned")
+    hdr = open(os.path.join(ROOT, "include", "jxamd.h")).read()
+    val = {k: float(re.search(rf"#define {k}\s+([0-9.]+)", hdr).group(1))
+           for k in ("JXAMD_MLP_LO_MIN", "JXAMD_MLP_LO_MAX", "JXAMD_MLP_HI_MIN", "JXAMD_MLP_HI_MAX")}
+    assert (min(learned.LO), max(learned.LO)) == (val["JXAMD_MLP_LO_MIN"], val["JXAMD_MLP_LO_MAX"])
+    assert (min(learned.HI), max(learned.HI)) == (val["JXAMD_MLP_HI_MIN"], val["JXAMD_MLP_HI_MAX"])
