@@ -299,7 +299,7 @@ def main():
         legs = {}
         stage2_ref = dict(stage2_maxit=2, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=1)
         stage2_one = dict(stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8, stage2_chunk=s2chunk, stage2_precision=2)
-        mixed_kw = dict(precond_precision=2, poly_degree=2, relax_chunk=chunk, **stage2_one)
+        mixed_kw = dict(precond_precision=2, poly_degree=2, relax_chunk=chunk, coarse_rows=4096, **stage2_one)
         runs = [("mixed_fp32_relax6", dict(precond_precision=1, relax_chunk=chunk, **stage2_ref)),
                 ("mixed_default_poly", mixed_kw)]
         if net is not None:
@@ -310,15 +310,17 @@ def main():
             used = None
             if name in ("mixed_default_poly", "mixed_learned"):
                 # judged level by level against relax 6, the fallback
-                used, f_j, f_p = sm.apply_mlp(net.P.data_ptr()) if name == "mixed_learned" else sm.judge_poly()
+                _, f_j, f_p = sm.apply_mlp(net.P.data_ptr()) if name == "mixed_learned" else sm.judge_poly()
+                kinds = sm.level_smoothers()[:-1]
+                used = {"relax6": kinds.count("R"), "default_roots": kinds.count("D"), "learned_roots": kinds.count("L")}
                 log(f"{name}: polynomial on {used} level(s), V-cycle factor {f_j:.4f} (relax 6 everywhere "
-                    f"{f_p:.4f}), relax-6 levels {[l for l, v in enumerate(sm.level_smoothers()[:-1]) if v]}")
+                    f"{f_p:.4f}), level smoothers {sm.level_smoothers()[:-1]} (R relax 6, D default roots, L learned)")
             t_m, st_m = timed_solves(sm, args.steps)
             assert st_m.converged == 1 and st_m.true_rel_res <= 1e-6, f"{name} did not reach rtol 1e-6"
             legs[name] = {"s": round(t_m, 4), "iterations": st_m.iterations, "true_rel_res": st_m.true_rel_res,
                           "speedup_vs_plain_fp64": round(sec / t_m, 3)}
             if used is not None:
-                legs[name]["polynomial_levels"] = used
+                legs[name]["level_smoothers"] = used
                 legs[name]["vcycle_factor"] = round(f_j, 4)
                 legs[name]["vcycle_factor_relax6"] = round(f_p, 4)
             log(f"{name}: {t_m * 1e3:.1f} ms iters {st_m.iterations} true_rel {st_m.true_rel_res:.3e}")

@@ -224,6 +224,12 @@ typedef struct {
      * factor, JXF_GMRESSetConvFacThreshold (pgmres.c:201, default 1e4 at
      * :449, test at :1325-1333) */
     double convfac_threshold;
+    /* V-cycle depth: 0 = every level of the imported hierarchy (the
+     * reference's coarsest level, at most 1000 rows, jxf_cpr.c:346); c > 0
+     * stops at the first level with at most c rows (c <= 8192) and solves it
+     * with an explicit dense inverse (one GEMV per cycle), dropping the
+     * levels below it.  No reference counterpart. */
+    int coarse_rows;
 } jxamd_params;
 
 /* Values of src/test_bsr_cprgmres.c:317-367 with tol 1e-6 and one-thread
@@ -332,24 +338,29 @@ int jxamd_solver_poly_features(jxamd_solver *s, int level, float *feat);
 int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor);
 /* The learned smoother and its judge.  The plain smoother of the reference
  * (relax 6 as configured by relax_chunk / relax_threads) is both the
- * fallback and the judge: from relax 6 on every smoothing level, each
- * level in turn switches to its polynomial smoother unless that makes the
- * measured V-cycle contraction factor (jxamd_solver_vcycle_factor, 4
- * cycles) worse, and stays on relax 6 otherwise (a second pass retries the
- * levels left on relax 6); levels whose predicted roots are out of range
- * stay on relax 6.  The result never contracts worse than relax 6.
- * apply_mlp: polynomial roots tau_k predicted by the MLP (d_params on the
- * device; w_k = 1 / (tau_k lambda_max), the first poly_degree <= 4 outputs,
- * each clamped into the span of the training labels' roots for that degree,
- * JXAMD_MLP_LO_* / JXAMD_MLP_HI_*).
- * judge_poly: the polynomial steps in use (the default Chebyshev roots, or
- * what set_poly installed), i.e. the same method without the network.
- * *used = levels left on the polynomial; the final factor and that of
- * relax 6 on every level are reported if non-NULL. */
+ * fallback and the judge.  Each smoothing level has up to two candidate
+ * polynomials: the network's roots (apply_mlp only) and the default
+ * Chebyshev roots.  From relax 6 on every smoothing level, each level in
+ * turn takes its candidate with the lower measured V-cycle contraction
+ * factor (jxamd_solver_vcycle_factor, 4 cycles) unless that is worse than
+ * relax 6 there, in which case it stays on relax 6 (a second pass retries
+ * the levels left on relax 6).  The result never contracts worse than
+ * relax 6 everywhere.
+ * apply_mlp: the network's roots tau_k (d_params on the device; w_k =
+ * 1 / (tau_k lambda_max), the first poly_degree <= 4 outputs, each clamped
+ * into the span of the training labels' roots for that degree,
+ * JXAMD_MLP_LO_* / JXAMD_MLP_HI_*); *used = levels on the network's roots.
+ * judge_poly: the default Chebyshev roots alone, i.e. the same method
+ * without the network; *used = levels on the polynomial.
+ * The final factor and that of relax 6 on every level are reported if
+ * non-NULL. */
 int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, double *factor_learned,
                            double *factor_plain);
 int jxamd_solver_judge_poly(jxamd_solver *s, int *used, double *factor, double *factor_plain);
-/* a level's smoother when poly_degree > 0: plain = 1 relax 6, 0 the polynomial */
+/* a level's smoother when poly_degree > 0: set plain = 1 relax 6, 0 the
+ * polynomial with the steps in use; get: 1 relax 6, 0 a polynomial on the
+ * default (or set_poly) steps, 2 a polynomial on the network's roots, 3 the
+ * coarsest level (a direct solve) */
 int jxamd_solver_set_level_smoother(jxamd_solver *s, int level, int plain);
 int jxamd_solver_get_level_smoother(jxamd_solver *s, int level, int *plain);
 

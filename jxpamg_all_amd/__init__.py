@@ -58,7 +58,8 @@ class Params(ctypes.Structure):
                 ("stage2_omega", ctypes.c_double), ("relax_threads", ctypes.c_int), ("relax_chunk", ctypes.c_int),
                 ("precond_precision", ctypes.c_int), ("poly_degree", ctypes.c_int), ("stage2_chunk", ctypes.c_int),
                 ("stage2_precision", ctypes.c_int), ("arnoldi_kernel", ctypes.c_int),
-                ("solution_update", ctypes.c_int), ("convfac_threshold", ctypes.c_double)]
+                ("solution_update", ctypes.c_int), ("convfac_threshold", ctypes.c_double),
+                ("coarse_rows", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -510,13 +511,15 @@ class Solver:
               "jxamd_solver_set_level_smoother")
 
     def level_smoothers(self):
-        """[1 if level l smooths with relax 6 else 0 for each level]"""
+        """per level: "R" relax 6, "D" a polynomial on the default (or set)
+        roots, "L" a polynomial on the network's roots, "C" the coarsest
+        level (a direct solve)"""
         out = []
         for l in range(self.num_levels):
             v = ctypes.c_int()
             check(self._L.jxamd_solver_get_level_smoother(self._s, l, ctypes.byref(v)), "get_level_smoother")
-            out.append(v.value)
-        return out
+            out.append("DRLC"[v.value])
+        return "".join(out)
 
     def bilu_solve(self, d_r, d_z):
         check(self._L.jxamd_op_bilu_solve(self._s, ctypes.c_void_p(d_r), ctypes.c_void_p(d_z)), "jxamd_op_bilu_solve")

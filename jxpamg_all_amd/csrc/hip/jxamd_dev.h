@@ -105,7 +105,9 @@ int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *sptr, const i
 int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *ia, const int *ja, const void *val,
                          const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
                          void *stream);
-int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream);
+/* y = M x, M n x n row-major with rows ld apart (ld a multiple of 4 >= n,
+ * zeros beyond column n); x readable up to ld entries */
+int jxamd_k_dense_gemv(int xt, int n, int ld, const void *M, const void *x, void *y, void *stream);
 int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream);
 int jxamd_k_widen(int n, const float *x, double *y, void *stream);
 /* y = (float) x[i*stride + off] and un = omega dinv y (a first polynomial

@@ -89,6 +89,11 @@ template <> struct row_scaled<__half> { static constexpr bool value = true; };
 template <typename V> struct lds_block { typedef float T; };
 template <> struct lds_block<double> { typedef double T; };
 
+// 16-byte vector of X
+template <typename X> struct vec16;
+template <> struct vec16<double> { typedef double __attribute__((ext_vector_type(2))) T; };
+template <> struct vec16<float> { typedef float __attribute__((ext_vector_type(4))) T; };
+
 // XCD-aware tile order for streaming kernels: consecutive tiles land on the
 // same XCD (dispatch is round-robin over the 8 XCDs), so the x entries a
 // stencil row shares with its neighbours stay in one L2.
@@ -1098,14 +1103,38 @@ __global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cp
 
 // dense y = M x (row-major n x n), one wavefront per row
 template <typename X>
-__global__ void __launch_bounds__(256) k_dense_gemv(int n, const X *__restrict__ M, const X *__restrict__ x,
+__global__ void __launch_bounds__(256) k_dense_gemv(int n, int ld, const X *__restrict__ M, const X *__restrict__ x,
                                                     X *__restrict__ y)
 {
+    // one wavefront per row; rows padded to ld (a multiple of 16 bytes), so
+    // every lane loads 16-byte vectors, four of them before it adds
+    typedef typename vec16<X>::T V;
+    constexpr int E = sizeof(V) / sizeof(X);
     const int row = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (row >= n) return;
+    const V *m = reinterpret_cast<const V *>(M + (size_t)row * ld);
+    const V *xv = reinterpret_cast<const V *>(x);
+    const int nv = ld / E;
     X acc = 0;
-    for (int j = lane; j < n; j += WAVE) acc += M[(size_t)row * n + j] * x[j];
+    int j = lane;
+    for (; j + 3 * WAVE < nv; j += 4 * WAVE) {
+        V a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a[u] = __builtin_nontemporal_load(m + j + u * WAVE);
+            b[u] = xv[j + u * WAVE];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int e = 0; e < E; e++) acc += a[u][e] * b[u][e];
+    }
+    for (; j < nv; j += WAVE) {
+        const V a = __builtin_nontemporal_load(m + j), b = xv[j];
+#pragma unroll
+        for (int e = 0; e < E; e++) acc += a[e] * b[e];
+    }
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) y[row] = acc;
@@ -1883,14 +1912,14 @@ int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *i
     return chk(hipGetLastError());
 }
 
-int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream)
+int jxamd_k_dense_gemv(int xt, int n, int ld, const void *M, const void *x, void *y, void *stream)
 {
     if (n == 0) return 0;
-    if (xt == JXAMD_T_F64)
-        k_dense_gemv<double><<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, (const double *)M, (const double *)x,
-                                                                            (double *)y);
-    else if (xt == JXAMD_T_F32)
-        k_dense_gemv<float><<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, (const float *)M, (const float *)x,
+    if (xt == JXAMD_T_F64 && ld % 2 == 0 && ld >= n)
+        k_dense_gemv<double><<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, ld, (const double *)M,
+                                                                            (const double *)x, (double *)y);
+    else if (xt == JXAMD_T_F32 && ld % 4 == 0 && ld >= n)
+        k_dense_gemv<float><<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(n, ld, (const float *)M, (const float *)x,
                                                                            (float *)y);
     else return JXAMD_ERROR_ARG;
     return chk(hipGetLastError());

@@ -102,7 +102,7 @@ struct jxamd_solver_s {
     int nchunks[JXAMD_MAX_LEVELS]; /* 0: sequential sweeps on this level */
     dsell R[JXAMD_MAX_LEVELS], P[JXAMD_MAX_LEVELS];
     void *cinv;            /* coarsest inverse, type xt */
-    int nc;
+    int nc, ncld;          /* coarsest size, row stride of its inverse */
     int vt, xt;            /* V-cycle storage / vector types (JXAMD_T_*) */
     size_t xsz;            /* bytes per V-cycle vector entry */
     void *F[JXAMD_MAX_LEVELS], *U[JXAMD_MAX_LEVELS], *T[JXAMD_MAX_LEVELS], *Vt[JXAMD_MAX_LEVELS];
@@ -110,6 +110,8 @@ struct jxamd_solver_s {
     dtri Lf, Ub;           /* stage-2 BILU(0) factors (stage2_solver_type 5) */
     void *dinvl[JXAMD_MAX_LEVELS];          /* polynomial smoother: 1/a_ii (0 if a_ii = 0), type xt */
     int plain[JXAMD_MAX_LEVELS];            /* with poly_degree > 0: 1 = this level smooths with relax 6 */
+    int learned[JXAMD_MAX_LEVELS];          /* 1 = the level's polynomial uses the network's roots */
+    double poly_def[JXAMD_MAX_LEVELS][8];   /* the default Chebyshev steps */
     double poly_w[JXAMD_MAX_LEVELS][8];     /* its step sizes */
     double lam[JXAMD_MAX_LEVELS];           /* lambda_max(D^{-1} A_l) estimate */
     float feat[JXAMD_MAX_LEVELS][JXAMD_MLP_NFEAT]; /* level features of the learned step sizes */
@@ -895,6 +897,9 @@ static int dense_inverse(double *M, int n)
         double d = 1.0 / M[(size_t)k * n + k];
         M[(size_t)k * n + k] = 1.0;
         for (int j = 0; j < n; j++) M[(size_t)k * n + j] *= d;
+        /* the row updates of one pivot step are independent: host threads
+         * share them on large coarsest levels (same arithmetic per entry) */
+#pragma omp parallel for schedule(static) if (n >= 512)
         for (int i = 0; i < n; i++) {
             if (i == k) continue;
             double f = M[(size_t)i * n + k];
@@ -1378,15 +1383,26 @@ static int build_transfer(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         for (int i = 0; i < n; i++)
             for (int k = A->ia[i]; k < A->ia[i + 1]; k++) M[(size_t)i * n + A->ja[k]] = A->a[k];
         e = dense_inverse(M, n);
-        if (!e && s->xt != JXAMD_T_F64) {
-            float *Mf = (float *)M; /* in place: element k of the float array lies before element k of the double one */
-            for (size_t k = 0; k < (size_t)n * n; k++) Mf[k] = (float)M[k];
+        /* rows padded to ld entries (a multiple of four, zeros beyond n) so
+         * the GEMV reads 16-byte vectors */
+        const int ld = (n + 3) & ~3;
+        void *Mp = e ? NULL : calloc((size_t)ld * n + 1, s->xsz);
+        if (!e && !Mp) e = JXAMD_ERROR_MEMORY;
+        if (!e) {
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++) {
+                    const double v = M[(size_t)i * n + j];
+                    if (s->xt == JXAMD_T_F64) ((double *)Mp)[(size_t)i * ld + j] = v;
+                    else ((float *)Mp)[(size_t)i * ld + j] = (float)v;
+                }
         }
-        if (!e) e = jxamd_dev_malloc(&s->cinv, s->xsz * ((size_t)n * n + 1));
-        if (!e) e = jxamd_h2d(s->cinv, M, s->xsz * (size_t)n * n, s->stream);
+        if (!e) e = jxamd_dev_malloc(&s->cinv, s->xsz * ((size_t)ld * n + 1));
+        if (!e) e = jxamd_h2d(s->cinv, Mp, s->xsz * (size_t)ld * n, s->stream);
         if (!e) e = jxamd_stream_sync(s->stream);
         free(M);
+        free(Mp);
         s->nc = n;
+        s->ncld = ld;
         CK(e);
     }
     return 0;
@@ -1479,7 +1495,7 @@ static int build_poly(jxamd_solver *s, int l, const jxamd_hierarchy *H)
     }
     const int m = s->prm.poly_degree;
     const double lo = 0.3 * lam, hi = 1.1 * lam, c = 0.5 * (hi + lo), d = 0.5 * (hi - lo);
-    for (int k = 0; k < m; k++) s->poly_w[l][k] = 1.0 / (c + d * cos(M_PI * (2.0 * k + 1.0) / (2.0 * m)));
+    for (int k = 0; k < m; k++) s->poly_w[l][k] = s->poly_def[l][k] = 1.0 / (c + d * cos(M_PI * (2.0 * k + 1.0) / (2.0 * m)));
     int e;
     if (s->xt == JXAMD_T_F64) {
         e = jxamd_dev_malloc(&s->dinvl[l], sizeof(double) * (n > 0 ? n : 1));
@@ -1606,6 +1622,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->arnoldi_kernel < 0 || p->arnoldi_kernel > 1) return JXAMD_ERROR_ARG;
     if (p->solution_update < 0 || p->solution_update > 1) return JXAMD_ERROR_ARG;
     if (!(p->convfac_threshold > 0.0)) return JXAMD_ERROR_ARG;
+    if (p->coarse_rows < 0 || p->coarse_rows > 8192) return JXAMD_ERROR_ARG;
     {
         jxamd_csr pat = {A->nb, A->nb, A->nnzb, A->ia, A->ja, A->a};
         if (A->ia[A->nb] != A->nnzb) return JXAMD_ERROR_ARG;
@@ -1631,6 +1648,12 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     s->b = A->b;
     s->N = (size_t)A->nb * A->b;
     s->L = H->num_levels;
+    if (p->coarse_rows > 0)
+        for (int l = 0; l < H->num_levels; l++)
+            if (H->A[l].n <= p->coarse_rows) {
+                s->L = l + 1; /* this level becomes the coarsest: explicit inverse */
+                break;
+            }
     s->vt = p->precond_precision == 2 ? JXAMD_T_F16 : (p->precond_precision == 1 ? JXAMD_T_F32 : JXAMD_T_F64);
     s->xt = p->precond_precision == 0 ? JXAMD_T_F64 : JXAMD_T_F32;
     s->xsz = type_size(s->xt);
@@ -1653,11 +1676,16 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
         e = build_level(s, l, H);
     }
     for (int l = 0; l < s->L && !e; l++) {
-        size_t n = (size_t)s->nlev[l] + 1;
+        /* four entries beyond the level size, zero (the coarsest level's
+         * GEMV reads its right-hand side in 16-byte vectors up to a multiple
+         * of four entries) */
+        size_t n = (size_t)s->nlev[l] + 4;
         e = jxamd_dev_malloc(&s->F[l], s->xsz * n);
         if (!e) e = jxamd_dev_malloc(&s->U[l], s->xsz * n);
         if (!e) e = jxamd_dev_malloc(&s->T[l], s->xsz * n);
         if (!e) e = jxamd_dev_malloc(&s->Vt[l], s->xsz * n);
+        if (!e) e = s->xt == JXAMD_T_F64 ? jxamd_k_fill(n, (double *)s->F[l], 0.0, s->stream)
+                                         : jxamd_k_fill_f32((int)n, (float *)s->F[l], 0.0f, s->stream);
     }
     if (!e && s->xt != JXAMD_T_F64) e = jxamd_dev_malloc((void **)&s->up64, sizeof(double) * ((size_t)s->nb + 1));
     const size_t N = s->N;
@@ -1919,7 +1947,7 @@ static int vcycle(jxamd_solver *s, const void *f0, int fs0, int u0_zero)
         CK(restrict_launch(s, l));
     }
     int pc = prof_begin(s);
-    CK(jxamd_k_dense_gemv(s->xt, s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
+    CK(jxamd_k_dense_gemv(s->xt, s->nc, s->ncld, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
     prof_end(s, pc, JXAMD_PROF_COARSE, (double)s->xsz * s->nc * ((double)s->nc + 2.0));
     for (int l = L - 2; l >= 0; l--) {
         CK(csr_launch(s, &s->P[l], s->U[l + 1], NULL, 1, s->U[l], 2));
@@ -2129,7 +2157,7 @@ int jxamd_op_coarse_solve(jxamd_solver *s, const double *d_f, double *d_u)
     if (!s || s->L < 2) return JXAMD_ERROR_ARG;
     const int L = s->L;
     CK(to_level(s, s->nc, d_f, s->F[L - 1]));
-    CK(jxamd_k_dense_gemv(s->xt, s->nc, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
+    CK(jxamd_k_dense_gemv(s->xt, s->nc, s->ncld, s->cinv, s->F[L - 1], s->U[L - 1], s->stream));
     CK(from_level(s, s->nc, s->U[L - 1], d_u));
     return jxamd_stream_sync(s->stream);
 }
@@ -2228,45 +2256,66 @@ int jxamd_solver_vcycle_factor(jxamd_solver *s, int ncycles, double *factor)
     return 0;
 }
 
-/* The polynomial smoother with the step sizes w[l] (levels with ok[l] = 0
- * excluded), judged level by level against the plain smoother, which is
- * also the fallback: from relax 6 on every smoothing level, each level in
- * turn switches to its polynomial unless that makes the measured V-cycle
- * contraction factor worse (a second pass retries the levels still on
- * relax 6 in the final context).  The result never contracts worse than
- * relax 6 everywhere, and a level with a poor polynomial (a predicted root
- * far outside the useful range) stays on relax 6 without disturbing the
- * judgement of the others. */
-static int judge_levels(jxamd_solver *s, double (*w)[8], const int *ok, int *used, double *factor, double *fplain)
+/* Judge of the polynomial smoother against the plain smoother, which is also
+ * the fallback.  Each level has up to two candidate step sets: w[l] (the
+ * network's roots, where ok[l]; none when w is NULL) and the default
+ * Chebyshev steps.  From relax 6 on every smoothing level, each level in
+ * turn takes the candidate with the lower measured V-cycle contraction
+ * factor unless that is worse than keeping relax 6 there (a second pass
+ * retries the levels left on relax 6 in the final context).  The result
+ * never contracts worse than relax 6 everywhere; a level whose candidates
+ * are poor stays on relax 6 without disturbing the judgement of the others.
+ * *used = levels on the network's roots, *used_default = on the defaults. */
+static int judge_levels(jxamd_solver *s, double (*w)[8], const int *ok, int *used, int *used_default, double *factor,
+                        double *fplain)
 {
     const int m = s->prm.poly_degree;
     const int Ls = s->L > 1 ? s->L - 1 : 1;
-    for (int l = 0; l < Ls; l++) s->plain[l] = 1;
+    for (int l = 0; l < Ls; l++) {
+        s->plain[l] = 1;
+        s->learned[l] = 0;
+    }
     double best = 0.0;
     CK(jxamd_solver_vcycle_factor(s, 4, &best));
     if (fplain) *fplain = best;
-    int kept = 0;
     for (int pass = 0; pass < 2; pass++) {
         for (int l = 0; l < Ls; l++) {
-            if (!ok[l] || !s->plain[l]) continue;
-            double keep[8];
-            for (int k = 0; k < m; k++) {
-                keep[k] = s->poly_w[l][k];
-                s->poly_w[l][k] = w[l][k];
+            if (!s->plain[l]) continue;
+            double fbest = 0.0, wbest[8];
+            int have = 0, lrn = 0;
+            for (int cand = 0; cand < 2; cand++) {
+                const double *wc = cand == 0 ? (w && ok[l] ? w[l] : NULL) : s->poly_def[l];
+                if (!wc) continue;
+                for (int k = 0; k < m; k++) s->poly_w[l][k] = wc[k];
+                s->plain[l] = 0;
+                double f = 0.0;
+                CK(jxamd_solver_vcycle_factor(s, 4, &f));
+                if (!have || f < fbest) {
+                    fbest = f;
+                    lrn = cand == 0;
+                    for (int k = 0; k < m; k++) wbest[k] = wc[k];
+                    have = 1;
+                }
             }
-            s->plain[l] = 0;
-            double f = 0.0;
-            CK(jxamd_solver_vcycle_factor(s, 4, &f));
-            if (f <= best) {
-                best = f;
-                kept++;
+            if (have && fbest <= best) {
+                best = fbest;
+                s->plain[l] = 0;
+                s->learned[l] = lrn;
+                for (int k = 0; k < m; k++) s->poly_w[l][k] = wbest[k];
             } else {
                 s->plain[l] = 1; /* worse per cycle: stay on relax 6 */
-                for (int k = 0; k < m; k++) s->poly_w[l][k] = keep[k];
+                for (int k = 0; k < m; k++) s->poly_w[l][k] = s->poly_def[l][k];
             }
         }
     }
-    if (used) *used = kept;
+    int nl = 0, nd = 0;
+    for (int l = 0; l < Ls; l++)
+        if (!s->plain[l]) {
+            if (s->learned[l]) nl++;
+            else nd++;
+        }
+    if (used) *used = nl;
+    if (used_default) *used_default = nd;
     if (factor) *factor = best;
     return 0;
 }
@@ -2274,26 +2323,22 @@ static int judge_levels(jxamd_solver *s, double (*w)[8], const int *ok, int *use
 int jxamd_solver_judge_poly(jxamd_solver *s, int *used, double *factor, double *factor_plain)
 {
     if (!s || s->prm.poly_degree < 1) return JXAMD_ERROR_ARG;
-    double w[JXAMD_MAX_LEVELS][8];
-    int ok[JXAMD_MAX_LEVELS];
-    for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
-        ok[l] = 1;
-        for (int k = 0; k < 8; k++) w[l][k] = s->poly_w[l][k];
-    }
-    return judge_levels(s, w, ok, used, factor, factor_plain);
+    return judge_levels(s, NULL, NULL, NULL, used, factor, factor_plain);
 }
 
 int jxamd_solver_set_level_smoother(jxamd_solver *s, int level, int plain)
 {
     if (!s || s->prm.poly_degree < 1 || level < 0 || level >= s->L) return JXAMD_ERROR_ARG;
     s->plain[level] = plain ? 1 : 0;
+    s->learned[level] = 0;
     return 0;
 }
 
 int jxamd_solver_get_level_smoother(jxamd_solver *s, int level, int *plain)
 {
     if (!s || !plain || level < 0 || level >= s->L) return JXAMD_ERROR_ARG;
-    *plain = s->prm.poly_degree < 1 || s->plain[level];
+    if (s->L > 1 && level == s->L - 1) *plain = 3; /* the coarsest level: direct solve */
+    else *plain = (s->prm.poly_degree < 1 || s->plain[level]) ? 1 : (s->learned[level] ? 2 : 0);
     return 0;
 }
 
@@ -2333,7 +2378,7 @@ int jxamd_solver_apply_mlp(jxamd_solver *s, const float *d_params, int *used, do
             w[l][k] = 1.0 / (tau * s->lam[l]);
         }
     }
-    return judge_levels(s, w, ok, used, factor_learned, factor_plain);
+    return judge_levels(s, w, ok, used, NULL, factor_learned, factor_plain);
 }
 
 int jxamd_op_bilu_solve(jxamd_solver *s, const double *d_r, double *d_z)

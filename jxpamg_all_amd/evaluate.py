@@ -45,9 +45,10 @@ def _solve(Ad, bd, H, params, reps=2, mlp=None, judge=None):
             s.set_level_smoother(l, 1)
         info = {"polynomial_levels": 0, "factor": round(s.vcycle_factor(), 4)}
     elif judge in ("default", "learned"):
-        used, f, fp = s.apply_mlp(mlp.P.data_ptr()) if judge == "learned" else s.judge_poly()
-        info = {"polynomial_levels": used, "factor": round(f, 4), "factor_relax6": round(fp, 4),
-                "relax6_levels": [l for l, v in enumerate(s.level_smoothers()[:-1]) if v]}
+        _, f, fp = s.apply_mlp(mlp.P.data_ptr()) if judge == "learned" else s.judge_poly()
+        kinds = s.level_smoothers()[:-1]
+        info = {"polynomial_levels": sum(c != "R" for c in kinds), "learned_levels": kinds.count("L"),
+                "factor": round(f, 4), "factor_relax6": round(fp, 4), "level_smoothers": kinds}
     b = torch.tensor(bd, dtype=torch.float64, device="cuda")
     x = torch.zeros_like(b)
     hist = np.zeros(params.max_iter + 1)
@@ -108,12 +109,14 @@ def evaluate_c4():
 # level matrices, FP32 vectors), degree-2 polynomial smoother, one stage-2
 # sweep (w 1.0, omega 0.8) with FP16 blocks; GMRES FP64
 MIXED = dict(precond_precision=2, poly_degree=2, stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8,
-             stage2_chunk=4, stage2_precision=2, relax_chunk=4)
+             stage2_chunk=4, stage2_precision=2, relax_chunk=4, coarse_rows=4096)
 # the 120x440x85 learned-smoother configuration: FP32 preconditioner (stage-1
 # level matrices, vectors and accumulation; FP32 stage-2 blocks), degree-2
-# polynomial smoother judged against relax 6 (chunks of 4 rows) level by level
+# polynomial smoother judged against relax 6 (chunks of 4 rows) level by
+# level, the V-cycle ending at the first level of at most 4096 rows with an
+# explicit inverse
 C5_MIXED = dict(precond_precision=1, poly_degree=2, stage2_maxit=1, stage2_weight=1.0, stage2_omega=0.8,
-                stage2_chunk=4, stage2_precision=1, relax_chunk=4)
+                stage2_chunk=4, stage2_precision=1, relax_chunk=4, coarse_rows=4096)
 
 
 def train_c5_network(train_seeds=20, precision=1, m=2):

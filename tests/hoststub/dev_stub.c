@@ -84,8 +84,8 @@ int jxamd_k_gs_chunk(int vt, int xt, int n, int nchunks, const int *wstep, const
                      const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
                      void *un, void *stream)
 { (void)vt; (void)xt; (void)n; (void)nchunks; (void)wstep; (void)sptr; (void)col; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
-int jxamd_k_dense_gemv(int xt, int n, const void *M, const void *x, void *y, void *stream)
-{ (void)xt; (void)n; (void)M; (void)x; (void)y; (void)stream; return 0; }
+int jxamd_k_dense_gemv(int xt, int n, int ld, const void *M, const void *x, void *y, void *stream)
+{ (void)xt; (void)n; (void)ld; (void)M; (void)x; (void)y; (void)stream; return 0; }
 int jxamd_k_gather_cast(int n, const double *x, int stride, int off, float *y, void *stream)
 { (void)n; (void)x; (void)stride; (void)off; (void)y; (void)stream; return 0; }
 int jxamd_k_widen(int n, const float *x, double *y, void *stream) { (void)n; (void)x; (void)y; (void)stream; return 0; }
