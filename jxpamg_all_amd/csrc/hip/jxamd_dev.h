@@ -75,10 +75,11 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
                        unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
 /* one stage-2 sweep as on nparts ranks (partitions of at most 8 rows);
  * blocks vt = F64 (dinv F64), F32 (dinv F32) or F16 scaled by rsc per block
- * row (dinv F32) */
-int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const void *xp,
-                      int xp_f32, int pidx, double *xn, double w, double omega, void *stream);
+ * row (dinv F32), stored two slots per 32-bit word from pair pptr[s] of each
+ * slice */
+int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *pptr, const int *col,
+                      const void *val, const float *rsc, const void *dinv, const double *rhs, const double *xo,
+                      const void *xp, int xp_f32, int pidx, double *xn, double w, double omega, void *stream);
 /* BILU(0) block triangular solve (vt: F64 or F32 factor storage) */
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
                  const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,

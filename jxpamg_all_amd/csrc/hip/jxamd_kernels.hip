@@ -78,6 +78,8 @@ __device__ __forceinline__ double cvt(double v) { return v; }
 __device__ __forceinline__ float cvt(float v) { return v; }
 __device__ __forceinline__ float cvt(__half v) { return __half2float(v); }
 template <typename V> struct row_scaled { static constexpr bool value = false; };
+template <typename V> struct constexpr_is_half { static constexpr bool value = false; };
+template <> struct constexpr_is_half<__half> { static constexpr bool value = true; };
 // non-temporal load for data streamed once per launch (the fine BSR matrix;
 // the AMG level matrices are re-read several times per V-cycle and gain from
 // staying cached, so their kernels use ordinary loads)
@@ -588,6 +590,7 @@ k_bhgs_multi(int nsl, int nvs, const int *__restrict__ vorder, const int *__rest
 // is read); arithmetic in double.  PU slots per batch.
 template <int B, int LMAX, typename V, typename DV, int PU, typename XP>
 __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, const int *__restrict__ sptr,
+                                                   const int *__restrict__ pptr,
                                                    const int *__restrict__ col, const V *__restrict__ val,
                                                    const float *__restrict__ rsc, const DV *__restrict__ dinv,
                                                    const double *__restrict__ rhs,
@@ -628,12 +631,30 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
         }
         const int s = i / WAVE, lane = i % WAVE;
         const double sc = row_scaled<V>::value ? (double)rsc[i] : 1.0;
-        const int k1 = sptr[s + 1];
-        for (int k = sptr[s]; k < k1; k += PU) {
+        const int ks = sptr[s], k1 = sptr[s + 1];
+        for (int k = ks; k < k1; k += PU) {
             // a batch of PU slots: indices, then the neighbours' x, then the
             // blocks (their loads do not wait on the gathers)
             int jj[PU];
             double xj[PU][B];
+            // FP16 blocks: two slots' values per 32-bit word (slot pairs
+            // from the slice's first slot, pair base pptr[s]), loaded whole
+            float hv[constexpr_is_half<V>::value ? PU : 1][B * B];
+            if constexpr (constexpr_is_half<V>::value) {
+                const __half2 *v2 = reinterpret_cast<const __half2 *>(val);
+#pragma unroll
+                for (int u = 0; u < PU; u += 2) {
+                    if (k + u < k1) {
+                        const __half2 *pv = v2 + ((size_t)(pptr[s] + (k - ks + u) / 2) * B * B) * WAVE + lane;
+#pragma unroll
+                        for (int e = 0; e < B * B; e++) {
+                            const __half2 h = pv[e * WAVE];
+                            hv[u][e] = __low2float(h);
+                            hv[u + 1][e] = __high2float(h);
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int u = 0; u < PU; u++) jj[u] = k + u < k1 ? col[(size_t)(k + u) * WAVE + lane] : i;
 #pragma unroll
@@ -648,17 +669,21 @@ __global__ void __launch_bounds__(256) k_bhgs_part(int nb, int nparts, int cpb, 
                 const int j = jj[u];
                 if (j == i) continue; // the diagonal block, padding, or past the slice
                 const V *vv = val + (size_t)(k + u) * B * B * WAVE + lane;
+                auto blk = [&](int e) -> double {
+                    if constexpr (constexpr_is_half<V>::value) return (double)hv[u][e];
+                    else return (double)cvt(ldnt(vv + e * WAVE));
+                };
                 const bool in = j >= ns && j < ns + len;
                 if (in && j < i) {
                     LT *L = Lb + (size_t)(c - c0) * LST + ((ti * (ti - 1)) / 2 + (j - ns)) * B * B;
 #pragma unroll
-                    for (int e = 0; e < B * B; e++) L[e] = (LT)(sc * (double)cvt(ldnt(vv + e * WAVE)));
+                    for (int e = 0; e < B * B; e++) L[e] = (LT)(sc * blk(e));
                 }
 #pragma unroll
                 for (int rr = 0; rr < B; rr++) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int q = 0; q < B; q++) acc += (double)cvt(ldnt(vv + (rr * B + q) * WAVE)) * xj[u][q];
+                    for (int q = 0; q < B; q++) acc += blk(rr * B + q) * xj[u][q];
                     acc *= sc;
                     if (!in) res[rr] -= acc;
                     else {
@@ -1688,14 +1713,14 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
     return chk(hipGetLastError());
 }
 
-int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const void *xp,
-                      int xp_f32, int pidx, double *xn, double w, double omega, void *stream)
+int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *pptr, const int *col,
+                      const void *val, const float *rsc, const void *dinv, const double *rhs, const double *xo,
+                      const void *xp, int xp_f32, int pidx, double *xn, double w, double omega, void *stream)
 {
     if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32 && vt != JXAMD_T_F16) return JXAMD_ERROR_ARG;
     if (nb == 0) return 0;
     const int maxlen = jxamd_gs_chunk_lds_rows(nb, nparts);
-    if (maxlen < 1 || maxlen > 8 || (!xo && !xp) || (vt == JXAMD_T_F16 && !rsc)) return JXAMD_ERROR_ARG;
+    if (maxlen < 1 || maxlen > 8 || (!xo && !xp) || (vt == JXAMD_T_F16 && (!rsc || !pptr))) return JXAMD_ERROR_ARG;
     hipStream_t st = (hipStream_t)stream;
     // four slots per batch (FP64 blocks: 3.90 -> 3.76 ms of stage 2 per
     // 60x220x85 solve against one slot per step)
@@ -1712,15 +1737,18 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
         const size_t lds = sizeof(double) * 2 * 256 * BS + (vt == JXAMD_T_F64 ? sizeof(double) : sizeof(float)) * lb; \
         if (vt == JXAMD_T_F64)                                                                                        \
             k_bhgs_part<BS, LM, double, double, 4, XT><<<grid, 256, lds, st>>>(                                        \
-                nb, nparts, cpb, sptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo, (const XT *)xp,   \
+                nb, nparts, cpb, sptr, pptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo,             \
+                (const XT *)xp,                                                                                        \
                 pidx, xn, w, omega);                                                                                   \
         else if (vt == JXAMD_T_F32)                                                                                    \
             k_bhgs_part<BS, LM, float, float, 4, XT><<<grid, 256, lds, st>>>(                                          \
-                nb, nparts, cpb, sptr, col, (const float *)val, rsc, (const float *)dinv, rhs, xo, (const XT *)xp,     \
+                nb, nparts, cpb, sptr, pptr, col, (const float *)val, rsc, (const float *)dinv, rhs, xo,               \
+                (const XT *)xp,                                                                                        \
                 pidx, xn, w, omega);                                                                                   \
         else                                                                                                           \
             k_bhgs_part<BS, LM, __half, float, 4, XT><<<grid, 256, lds, st>>>(                                         \
-                nb, nparts, cpb, sptr, col, (const __half *)val, rsc, (const float *)dinv, rhs, xo, (const XT *)xp,    \
+                nb, nparts, cpb, sptr, pptr, col, (const __half *)val, rsc, (const float *)dinv, rhs, xo,              \
+                (const XT *)xp,                                                                                        \
                 pidx, xn, w, omega);                                                                                   \
     } while (0)
 #define PART_B(BS)                                                                                                     \

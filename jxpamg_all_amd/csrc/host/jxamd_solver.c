@@ -57,6 +57,7 @@ typedef struct {
     double *gval, *dinv, *xs;  /* xs: x^(k) of the K sweeps in slice order, then x^(-1) */
     void *val32, *dinv32;      /* FP32 (stage2_precision 1) or FP16 blocks (2; FP32 dinv) for the partitioned stage 2 */
     float *rsc16;              /* stage2_precision 2: power-of-two scale per block row */
+    int *pptr16;               /* stage2_precision 2: first slot pair of each slice (blocks stored two slots per word) */
     double spmv_bytes, sweep_bytes; /* algorithmic bytes: y = A x, one fused one-rank block-HGS sweep */
     int unit_diag;             /* every diagonal block is exactly the identity: the partitioned sweeps skip Dinv */
     double part_bytes, part_bytes32; /* algorithmic bytes of one partitioned sweep (FP64, narrow blocks) */
@@ -1041,10 +1042,19 @@ static int stage2_vt(int p) { return p == 2 ? JXAMD_T_F16 : (p == 1 ? JXAMD_T_F3
  * most 1 in magnitude; the scales go to d->rsc16 */
 static int upload_blocks_f16(dbsr *d, const hsell *h, const double *v, int nb, int bb, void *st)
 {
-    const size_t slots = (size_t)(h->nsl ? h->sptr[h->nsl] : 0);
+    /* slice s's slots k0..k1-1 are stored in pairs (k0 + 2p, k0 + 2p + 1),
+     * element e of both in one 32-bit word (low half the first slot; a
+     * zero high half past the slice's last slot), pairs of slice s from
+     * pptr[s] on: word ((pptr[s] + p) * bb + e) * 64 + lane */
+    const int nsl = h->nsl;
+    int *pp = (int *)malloc(sizeof(int) * ((size_t)nsl + 1));
     float *sc = (float *)malloc(sizeof(float) * (nb > 0 ? nb : 1));
-    uint16_t *t = (uint16_t *)calloc(slots * bb * SELLW + 1, sizeof(uint16_t));
-    if (!sc || !t) { free(sc); free(t); return JXAMD_ERROR_MEMORY; }
+    if (!pp || !sc) { free(pp); free(sc); return JXAMD_ERROR_MEMORY; }
+    pp[0] = 0;
+    for (int sl = 0; sl < nsl; sl++) pp[sl + 1] = pp[sl] + (h->sptr[sl + 1] - h->sptr[sl] + 1) / 2;
+    const size_t pairs = (size_t)pp[nsl];
+    uint32_t *t = (uint32_t *)calloc(pairs * bb * SELLW + 1, sizeof(uint32_t));
+    if (!t) { free(pp); free(sc); return JXAMD_ERROR_MEMORY; }
     for (int i = 0; i < nb; i++) {
         const int sl = i / SELLW, l = i % SELLW;
         double mx = 0.0;
@@ -1064,17 +1074,20 @@ static int upload_blocks_f16(dbsr *d, const hsell *h, const double *v, int nb, i
         sc[i] = (float)scale;
         for (int k = h->sptr[sl]; k < h->sptr[sl + 1]; k++) {
             if (h->col[(size_t)k * SELLW + l] == i) continue;
+            const int kk = k - h->sptr[sl];
             for (int e = 0; e < bb; e++) {
-                const size_t o = ((size_t)k * bb + e) * SELLW + l;
-                t[o] = half_bits((float)(v[o] / scale));
+                const uint32_t hb = half_bits((float)(v[((size_t)k * bb + e) * SELLW + l] / scale));
+                t[(((size_t)pp[sl] + kk / 2) * bb + e) * SELLW + l] |= hb << (16 * (kk & 1));
             }
         }
     }
-    int rc = jxamd_dev_malloc(&d->val32, sizeof(uint16_t) * (slots * bb * SELLW + 1));
-    if (!rc) rc = jxamd_h2d(d->val32, t, sizeof(uint16_t) * slots * bb * SELLW, st);
+    int rc = jxamd_dev_malloc(&d->val32, sizeof(uint32_t) * (pairs * bb * SELLW + 1));
+    if (!rc) rc = jxamd_h2d(d->val32, t, sizeof(uint32_t) * pairs * bb * SELLW, st);
     if (!rc) rc = jxamd_dev_malloc((void **)&d->rsc16, sizeof(float) * (nb > 0 ? nb : 1));
     if (!rc) rc = jxamd_h2d(d->rsc16, sc, sizeof(float) * nb, st);
+    if (!rc) rc = upload_ints(&d->pptr16, pp, (size_t)nsl + 1, st);
     if (!rc) rc = jxamd_stream_sync(st);
+    free(pp);
     free(sc);
     free(t);
     return rc;
@@ -1737,7 +1750,7 @@ int jxamd_solver_destroy(jxamd_solver *s)
     jxamd_dev_free(d->gval); jxamd_dev_free(d->dinv); jxamd_dev_free(d->flag);
     jxamd_dev_free(d->gdep_ptr); jxamd_dev_free(d->gdep_list);
     jxamd_dev_free(d->gall_ptr); jxamd_dev_free(d->gall_list); jxamd_dev_free(d->gvorder); jxamd_dev_free(d->xs);
-    jxamd_dev_free(d->val32); jxamd_dev_free(d->dinv32); jxamd_dev_free(d->rsc16);
+    jxamd_dev_free(d->val32); jxamd_dev_free(d->dinv32); jxamd_dev_free(d->rsc16); jxamd_dev_free(d->pptr16);
     for (int l = 0; l < JXAMD_MAX_LEVELS; l++) {
         dsell_free(&s->Ares[l]); dsell_free(&s->Gf[l]); dsell_free(&s->Gb[l]); dsell_free(&s->Gc[l]);
         jxamd_dev_free(s->wstep[l]);
@@ -2047,7 +2060,7 @@ static int cpr_apply(jxamd_solver *s, const double *r, double *x)
         for (int it = 0; it < m; it++) {
             int pq = prof_begin(s);
             const int nar = s->prm.stage2_precision; /* 0 FP64, 1 FP32, 2 FP16 blocks */
-            CK(jxamd_k_bhgs_part(b, stage2_vt(nar), s->nb, parts, d->sptr, d->col,
+            CK(jxamd_k_bhgs_part(b, stage2_vt(nar), s->nb, parts, d->sptr, d->pptr16, d->col,
                                  nar ? d->val32 : (const void *)d->val, d->rsc16,
                                  d->unit_diag ? NULL : (nar ? d->dinv32 : (const void *)d->dinv), r, src,
                                  xp_f32 ? s->U[0] : (const void *)xp, xp_f32, pi, dst, s->prm.stage2_weight,
@@ -2394,7 +2407,7 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
     if (s->prm.stage2_chunk > 0) {
         const int parts = (s->nb + s->prm.stage2_chunk - 1) / s->prm.stage2_chunk;
         const int nar = s->prm.stage2_precision;
-        CK(jxamd_k_bhgs_part(s->b, stage2_vt(nar), s->nb, parts, s->A.sptr, s->A.col,
+        CK(jxamd_k_bhgs_part(s->b, stage2_vt(nar), s->nb, parts, s->A.sptr, s->A.pptr16, s->A.col,
                              nar ? s->A.val32 : (const void *)s->A.val, s->A.rsc16,
                              s->A.unit_diag ? NULL : (nar ? s->A.dinv32 : (const void *)s->A.dinv), d_b, d_x, NULL, 0, 0,
                              s->xtmp, w, omega, s->stream));

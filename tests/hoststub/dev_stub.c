@@ -138,7 +138,7 @@ int jxamd_k_gs_chunk_grp(int G, int vt, int xt, int n, int nchunks, const int *i
                          void *stream)
 { (void)G; (void)vt; (void)xt; (void)n; (void)nchunks; (void)ia; (void)ja; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)stream; return 0; }
 
-int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *col, const void *val,
-                      const float *rsc, const void *dinv, const double *rhs, const double *xo, const void *xp,
-                      int xp_f32, int pidx, double *xn, double w, double omega, void *stream)
-{ (void)b; (void)vt; (void)nb; (void)nparts; (void)sptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)xp_f32; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }
+int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *pptr, const int *col,
+                      const void *val, const float *rsc, const void *dinv, const double *rhs, const double *xo,
+                      const void *xp, int xp_f32, int pidx, double *xn, double w, double omega, void *stream)
+{ (void)b; (void)vt; (void)nb; (void)nparts; (void)sptr; (void)pptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)rhs; (void)xo; (void)xp; (void)xp_f32; (void)pidx; (void)xn; (void)w; (void)omega; (void)stream; return 0; }
