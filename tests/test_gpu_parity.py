@@ -405,3 +405,31 @@ def test_exact_abf_4x4_solve_vs_oracle():
             assert relerr(x, xo) < X_TOL
         assert raw < 1e-5  # D^-1 scaling: the raw relative residual is within cond(D) of the decoupled one
         s.close()
+
+
+@pytest.mark.parametrize("name,rows", [("lvl3", 2000), ("layered", 1500)])
+def test_truncated_vcycle_vs_oracle(name, rows):
+    """coarse_rows: the V-cycle ends at the first level with at most `rows`
+    rows (here level 1) and solves it with an explicit inverse; the oracle
+    runs the same V-cycle on the hierarchy cut there, its coarsest level
+    solved by Gaussian elimination (relax 9)."""
+    d, Ad, bd, H, meta, s = _solver(name, coarse_rows=rows)
+    s.close()
+    Ht = jx.Hierarchy(H.A[:2], H.P[:1])
+    assert len(H.A[1][0]) - 1 <= rows < len(H.A[0][0]) - 1
+    p = jx.default_params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]), tol=meta["rtol"],
+                          stage2_maxit=int(meta["stage2_maxit"]), coarse_rows=rows)
+    s = jx.Solver(Ad, H, p)
+    assert s.num_levels == 2
+    o = Oracle(Ad, Ht)
+    f = np.random.default_rng(4).standard_normal(len(H.A[0][0]) - 1)
+    uv = dev(np.zeros_like(f))
+    s.vcycle(dev(f).data_ptr(), uv.data_ptr())
+    assert relerr(host(uv), o.vcycle(f)) < 1e-9
+    x, st, hist = s.solve(bd)
+    xo, sto, histo, rc = o.gmres(bd, o.params(k_dim=int(meta["k_dim"]), max_iter=int(meta["max_iter"]),
+                                             tol=meta["rtol"], stage2_maxit=int(meta["stage2_maxit"])))
+    print(f"[gpu] truncated {name} at {rows} rows: iters {st.iterations} oracle {sto.iterations}", flush=True)
+    assert st.converged == 1 and abs(st.iterations - sto.iterations) <= 2
+    assert np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd) <= meta["rtol"]
+    s.close()
