@@ -62,6 +62,31 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+class _Roctx:
+    """ROCTx ranges around the legs (rocprofv3 --marker-trace records them, so
+    a kernel trace can be split by leg); no-ops where the library is absent."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+
+    def push(self, name):
+        if self.lib is not None:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib is not None:
+            self.lib.roctxRangePop()
+
+
 def read_traffic(key):
     """HBM bytes per launch (kernel class or kernel) from the committed rocprof
     counter summary, if any."""
@@ -248,7 +273,9 @@ def main():
         x_d.zero_()
         st = s.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
         log(f"warmup {w}: iters {st.iterations} true_rel {st.true_rel_res:.3e} device {st.t_solve_ms:.1f} ms")
+    rtx = _Roctx()
     times = []
+    rtx.push("plain_fp64")
     for k in range(args.steps):
         x_d.zero_()
         torch.cuda.synchronize()
@@ -257,12 +284,15 @@ def main():
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         log(f"step {k}: {times[-1] * 1e3:.1f} ms iters {st.iterations} true_rel {st.true_rel_res:.3e}")
+    rtx.pop()
     sec = float(np.mean(times))
 
-    def timed_solves(solver, n, name=""):
+    def timed_solves(solver, n, name="", rng=None):
         ts, stx = [], None
         x_d.zero_()
         solver.solve_device(b_d.data_ptr(), x_d.data_ptr(), hist)
+        if rng:
+            rtx.push(rng)
         for k in range(n):
             x_d.zero_()
             torch.cuda.synchronize()
@@ -272,6 +302,8 @@ def main():
             ts.append(time.perf_counter() - t1)
             if name:
                 log(f"{name} solve {k}: {ts[-1] * 1e3:.1f} ms")
+        if rng:
+            rtx.pop()
         return float(np.mean(ts)), stx
 
     mixed = None
@@ -305,7 +337,7 @@ def main():
         if net is not None:
             runs.append(("mixed_learned", mixed_kw))
         for name, kw in runs:
-            pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, **kw)
+            pm = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, arnoldi_kernel=args.arnoldi_kernel, **kw)
             sm = jx.Solver(Ad, H, pm)
             used = None
             if name in ("mixed_default_poly", "mixed_learned"):
@@ -315,7 +347,7 @@ def main():
                 used = {"relax6": kinds.count("R"), "default_roots": kinds.count("D"), "learned_roots": kinds.count("L")}
                 log(f"{name}: polynomial on {used} level(s), V-cycle factor {f_j:.4f} (relax 6 everywhere "
                     f"{f_p:.4f}), level smoothers {sm.level_smoothers()[:-1]} (R relax 6, D default roots, L learned)")
-            t_m, st_m = timed_solves(sm, args.steps)
+            t_m, st_m = timed_solves(sm, args.steps, rng=name)
             assert st_m.converged == 1 and st_m.true_rel_res <= 1e-6, f"{name} did not reach rtol 1e-6"
             legs[name] = {"s": round(t_m, 4), "iterations": st_m.iterations, "true_rel_res": st_m.true_rel_res,
                           "speedup_vs_plain_fp64": round(sec / t_m, 3)}
@@ -343,10 +375,11 @@ def main():
         extra = {}
         # the reference's own one-thread semantics: sequential relax-6 sweeps
         # (k_gs_sweep) and the one-rank stage 2 (k_bhgs_multi)
-        pe = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=0, stage2_chunk=0)
+        pe = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=0, stage2_chunk=0,
+                               arnoldi_kernel=args.arnoldi_kernel)
         se = jx.Solver(Ad, H, pe)
         log("reference-exact solver created")
-        t_e, st_e = timed_solves(se, 1, "reference-exact")
+        t_e, st_e = timed_solves(se, 1, "reference-exact", rng="reference_exact")
         pr, lv = profiled(se)
         log("reference-exact profiled solve done")
         se.close()
@@ -358,9 +391,10 @@ def main():
                              "GBs": gbs(pr["bhgs_sweep"])}}
         log(f"reference-exact (one-thread relax 6, one-rank stage 2): {t_e * 1e3:.1f} ms iters {st_e.iterations}")
         # stage 2 as BILU(0) corrections with level-scheduled block triangular solves (k_btri)
-        pb = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_solver_type=5)
+        pb = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_solver_type=5,
+                               arnoldi_kernel=args.arnoldi_kernel)
         sb = jx.Solver(Ad, H, pb)
-        t_b, st_b = timed_solves(sb, args.steps, "BILU(0) stage 2")
+        t_b, st_b = timed_solves(sb, args.steps, "BILU(0) stage 2", rng="bilu0_stage2")
         pr, _ = profiled(sb)
         sb.close()
         extra["bilu0_stage2"] = {"s": round(t_b, 4), "iterations": st_b.iterations, "true_rel_res": st_b.true_rel_res,
@@ -369,7 +403,6 @@ def main():
                                             "frac": round((gbs(pr["bilu_solve"]) or 0) / PEAK_HBM_GBS, 4)}}
         log(f"BILU(0) stage 2: {t_b * 1e3:.1f} ms iters {st_b.iterations} k_btri {extra['bilu0_stage2']['k_btri']}")
         extra["decoupling"] = decouple_legs(A, rhs)
-    iters = st.iterations
     iters = st.iterations
     assert st.converged == 1 and st.true_rel_res <= 1e-6, "solve did not reach rtol 1e-6"
 
@@ -418,7 +451,7 @@ def main():
     # BSR SpMV bandwidth (second half of the metric).  The ABF-decoupled
     # system's identity diagonal blocks are not stored (the kernel starts each
     # row from x_i), so the bytes the kernel moves are the off-diagonal
-    # blocks with their column indices, row pointers, x and y: 568.8 MB here
+    # blocks with their column indices, row pointers, x and y: 564.4 MB here
     # against 649.6 MB in the all-blocks formula.  Timed over the SpMV
     # launches of two profiled solves (HIP events per launch) and back to back.
     nb_, bb_ = Ad.nb, Ad.b * Ad.b
@@ -436,7 +469,9 @@ def main():
     us_in = ms_sp / n_sp * 1e3
     y_d = torch.empty_like(b_d)
     s.bsr_matvec_timed(b_d.data_ptr(), y_d.data_ptr(), 5)
+    rtx.push("spmv_back_to_back")
     ms = s.bsr_matvec_timed(b_d.data_ptr(), y_d.data_ptr(), args.spmv_reps)
+    rtx.pop()
     spmv = {"kernel": "k_bsr_spmv<3, 8, true>", "bytes_read": spmv_read, "bytes_all_blocks_formula": spmv_all,
             "in_solve_launches": int(n_sp), "in_solve_mean_launch_us": round(us_in, 2),
             "achieved_GBs": round(spmv_read / (us_in * 1e-6) / 1e9, 1),
@@ -453,7 +488,7 @@ def main():
         del A4, r4
         H4 = jx.amg_setup(*jx.extract_pressure(A4d))
         s4 = jx.Solver(A4d, H4, jx.default_params(tol=1e-8, max_iter=100, k_dim=30, relax_chunk=chunk,
-                                                  stage2_chunk=s2chunk))
+                                                  stage2_chunk=s2chunk, arnoldi_kernel=args.arnoldi_kernel))
         x4 = torch.ones(A4d.n, dtype=torch.float64, device="cuda")
         y4 = torch.empty_like(x4)
         s4.bsr_matvec_timed(x4.data_ptr(), y4.data_ptr(), 5)
