@@ -63,6 +63,13 @@ int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, 
                      const float *rsc, const void *diag, const void *f, int fs, const void *uo, void *un,
                      const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
                      unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
+/* the same sweep for long rows: slices of 64/G rows of one dependency level
+ * (perm: position -> row or -1), a group of G lanes (4..32) per row over the
+ * plain CSR rows (ia, ja, val; diagonal first, diag and rsc indexed by row) */
+int jxamd_k_gs_sweep_grp(int G, int vt, int xt, int nsl, const int *perm, const int *ia, const int *ja,
+                         const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
+                         void *un, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
+                         unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
 /* stage 2: x^(-1) into slice order, then K fused sweeps (columns of the
  * layout are positions, ~pos for lower neighbours; xs holds K slice-ordered
  * vectors of xstride doubles; the last sweep also writes xout in row order) */

@@ -1126,6 +1126,77 @@ __global__ void __launch_bounds__(256) k_gs_chunk_grp(int n, int nchunks, int cp
     for (int r = threadIdx.x; r < row1 - row0; r += blockDim.x) un[row0 + r] = ux[r];
 }
 
+// Part of one row's Gauss-Seidel sum for a lane of a row group: entries
+// k, k + G, ... < ke of the plain CSR row i, only the dependent couplings
+// (DEP: j < i forward, j > i backward; x = the new values, read coherently)
+// or only the others (x = the entry values, ordinary loads).  U entries per
+// lane are loaded before any is used, then U gathers of x.
+template <int U, int G, typename V, typename X, bool DEP>
+__device__ __forceinline__ X gs_row_part(int k, int ke, int i, int dir, const int *__restrict__ ja,
+                                         const V *__restrict__ val, const X *x)
+{
+    X acc = 0;
+    for (; k < ke; k += U * G) {
+        int j[U];
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kk = k + u * G;
+            j[u] = kk < ke ? ja[kk] : -1;
+            v[u] = kk < ke ? val[kk] : (V)0;
+        }
+        X xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool dep = dir > 0 ? j[u] < i : j[u] > i;
+            const bool use = j[u] >= 0 && j[u] != i && dep == DEP;
+            if (DEP) xv[u] = use ? ld_coherent(x + j[u]) : (X)0;
+            else xv[u] = use ? x[j[u]] : (X)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += (X)cvt(v[u]) * xv[u];
+    }
+    return acc;
+}
+
+// One Gauss-Seidel sweep as k_gs_sweep for levels with long rows: a slice
+// holds R = 64/G rows of one dependency level (perm[p] = row, -1 for
+// padding), each row summed by a group of G lanes over its plain CSR row
+// (ia, ja, val; the diagonal first, kept apart in diag and rsc, both indexed
+// by row).  The couplings on entry values are summed before the wait for the
+// dependency slices, the dependent ones after it; one lane per group then
+// writes the new value.  A row with a zero diagonal keeps its old value.
+template <int G, typename V, typename X>
+__global__ void __launch_bounds__(256) k_gs_sweep_grp(int nsl, const int *__restrict__ perm,
+                                                      const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const V *__restrict__ val, const float *__restrict__ rsc,
+                                                      const X *__restrict__ diag, const X *__restrict__ f, int fs,
+                                                      const X *uo, X *un, const int *__restrict__ dep_ptr,
+                                                      const int *__restrict__ dep_list, int *sflag, int epoch,
+                                                      int dir, unsigned long long *ticket, unsigned long long base,
+                                                      int *err)
+{
+    constexpr int R = WAVE / G;
+    const int lane = threadIdx.x % WAVE, gl = lane % G;
+    for (;;) {
+        const int s = claim_slice(ticket, base, lane);
+        if (s < 0 || s >= nsl) break;
+        const int i = perm[s * R + lane / G];
+        const int k0 = i >= 0 ? ia[i] + 1 + gl : 0, k1 = i >= 0 ? ia[i + 1] : 0;
+        X acc = uo ? gs_row_part<GRP_U, G, V, X, false>(k0, k1, i, dir, ja, val, uo) : (X)0;
+        wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
+        acc += gs_row_part<GRP_U, G, V, X, true>(k0, k1, i, dir, ja, val, un);
+        acc = group_sum<G>(acc);
+        if (i >= 0 && gl == 0) {
+            const X d = diag[i];
+            const X sc = row_scaled<V>::value ? (X)rsc[i] : (X)1;
+            const X res = f[(size_t)i * fs] - sc * acc;
+            st_coherent(un + i, d != (X)0 ? res / d : (uo ? uo[i] : (X)0));
+        }
+        st_flag_release(sflag + s, epoch);
+    }
+}
+
 // dense y = M x (row-major n x n), one wavefront per row
 template <typename X>
 __global__ void __launch_bounds__(256) k_dense_gemv(int n, int ld, const X *__restrict__ M, const X *__restrict__ x,
@@ -1910,6 +1981,25 @@ int jxamd_k_poly_step_grp(int G, int vt, int xt, int n, const int *sptr, const i
                                (k_poly_step_grp<G_, V_, X_, P_><<<grid, 256, 0, (hipStream_t)stream>>>(
                                    n, sptr, col, (const V_ *)val, rsc, (const X_ *)dinv, (const X_ *)f, fs,
                                    (const X_ *)uo, (X_ *)un, (X_)omega))));
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_gs_sweep_grp(int G, int vt, int xt, int nsl, const int *perm, const int *ia, const int *ja,
+                         const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
+                         void *un, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
+                         unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+{
+    if (nsl == 0) return 0;
+    if (G < 4 || G > 32) return JXAMD_ERROR_ARG;
+    const int waves = jxamd_sweep_waves(nsl);
+    const int grid = (waves + 3) / 4;
+    hipStream_t st = (hipStream_t)stream;
+    DISPATCH_G(G, DISPATCH_VX(vt, xt,
+                              (k_gs_sweep_grp<G_, V_, X_><<<grid, 256, 0, st>>>(
+                                  nsl, perm, ia, ja, (const V_ *)val, rsc, (const X_ *)diag, (const X_ *)f, fs,
+                                  (const X_ *)uo, (X_ *)un, dep_ptr, dep_list, sflag, epoch, dir, ticket, *base,
+                                  err))));
+    *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
     return chk(hipGetLastError());
 }
 

@@ -208,10 +208,11 @@ void jxamd_params_default(jxamd_params *p)
 
 /* Sweep schedule of one GS direction: forward (dir>0) depends on the strictly
  * lower part, backward on the strictly upper part.  Rows are sorted by
- * dependency level and every level is padded to a multiple of 64 positions
- * (perm = -1), so no 64-wide slice holds a row together with one of its
- * dependencies.  dep_ptr/dep_list: per slice, the distinct earlier slices
- * that hold its rows' dependencies. */
+ * dependency level and every level is padded to a multiple of W positions
+ * (perm = -1; W = 64 rows per slice, or 64/G for row groups of G lanes), so
+ * no slice holds a row together with one of its dependencies.
+ * dep_ptr/dep_list: per slice, the distinct earlier slices that hold its
+ * rows' dependencies. */
 typedef struct {
     int npos, nsl, nlevels;
     int *perm, *dep_ptr, *dep_list, *slev; /* slev: level of each slice */
@@ -223,7 +224,7 @@ static void hsched_free(hsched *h)
     memset(h, 0, sizeof(*h));
 }
 
-static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *o)
+static int build_schedule_w(int n, const int *ia, const int *ja, int dir, int W, hsched *o)
 {
     memset(o, 0, sizeof(*o));
     int *lev = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
@@ -244,23 +245,23 @@ static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *
     int *cnt = (int *)calloc((size_t)nl + 1, sizeof(int));
     if (!start || !cnt) { free(lev); free(start); free(cnt); return JXAMD_ERROR_MEMORY; }
     for (int i = 0; i < n; i++) cnt[lev[i]]++;
-    for (int l = 0; l < nl; l++) start[l + 1] = start[l] + ((cnt[l] + SELLW - 1) / SELLW) * SELLW;
+    for (int l = 0; l < nl; l++) start[l + 1] = start[l] + ((cnt[l] + W - 1) / W) * W;
     if (start[nl] > 0x7fffffffL) { free(lev); free(start); free(cnt); return JXAMD_ERROR_ARG; }
     o->npos = (int)start[nl];
-    o->nsl = o->npos / SELLW;
+    o->nsl = o->npos / W;
     o->nlevels = nl;
     o->perm = (int *)malloc(sizeof(int) * (o->npos > 0 ? o->npos : 1));
     o->slev = (int *)malloc(sizeof(int) * (o->nsl > 0 ? o->nsl : 1));
     int *sl = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
     if (!o->perm || !o->slev || !sl) { free(lev); free(start); free(cnt); free(sl); return JXAMD_ERROR_MEMORY; }
     for (int l = 0; l < nl; l++)
-        for (long q = start[l] / SELLW; q < start[l + 1] / SELLW; q++) o->slev[q] = l;
+        for (long q = start[l] / W; q < start[l + 1] / W; q++) o->slev[q] = l;
     for (int p = 0; p < o->npos; p++) o->perm[p] = -1;
     for (int l = 0; l < nl; l++) cnt[l] = 0;
     for (int i = 0; i < n; i++) { /* rows in ascending order inside a level */
         long p = start[lev[i]] + cnt[lev[i]]++;
         o->perm[p] = i;
-        sl[i] = (int)(p / SELLW);
+        sl[i] = (int)(p / W);
     }
     /* dependency slices */
     o->dep_ptr = (int *)malloc(sizeof(int) * ((size_t)o->nsl + 1));
@@ -275,8 +276,8 @@ static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *
     int bad = 0;
     for (int q = 0; q < o->nsl; q++) {
         o->dep_ptr[q] = (int)used;
-        for (int l = 0; l < SELLW; l++) {
-            int i = o->perm[q * SELLW + l];
+        for (int l = 0; l < W; l++) {
+            int i = o->perm[(size_t)q * W + l];
             if (i < 0) continue;
             for (int k = ia[i]; k < ia[i + 1]; k++) {
                 int j = ja[k];
@@ -299,6 +300,11 @@ static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *
     free(lev); free(start); free(cnt); free(sl); free(mark);
     if (bad) { hsched_free(o); return bad == 2 ? JXAMD_ERROR_MEMORY : JXAMD_ERROR_GENERIC; }
     return 0;
+}
+
+static int build_schedule(int n, const int *ia, const int *ja, int dir, hsched *o)
+{
+    return build_schedule_w(n, ia, ja, dir, SELLW, o);
 }
 
 /* Per slice of a schedule (rows by position in perm): the distinct slices
@@ -1599,6 +1605,26 @@ static int build_level(jxamd_solver *s, int l, const jxamd_hierarchy *H)
         s->nchunks[l] = T;
         /* forward + backward pass over the matrix, old + new + rhs vectors */
         s->Gc[l].abytes = 2.0 * level_bytes(&s->Gc[l], A->ia[n], n, 0.0) + 3.0 * type_size(s->xt) * n;
+    } else if ((l < s->L - 1 || s->L == 1) && GA > 1) {
+        /* long rows: sweep orders with slices of 64/GA rows of one dependency
+         * level, each row summed by a group of GA lanes over its plain CSR
+         * row (k_gs_sweep_grp) */
+        for (int dir = 1; dir >= -1; dir -= 2) {
+            dsell *d = dir > 0 ? &s->Gf[l] : &s->Gb[l];
+            hsched sc;
+            CK(build_schedule_w(n, A->ia, A->ja, dir, SELLW / GA, &sc));
+            e = dcsr_upload(d, n, n, A->ia, A->ja, A->a, s->vt, s->xt, GA, 1, s->stream);
+            if (!e) e = upload_ints(&d->perm, sc.perm, (size_t)sc.npos, s->stream);
+            if (!e) e = upload_ints(&d->dep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
+            if (!e) e = upload_ints(&d->dep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
+            if (!e) e = zero_ints(&d->flag, (size_t)sc.nsl, s->stream);
+            if (!e) e = jxamd_stream_sync(s->stream);
+            d->nsl = sc.nsl;
+            hsched_free(&sc);
+            CK(e);
+            /* sweep: matrix once + old x + new x + rhs */
+            d->abytes = level_bytes(d, A->ia[n], n, 3.0);
+        }
     } else if (l < s->L - 1 || s->L == 1) {
         /* forward and backward sweep orders, off-diagonal entries */
         for (int dir = 1; dir >= -1; dir -= 2) {
@@ -1815,18 +1841,22 @@ static int reset_sync_state(jxamd_solver *s)
  * uo_zero: the incoming u is known to be zero (first visit of a level). */
 static int hsgs(jxamd_solver *s, int l, const void *f, int fs, void *u, int uo_zero)
 {
-    dsell *g = &s->Gf[l], *h = &s->Gb[l];
-    s->epoch++;
-    int pq = prof_begin(s);
-    CK(jxamd_k_gs_sweep(g->vt, g->xt, g->nsl, g->perm, g->sptr, g->col, g->val, g->rsc, g->diag, f, fs,
-                        uo_zero ? NULL : u, s->T[l], g->dep_ptr, g->dep_list, g->flag, s->epoch, 1, s->ticket,
-                        &s->tbase, s->err, s->stream));
-    prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, l, g->abytes);
-    s->epoch++;
-    pq = prof_begin(s);
-    CK(jxamd_k_gs_sweep(h->vt, h->xt, h->nsl, h->perm, h->sptr, h->col, h->val, h->rsc, h->diag, f, fs, s->T[l], u,
-                        h->dep_ptr, h->dep_list, h->flag, s->epoch, -1, s->ticket, &s->tbase, s->err, s->stream));
-    prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, l, h->abytes);
+    for (int pass = 0; pass < 2; pass++) {
+        dsell *g = pass == 0 ? &s->Gf[l] : &s->Gb[l];
+        const void *uo = pass == 0 ? (uo_zero ? NULL : u) : s->T[l];
+        void *un = pass == 0 ? s->T[l] : u;
+        s->epoch++;
+        int pq = prof_begin(s);
+        if (g->G > 1)
+            CK(jxamd_k_gs_sweep_grp(g->G, g->vt, g->xt, g->nsl, g->perm, g->sptr, g->col, g->val, g->rsc, g->diag, f,
+                                    fs, uo, un, g->dep_ptr, g->dep_list, g->flag, s->epoch, pass == 0 ? 1 : -1,
+                                    s->ticket, &s->tbase, s->err, s->stream));
+        else
+            CK(jxamd_k_gs_sweep(g->vt, g->xt, g->nsl, g->perm, g->sptr, g->col, g->val, g->rsc, g->diag, f, fs, uo,
+                                un, g->dep_ptr, g->dep_list, g->flag, s->epoch, pass == 0 ? 1 : -1, s->ticket,
+                                &s->tbase, s->err, s->stream));
+        prof_end_level(s, pq, JXAMD_PROF_GS_SWEEP, l, g->abytes);
+    }
     return 0;
 }
 

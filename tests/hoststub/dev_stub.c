@@ -52,6 +52,11 @@ int jxamd_k_gs_sweep(int vt, int xt, int nsl, const int *perm, const int *sptr, 
                      const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
                      unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
 { (void)vt; (void)xt; (void)nsl; (void)perm; (void)sptr; (void)col; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)dep_ptr; (void)dep_list; (void)sflag; (void)epoch; (void)dir; (void)ticket; (void)base; (void)err; (void)stream; return 0; }
+int jxamd_k_gs_sweep_grp(int G, int vt, int xt, int nsl, const int *perm, const int *ia, const int *ja,
+                         const void *val, const float *rsc, const void *diag, const void *f, int fs, const void *uo,
+                         void *un, const int *dep_ptr, const int *dep_list, int *sflag, int epoch, int dir,
+                         unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+{ (void)G; (void)vt; (void)xt; (void)nsl; (void)perm; (void)ia; (void)ja; (void)val; (void)rsc; (void)diag; (void)f; (void)fs; (void)uo; (void)un; (void)dep_ptr; (void)dep_list; (void)sflag; (void)epoch; (void)dir; (void)ticket; (void)base; (void)err; (void)stream; return 0; }
 int jxamd_k_bhgs_gather(int npos, int b, const int *perm, const double *x0, const double *xp, int pidx, double *out,
                         void *stream)
 { (void)npos; (void)b; (void)perm; (void)x0; (void)xp; (void)pidx; (void)out; (void)stream; return 0; }

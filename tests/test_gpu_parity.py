@@ -433,3 +433,24 @@ def test_truncated_vcycle_vs_oracle(name, rows):
     assert st.converged == 1 and abs(st.iterations - sto.iterations) <= 2
     assert np.linalg.norm(bd - Ad.to_scipy() @ x) / np.linalg.norm(bd) <= meta["rtol"]
     s.close()
+
+
+@pytest.mark.parametrize("name", ["lvl3", "layered"])
+def test_exact_sweep_every_level_vs_oracle(name):
+    """The exact one-thread relax 6 (forward then backward Gauss-Seidel,
+    par_relax_6.c:224-273) on every smoothed level, from a nonzero guess:
+    level 0 (short rows) runs one lane per row, level 1 (19-21 entries per
+    row) row groups of 4 lanes over the plain CSR rows; both against the
+    oracle's sequential sweep."""
+    d, Ad, bd, H, meta, s = _solver(name)
+    o = Oracle(Ad, H)
+    rng = np.random.default_rng(11)
+    for l in range(H.num_levels - 1):
+        n = len(H.A[l][0]) - 1
+        f, u0 = rng.standard_normal(n), rng.standard_normal(n)
+        u = dev(u0)
+        s.hsgs(l, dev(f).data_ptr(), u.data_ptr())
+        err = relerr(host(u), o.relax6(l, f, u0))
+        print(f"[gpu] exact relax 6 {name} level {l} ({n} rows): relerr {err:.2e}", flush=True)
+        assert err < OP_TOL
+    s.close()
