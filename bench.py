@@ -402,6 +402,19 @@ def main():
                                             "GBs": gbs(pr["bilu_solve"]),
                                             "frac": round((gbs(pr["bilu_solve"]) or 0) / PEAK_HBM_GBS, 4)}}
         log(f"BILU(0) stage 2: {t_b * 1e3:.1f} ms iters {st_b.iterations} k_btri {extra['bilu0_stage2']['k_btri']}")
+        # the same with FP16 factors (a power of two per block row) and FP32 inverted diagonals
+        pb16 = jx.default_params(tol=1e-6, max_iter=100, k_dim=30, relax_chunk=chunk, stage2_solver_type=5,
+                                 stage2_precision=2, arnoldi_kernel=args.arnoldi_kernel)
+        sb16 = jx.Solver(Ad, H, pb16)
+        t_b16, st_b16 = timed_solves(sb16, args.steps, "BILU(0) FP16 factors", rng="bilu0_stage2_f16")
+        pr, _ = profiled(sb16)
+        sb16.close()
+        extra["bilu0_stage2_f16_factors"] = {
+            "s": round(t_b16, 4), "iterations": st_b16.iterations, "true_rel_res": st_b16.true_rel_res,
+            "k_btri": {"launches": pr["bilu_solve"][0], "ms": round(pr["bilu_solve"][1], 3),
+                       "GBs": gbs(pr["bilu_solve"]), "frac": round((gbs(pr["bilu_solve"]) or 0) / PEAK_HBM_GBS, 4)}}
+        log(f"BILU(0) FP16 factors: {t_b16 * 1e3:.1f} ms iters {st_b16.iterations} "
+            f"k_btri {extra['bilu0_stage2_f16_factors']['k_btri']}")
         extra["decoupling"] = decouple_legs(A, rhs)
     iters = st.iterations
     assert st.converged == 1 and st.true_rel_res <= 1e-6, "solve did not reach rtol 1e-6"

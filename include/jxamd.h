@@ -201,10 +201,13 @@ typedef struct {
      * (jxfpamg-0220/csrc/amg/par_relax_bsr.c:296-330, 523-557). */
     int stage2_chunk;
     /* Storage of the partitioned stage-2 sweeps' matrix blocks and inverted
-     * diagonal blocks: 0 FP64, 1 FP32, 2 FP16 blocks scaled by a power of
-     * two per block row with FP32 inverted diagonals (FP64 vectors and
-     * arithmetic throughout); used with stage2_chunk > 0.  No reference
-     * counterpart (its mixed driver keeps stage 2 in FP64). */
+     * diagonal blocks (stage2_solver_type 2 with stage2_chunk > 0), and of
+     * the BILU(0) factors (stage2_solver_type 5): 0 FP64 (BILU factors FP32
+     * when precond_precision > 0), 1 FP32, 2 FP16 blocks scaled by a power
+     * of two per block row (per factor triangle for BILU) with FP32 inverted
+     * diagonals; FP64 vectors and arithmetic throughout.  No reference
+     * counterpart (its FP64 BILU keeps its factors in FP64,
+     * src/bilu_double_helper.c:18-25; its mixed driver keeps stage 2 in FP64). */
     int stage2_precision;
     /* Arnoldi orthogonalisation kernels: 0 one cooperative launch per MGS
      * step with the new vector held in registers (when the device supports

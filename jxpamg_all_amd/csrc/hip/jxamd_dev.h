@@ -87,10 +87,12 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
 int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const int *pptr, const int *col,
                       const void *val, const float *rsc, const void *dinv, const double *rhs, const double *xo,
                       const void *xp, int xp_f32, int pidx, double *xn, double w, double omega, void *stream);
-/* BILU(0) block triangular solve (vt: F64 or F32 factor storage) */
+/* BILU(0) block triangular solve (vt: F64 or F32 factor storage, or F16
+ * scaled by rsc[position], a power of two per block row, with F32 dinv) */
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
-                 const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
-                 int epoch, unsigned long long *ticket, unsigned long long *base, int *err, void *stream);
+                 const float *rsc, const void *dinv, const double *rhs, double *out, const int *dep_ptr,
+                 const int *dep_list, int *sflag, int epoch, unsigned long long *ticket, unsigned long long *base,
+                 int *err, void *stream);
 /* hybrid SSOR with short chunks (at most JXAMD_CHUNK_LDS_MAX rows): SELL-64
  * of the off-diagonals in natural row order, diagonal in diag[i] */
 #define JXAMD_CHUNK_LDS_MAX 16

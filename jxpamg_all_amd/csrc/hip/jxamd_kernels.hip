@@ -772,11 +772,19 @@ __global__ void k_bhgs_gather(int npos, int B, const int *__restrict__ perm, con
 //   out_i = Dinv_i (rhs_i - sum_j M_ij out_j)   (Dinv_i = I when dinv == nullptr)
 // where the slice layout holds only the blocks M_ij of one triangle (the
 // dependencies: lower blocks for the forward L solve, upper blocks for the
-// backward U solve), stored as V (double or float), vectors double.
-template <int B, typename V>
+// backward U solve), stored as V (double, float, or __half scaled by the
+// power of two rsc[p] per block row: M_ij = rsc[p] * stored; FP16 elements
+// stored in pairs, slot k's word w at (k * ceil(B*B/2) + w) * 64 + lane), Dinv as DV,
+// vectors and arithmetic double.  A lane loads the column indices of BT_U
+// blocks, then their values and the dependency values, before it uses any,
+// so one level step costs two dependent round trips rather than two per
+// block.  The subtractions keep the order of the blocks in the row.
+#define BT_U 4
+template <int B, typename V, typename DV>
 __global__ void __launch_bounds__(256) k_btri(int nsl, const int *__restrict__ perm, const int *__restrict__ sptr,
                                               const int *__restrict__ col, const V *__restrict__ val,
-                                              const V *__restrict__ dinv, const double *__restrict__ rhs, double *out,
+                                              const float *__restrict__ rsc, const DV *__restrict__ dinv,
+                                              const double *__restrict__ rhs, double *out,
                                               const int *__restrict__ dep_ptr, const int *__restrict__ dep_list,
                                               int *sflag, int epoch, unsigned long long *ticket,
                                               unsigned long long base, int *err)
@@ -788,7 +796,7 @@ __global__ void __launch_bounds__(256) k_btri(int nsl, const int *__restrict__ p
         const int p = s * WAVE + lane;
         const int i = perm[p];
         const int k0 = sptr[s], k1 = sptr[s + 1];
-        double acc[B], D[B * B];
+        double acc[B], D[B * B], sc = 1.0;
         if (i >= 0) {
 #pragma unroll
             for (int r = 0; r < B; r++) acc[r] = rhs[(size_t)i * B + r];
@@ -796,20 +804,47 @@ __global__ void __launch_bounds__(256) k_btri(int nsl, const int *__restrict__ p
 #pragma unroll
                 for (int q = 0; q < B * B; q++) D[q] = (double)dinv[(size_t)i * B * B + q];
             }
+            if (rsc) sc = (double)rsc[p];
         }
         wait_slices(dep_ptr, dep_list, sflag, s, epoch, lane, err);
         if (i >= 0) {
-            for (int k = k0; k < k1; k++) {
-                const int j = col[(size_t)k * WAVE + lane];
-                if (j == i) continue; // padding slot
-                double xc[B];
+            for (int k = k0; k < k1; k += BT_U) {
+                int j[BT_U];
 #pragma unroll
-                for (int q = 0; q < B; q++) xc[q] = ld_coherent(out + (size_t)j * B + q);
-                const V *vv = val + (size_t)k * B * B * WAVE + lane;
+                for (int u = 0; u < BT_U; u++) j[u] = k + u < k1 ? col[(size_t)(k + u) * WAVE + lane] : i;
+                double xc[BT_U][B];
+                double m[BT_U][B * B];
 #pragma unroll
-                for (int r = 0; r < B; r++) {
+                for (int u = 0; u < BT_U; u++) {
+                    // the row itself marks a padding slot (and the batch's tail)
 #pragma unroll
-                    for (int q = 0; q < B; q++) acc[r] -= (double)vv[(r * B + q) * WAVE] * xc[q];
+                    for (int q = 0; q < B; q++) xc[u][q] = j[u] != i ? ld_coherent(out + (size_t)j[u] * B + q) : 0.0;
+                    if constexpr (constexpr_is_half<V>::value) {
+                        // FP16: elements 2w and 2w + 1 of a block in one
+                        // 32-bit word (a zero high half past the last), so
+                        // the loads are whole words
+                        constexpr int NW = (B * B + 1) / 2;
+                        const __half2 *vw = reinterpret_cast<const __half2 *>(val) + (size_t)(k + u) * NW * WAVE + lane;
+#pragma unroll
+                        for (int w = 0; w < NW; w++) {
+                            const __half2 h = j[u] != i ? vw[w * WAVE] : __half2{};
+                            m[u][2 * w] = (double)__low2float(h);
+                            if (2 * w + 1 < B * B) m[u][2 * w + 1] = (double)__high2float(h);
+                        }
+                    } else {
+                        const V *vv = val + (size_t)(k + u) * B * B * WAVE + lane;
+#pragma unroll
+                        for (int q = 0; q < B * B; q++) m[u][q] = j[u] != i ? (double)cvt(vv[q * WAVE]) : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < BT_U; u++) {
+                    if (j[u] == i) continue;
+#pragma unroll
+                    for (int r = 0; r < B; r++) {
+#pragma unroll
+                        for (int q = 0; q < B; q++) acc[r] -= (sc * m[u][r * B + q]) * xc[u][q];
+                    }
                 }
             }
             if (dinv) {
@@ -1841,35 +1876,35 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
 }
 
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
-                 const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
-                 int epoch, unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
+                 const float *rsc, const void *dinv, const double *rhs, double *out, const int *dep_ptr,
+                 const int *dep_list, int *sflag, int epoch, unsigned long long *ticket, unsigned long long *base,
+                 int *err, void *stream)
 {
     if (nsl == 0) return 0;
-    if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32) return JXAMD_ERROR_ARG;
+    if (vt != JXAMD_T_F64 && vt != JXAMD_T_F32 && vt != JXAMD_T_F16) return JXAMD_ERROR_ARG;
+    if ((vt == JXAMD_T_F16) != (rsc != nullptr)) return JXAMD_ERROR_ARG;
+    if (b < 2 || b > 4) return JXAMD_ERROR_ARG;
     const int waves = jxamd_sweep_waves(nsl);
     const int grid = (waves + 3) / 4;
     hipStream_t st = (hipStream_t)stream;
     const unsigned long long b0 = *base;
-#define LAUNCH_BTRI(BS, VT)                                                                                            \
-    k_btri<BS, VT><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, (const VT *)val, (const VT *)dinv, rhs, out, dep_ptr,  \
-                                         dep_list, sflag, epoch, ticket, b0, err)
-    if (vt == JXAMD_T_F64) {
-        switch (b) {
-        case 2: LAUNCH_BTRI(2, double); break;
-        case 3: LAUNCH_BTRI(3, double); break;
-        case 4: LAUNCH_BTRI(4, double); break;
-        default: return JXAMD_ERROR_ARG;
-        }
-    } else {
-        switch (b) {
-        case 2: LAUNCH_BTRI(2, float); break;
-        case 3: LAUNCH_BTRI(3, float); break;
-        case 4: LAUNCH_BTRI(4, float); break;
-        default: return JXAMD_ERROR_ARG;
-        }
-    }
+#define LAUNCH_BTRI(VT, DT)                                                                                            \
+    do {                                                                                                               \
+        if (b == 2)                                                                                                    \
+            k_btri<2, VT, DT><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, (const VT *)val, rsc, (const DT *)dinv, rhs,  \
+                                                    out, dep_ptr, dep_list, sflag, epoch, ticket, b0, err);            \
+        else if (b == 3)                                                                                               \
+            k_btri<3, VT, DT><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, (const VT *)val, rsc, (const DT *)dinv, rhs,  \
+                                                    out, dep_ptr, dep_list, sflag, epoch, ticket, b0, err);            \
+        else                                                                                                           \
+            k_btri<4, VT, DT><<<grid, 256, 0, st>>>(nsl, perm, sptr, col, (const VT *)val, rsc, (const DT *)dinv, rhs,  \
+                                                    out, dep_ptr, dep_list, sflag, epoch, ticket, b0, err);            \
+    } while (0)
+    if (vt == JXAMD_T_F64) LAUNCH_BTRI(double, double);
+    else if (vt == JXAMD_T_F32) LAUNCH_BTRI(float, float);
+    else LAUNCH_BTRI(__half, float);
 #undef LAUNCH_BTRI
-    *base += (unsigned long long)nsl + (unsigned long long)grid * 4;
+    *base = b0 + (unsigned long long)nsl + (unsigned long long)grid * 4;
     return chk(hipGetLastError());
 }
 

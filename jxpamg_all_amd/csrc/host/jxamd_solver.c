@@ -68,7 +68,8 @@ typedef struct {
 typedef struct {
     int nsl, vt;
     int *perm, *sptr, *col, *dep_ptr, *dep_list, *flag;
-    void *val, *dinv;
+    void *val, *dinv;        /* dinv: FP32 when the blocks are FP16 */
+    float *rsc;              /* FP16 blocks: power-of-two scale per position (block row) */
     double bytes;
 } dtri;
 
@@ -1264,20 +1265,72 @@ static int build_bsr(jxamd_solver *s, const jxamd_bsr *A)
     return 0;
 }
 
+/* FP16 copy of one BILU(0) triangle in its slice layout (v: slot-major, b*b
+ * planes of 64 lanes): position p's blocks are scaled by the smallest power
+ * of two at or above their largest magnitude, which goes to t->rsc[p];
+ * elements 2w and 2w+1 of a block share 32-bit word w (low half first, a zero
+ * high half past the last element), slot k's word w of lane l at
+ * (k * nw + w) * 64 + l, nw = ceil(b*b / 2) */
+static int upload_tri_f16(dtri *t, const hsell *h, const double *v, int bb, void *st)
+{
+    const size_t slots = (size_t)(h->nsl ? h->sptr[h->nsl] : 0);
+    const size_t npos = (size_t)h->nsl * SELLW;
+    const int nw = (bb + 1) / 2;
+    uint16_t *hv = (uint16_t *)calloc(slots * nw * 2 * SELLW + 1, sizeof(uint16_t));
+    float *sc = (float *)malloc(sizeof(float) * (npos ? npos : 1));
+    if (!hv || !sc) { free(hv); free(sc); return JXAMD_ERROR_MEMORY; }
+    for (int q = 0; q < h->nsl; q++)
+        for (int l = 0; l < SELLW; l++) {
+            double mx = 0.0;
+            for (int k = h->sptr[q]; k < h->sptr[q + 1]; k++)
+                for (int e = 0; e < bb; e++) {
+                    const double a = fabs(v[((size_t)k * bb + e) * SELLW + l]);
+                    if (a > mx) mx = a;
+                }
+            int ex = 0;
+            if (mx > 0.0) {
+                const double m = frexp(mx, &ex); /* mx = m * 2^ex, m in [0.5, 1) */
+                if (m == 0.5) ex -= 1;
+            }
+            const double scale = mx > 0.0 ? ldexp(1.0, ex) : 1.0;
+            sc[(size_t)q * SELLW + l] = (float)scale;
+            for (int k = h->sptr[q]; k < h->sptr[q + 1]; k++)
+                for (int e = 0; e < bb; e++) {
+                    const size_t x = ((size_t)k * bb + e) * SELLW + l;
+                    const size_t word = ((size_t)k * nw + e / 2) * SELLW + l;
+                    hv[2 * word + (e & 1)] = half_bits((float)(v[x] / scale));
+                }
+        }
+    int rc = jxamd_dev_malloc(&t->val, sizeof(uint32_t) * (slots * nw * SELLW + 1));
+    if (!rc) rc = jxamd_h2d(t->val, hv, sizeof(uint32_t) * slots * nw * SELLW, st);
+    if (!rc) rc = jxamd_dev_malloc((void **)&t->rsc, sizeof(float) * (npos ? npos : 1));
+    if (!rc) rc = jxamd_h2d(t->rsc, sc, sizeof(float) * npos, st);
+    if (!rc) rc = jxamd_stream_sync(st);
+    free(hv);
+    free(sc);
+    return rc;
+}
+
 static void dtri_free(dtri *t)
 {
     jxamd_dev_free(t->perm); jxamd_dev_free(t->sptr); jxamd_dev_free(t->col); jxamd_dev_free(t->dep_ptr);
     jxamd_dev_free(t->dep_list); jxamd_dev_free(t->flag); jxamd_dev_free(t->val); jxamd_dev_free(t->dinv);
+    jxamd_dev_free(t->rsc);
     memset(t, 0, sizeof(*t));
 }
 
 /* one triangle of the BILU(0) factors: dir > 0 the unit lower L (forward
- * level order), dir < 0 the upper U with its inverted diagonal (backward) */
+ * level order), dir < 0 the upper U with its inverted diagonal (backward).
+ * Storage: stage2_precision 1 FP32, 2 FP16 with each block row scaled by the
+ * smallest power of two at or above the largest magnitude of its blocks in
+ * this triangle (FP32 inverted diagonals), 0 FP64 (FP32 when stage 1 runs
+ * narrow, precond_precision > 0). */
 static int build_tri(jxamd_solver *s, const jxamd_bsr *A, const double *lu, const double *dinv, int dir, dtri *t)
 {
     const int nb = A->nb, b = A->b, bb = b * b;
     memset(t, 0, sizeof(*t));
-    t->vt = s->prm.precond_precision > 0 ? JXAMD_T_F32 : JXAMD_T_F64;
+    const int sp = s->prm.stage2_precision;
+    t->vt = sp == 2 ? JXAMD_T_F16 : (sp == 1 || s->prm.precond_precision > 0) ? JXAMD_T_F32 : JXAMD_T_F64;
     int *fia = (int *)malloc(sizeof(int) * ((size_t)nb + 1));
     int *fja = (int *)malloc(sizeof(int) * (size_t)(A->nnzb > 0 ? A->nnzb : 1));
     int *fidx = (int *)malloc(sizeof(int) * (size_t)(A->nnzb > 0 ? A->nnzb : 1));
@@ -1324,14 +1377,20 @@ static int build_tri(jxamd_solver *s, const jxamd_bsr *A, const double *lu, cons
         e = upload_ints(&t->perm, perm, (size_t)sc.npos, s->stream);
         if (!e) e = upload_ints(&t->sptr, h.sptr, (size_t)h.nsl + 1, s->stream);
         if (!e) e = upload_ints(&t->col, h.col, slots * SELLW, s->stream);
-        if (!e) e = upload_real(&t->val, v, slots * bb * SELLW, t->vt, s->stream);
-        if (!e && dir < 0) e = upload_real(&t->dinv, dinv, (size_t)nb * bb, t->vt, s->stream);
+        if (!e && t->vt == JXAMD_T_F16) e = upload_tri_f16(t, &h, v, bb, s->stream);
+        else if (!e) e = upload_real(&t->val, v, slots * bb * SELLW, t->vt, s->stream);
+        if (!e && dir < 0)
+            e = upload_real(&t->dinv, dinv, (size_t)nb * bb, t->vt == JXAMD_T_F16 ? JXAMD_T_F32 : t->vt, s->stream);
         if (!e) e = upload_ints(&t->dep_ptr, sc.dep_ptr, (size_t)sc.nsl + 1, s->stream);
         if (!e) e = upload_ints(&t->dep_list, sc.dep_list, (size_t)sc.dep_ptr[sc.nsl], s->stream);
         if (!e) e = zero_ints(&t->flag, (size_t)sc.nsl, s->stream);
         if (!e) e = jxamd_stream_sync(s->stream);
-        /* dependency blocks + indices, rhs read, result written, dependency values read */
-        t->bytes = (double)nf * (type_size(t->vt) * bb + 4.0) + 24.0 * nb * b + (dir < 0 ? (double)type_size(t->vt) * nb * bb : 0.0);
+        /* dependency blocks + indices, rhs read, result written, dependency
+         * values read; U: inverted diagonals; FP16: a scale per block row */
+        const double sd = t->vt == JXAMD_T_F16 ? 4.0 : (double)type_size(t->vt);
+        const double sb = t->vt == JXAMD_T_F16 ? 4.0 * ((bb + 1) / 2) : (double)type_size(t->vt) * bb;
+        t->bytes = (double)nf * (sb + 4.0) + 24.0 * nb * b + (dir < 0 ? sd * nb * bb : 0.0) +
+                   (t->vt == JXAMD_T_F16 ? 4.0 * nb : 0.0);
     }
     free(v);
     hsell_free(&h); /* frees perm */
@@ -2026,7 +2085,7 @@ static int tri_run(jxamd_solver *s, const dtri *t, const double *rhs, double *ou
 {
     s->epoch++;
     int pq = prof_begin(s);
-    CK(jxamd_k_btri(s->b, t->vt, t->nsl, t->perm, t->sptr, t->col, t->val, t->dinv, rhs, out, t->dep_ptr,
+    CK(jxamd_k_btri(s->b, t->vt, t->nsl, t->perm, t->sptr, t->col, t->val, t->rsc, t->dinv, rhs, out, t->dep_ptr,
                     t->dep_list, t->flag, s->epoch, s->ticket, &s->tbase, s->err, s->stream));
     prof_end(s, pq, JXAMD_PROF_BILU, t->bytes);
     return 0;

@@ -73,9 +73,10 @@ int jxamd_k_bhgs_multi(int b, int nsl, int K, const int *vorder, const int *perm
     return 0;
 }
 int jxamd_k_btri(int b, int vt, int nsl, const int *perm, const int *sptr, const int *col, const void *val,
-                 const void *dinv, const double *rhs, double *out, const int *dep_ptr, const int *dep_list, int *sflag,
-                 int epoch, unsigned long long *ticket, unsigned long long *base, int *err, void *stream)
-{ (void)b; (void)vt; (void)nsl; (void)perm; (void)sptr; (void)col; (void)val; (void)dinv; (void)rhs; (void)out; (void)dep_ptr; (void)dep_list; (void)sflag; (void)epoch; (void)ticket; (void)base; (void)err; (void)stream; return 0; }
+                 const float *rsc, const void *dinv, const double *rhs, double *out, const int *dep_ptr,
+                 const int *dep_list, int *sflag, int epoch, unsigned long long *ticket, unsigned long long *base,
+                 int *err, void *stream)
+{ (void)b; (void)vt; (void)nsl; (void)perm; (void)sptr; (void)col; (void)val; (void)rsc; (void)dinv; (void)rhs; (void)out; (void)dep_ptr; (void)dep_list; (void)sflag; (void)epoch; (void)ticket; (void)base; (void)err; (void)stream; return 0; }
 int jxamd_gs_chunk_lds_rows(int n, int nchunks)
 {
     if (nchunks < 1 || n < nchunks) return 0;
