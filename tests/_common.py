@@ -45,3 +45,13 @@ def bits(x):
 def relerr(x, y):
     x, y = np.asarray(x), np.asarray(y)
     return float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+
+
+def ranks_system(d, b):
+    """the block-size-b system of ranks.npz (ABF-decoupled for b < 4), its
+    matrix checked against the fixture's SHA-256"""
+    import hashlib
+    A, rhs = decoup_system(d, b)
+    A = jx.decouple_abf(A, rhs)[0] if b < 4 else A
+    assert hashlib.sha256(np.ascontiguousarray(A.a).tobytes()).hexdigest() == bytes(d[f"b{b}_a_sha256"]).decode()
+    return A

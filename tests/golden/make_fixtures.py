@@ -6,7 +6,9 @@ oracle/ref_driver.c, which replays src/test_bsr_cprgmres.c with fixture
 dumps.  Run this only where /root/reference exists; the outputs (inputs,
 expected outputs and the reference's AMG hierarchy) are committed.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py            # solver fixtures
+    python tests/golden/make_fixtures.py decoup     # decoupling types
+    python tests/golden/make_fixtures.py ranks      # stage 2 on P MPI ranks
 """
 import hashlib
 import os
@@ -57,6 +59,19 @@ DECOUP_GRIDS = {
             seed=33),
 }
 DECOUP_RUNS = [(b, t, th) for b in (2, 3, 4) for t in range(1, 7) for th in ((0, 1) if t in (2, 3, 4) else (0,))]
+# multi-rank stage-2 fixture (ranks.npz): one forward block hybrid SOR sweep
+# of the reference on P MPI ranks (oracle/ref_bhgs_mpi under mpirun), 24 block
+# rows, so P ranks own 24 / P rows each as the GPU's stage2_chunk = 24 / P
+# partitions do; b = 2, 3 ABF-decoupled, b = 4 as generated
+RANKS_GRIDS = {
+    2: dict(jx.CONFIGS["C1"], nx=4, ny=3, nz=2, block_size=2, seed=41),
+    3: dict(jx.CONFIGS["C1"], nx=4, ny=3, nz=2, block_size=3, seed=42),
+    4: dict(jx.CONFIGS["C1"], nx=4, ny=3, nz=2, block_size=4, field=jx.FIELD_SMOOTH, sigma_lnk=2.0, corr_len=2.0,
+            seed=43),
+}
+RANKS = (1, 2, 3, 4, 5, 6, 8, 12, 24)
+MPI_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_bhgs_mpi")
+MPIRUN = "/opt/conda/bin/mpirun"
 # OpenMP threads of the reference run (1 unless listed)
 THREADS = {"lvl3_t4": 4}
 # operator vectors not kept for these cases (the hierarchy and the solve are)
@@ -167,8 +182,40 @@ def decoup_fixture():
     print("decoup", len(DECOUP_RUNS) + 2, "runs")
 
 
+def ranks_system(b):
+    """the fixture's system of block size b: generated, ABF-decoupled for b < 4"""
+    A, rhs = jx.generate(**RANKS_GRIDS[b])
+    return jx.decouple_abf(A, rhs) if b < 4 else (A, rhs)
+
+
+def ranks_fixture():
+    rec = {"ranks": np.array(RANKS)}
+    for b, gp in RANKS_GRIDS.items():
+        rec[f"b{b}_gen_keys"] = np.array(list(gp.keys()))
+        rec[f"b{b}_gen_values"] = np.array([float(v) for v in gp.values()])
+        A, _ = ranks_system(b)
+        rng = np.random.default_rng(100 + b)
+        x0, rhs = rng.standard_normal(A.n), rng.standard_normal(A.n)
+        rec[f"b{b}_x0"], rec[f"b{b}_rhs"] = x0, rhs
+        rec[f"b{b}_a_sha256"] = np.frombuffer(sha(A.a).encode(), dtype=np.uint8)
+        with tempfile.TemporaryDirectory() as tmp:
+            fa, fx, fb, fo = (os.path.join(tmp, n) for n in ("A.bin", "x0.txt", "b.txt", "out.txt"))
+            jx.write_bsr_binary(fa, A)
+            jx.write_rhs_text(fx, x0)
+            jx.write_rhs_text(fb, rhs)
+            for P in RANKS:
+                subprocess.run([MPIRUN, "-np", str(P), MPI_DRIVER, fa, fx, fb, fo], capture_output=True, text=True,
+                               env=dict(os.environ, OMP_NUM_THREADS="1"), check=True, timeout=300)
+                out = jx.read_rhs_text(fo, A.n)
+                rec[f"b{b}_x_p{P}"] = out
+    np.savez_compressed(os.path.join(OUT, "ranks.npz"), **rec)
+    print("ranks", len(RANKS_GRIDS) * len(RANKS), "runs")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["decoup"]:
         decoup_fixture()
+    elif sys.argv[1:] == ["ranks"]:
+        ranks_fixture()
     else:
         main()

@@ -9,7 +9,7 @@ import pytest
 
 import jxpamg_all_amd as jx
 from oracle.cpu_ref import Oracle
-from tests._common import relerr, system
+from tests._common import ranks_system, relerr, system
 
 pytestmark = pytest.mark.gpu
 
@@ -503,3 +503,23 @@ def test_exact_sweep_every_level_vs_oracle(name):
         print(f"[gpu] exact relax 6 {name} level {l} ({n} rows): relerr {err:.2e}", flush=True)
         assert err < OP_TOL
     s.close()
+
+
+@pytest.mark.parametrize("b", [2, 3, 4])
+def test_stage2_partitions_vs_reference_mpi_ranks(b):
+    """stage2_chunk = c on 24 block rows is stage 2 as the reference runs it
+    on P = ceil(24 / c) MPI ranks: one sweep against the reference's own
+    multi-rank sweep (ranks.npz), for c = 8 down to 1 (uneven at c = 5)."""
+    from tests.conftest import load_fixture
+    d = load_fixture("ranks")
+    A = ranks_system(d, b)
+    H = jx.amg_setup(*jx.extract_pressure(A))
+    for c in (8, 6, 5, 4, 3, 2, 1):
+        P = -(-A.nb // c)
+        s = jx.Solver(A, H, jx.default_params(stage2_chunk=c))
+        x = dev(d[f"b{b}_x0"])
+        s.bhgs(x.data_ptr(), dev(d[f"b{b}_rhs"]).data_ptr(), 0.7, 0.9)
+        err = relerr(host(x), d[f"b{b}_x_p{P}"])
+        print(f"[gpu] b={b} stage2_chunk {c} = {P} ranks: relerr {err:.2e}", flush=True)
+        assert err < OP_TOL
+        s.close()

@@ -12,7 +12,8 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from oracle.cpu_ref import Oracle
-from tests._common import relerr, system
+from tests._common import ranks_system, relerr, system
+from tests.conftest import load_fixture
 
 OP_CASES = ["c1", "tiny", "b4", "b2", "lvl3", "nodec", "timpes"]
 
@@ -395,3 +396,18 @@ def test_stage2_rank_partitions(name):
         outside = r[sl] - S[sl, :] @ x0 + S[sl, sl] @ x0[sl]
         expect = spla.spsolve_triangular(DL, outside - Up @ x0[sl], lower=True)
         assert relerr(got[sl], expect) < 1e-11
+
+
+@pytest.mark.parametrize("b", [2, 3, 4])
+def test_partitioned_stage2_bit_exact_vs_reference_on_mpi_ranks(b):
+    """The oracle's partitioned stage-2 sweep (cref_bsr_hgs_parts) against
+    the reference itself run on P MPI ranks (jxf_ParBSRHGSRelax with the
+    matrix split over the ranks, ranks.npz written by oracle/ref_bhgs_mpi
+    under mpirun): bit for bit for every P from 1 to 24 on 24 block rows,
+    including uneven splits (P = 5) and one row per rank (P = 24)."""
+    d = load_fixture("ranks")
+    A = ranks_system(d, b)
+    o = Oracle(A)
+    for P in d["ranks"]:
+        got = o.bhgs_parts(d[f"b{b}_x0"], d[f"b{b}_rhs"], int(P))
+        assert np.array_equal(got, d[f"b{b}_x_p{int(P)}"]), f"P={P}"
