@@ -51,8 +51,9 @@ def parse():
                     help="block rows per stage-2 partition (0: one-rank sweeps); default: the package's value")
     ap.add_argument("--relax-chunk", type=int, default=None,
                     help="rows per relax-6 chunk (0: one-thread sweeps); default: the package's large-workload value")
-    ap.add_argument("--arnoldi-kernel", type=int, default=0,
-                    help="0: one cooperative launch per MGS step, 1: one launch per projection")
+    ap.add_argument("--arnoldi-kernel", type=int, default=2,
+                    help="0: one cooperative launch per MGS step, 1: one launch per projection, "
+                         "2: the one-launch step by an ordinary launch")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the subsystem legs (reference-exact, BILU(0), 4x4 SpMV, decoupling)")
     return ap.parse_args()

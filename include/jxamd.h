@@ -212,8 +212,13 @@ typedef struct {
     /* Arnoldi orthogonalisation kernels: 0 one cooperative launch per MGS
      * step with the new vector held in registers (when the device supports
      * cooperative launches and the vector fits the resident grid), 1 one
-     * launch per projection.  Same arithmetic (MGS, pgmres.c:1258-1290
-     * order); the reductions group their partial sums differently. */
+     * launch per projection, 2 (the default) the same one-launch step as 0
+     * by an ordinary launch of a grid within the resident capacity (no
+     * cooperative-launch synchronisation around it, which leaves the device
+     * idle about 30 us per step; a workgroup that waits too long ends the
+     * solve with an error instead of hanging).  Same arithmetic (MGS,
+     * pgmres.c:1258-1290 order); the reductions group their partial sums
+     * differently. */
     int arnoldi_kernel;
     /* Update of x at the end of a GMRES cycle: 0 x += Z y with the vectors
      * z_j = M^{-1} v_j that the Arnoldi steps formed kept in a second basis

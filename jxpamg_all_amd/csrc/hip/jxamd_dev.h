@@ -133,16 +133,18 @@ int jxamd_k_axpy(size_t n, const double *pa, double a, int neg, const double *x,
 int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *stream);
 int jxamd_dot_grid(size_t n);
 /* one MGS step of the Arnoldi process (k_mgs_coop): plan (grid 0: not
- * available) and cooperative launch; hval[0..m-1] = projections, hval[m] =
- * squared norm before scaling; part >= (m + 1) * grid doubles; hhost (device
- * address of mapped host memory, or NULL) also receives hval[0..m]; a
- * waiting workgroup gives up after spin_limit polls (err[0] = 1; cnt and
- * flag must then be cleared before the next launch) */
+ * available; coop_launch: the device must support cooperative launches) and
+ * launch, cooperative or ordinary (a grid within the resident capacity);
+ * hval[0..m-1] = projections, hval[m] = squared norm before scaling;
+ * part >= (m + 1) * grid doubles; hhost (device address of mapped host
+ * memory, or NULL) also receives hval[0..m]; a waiting workgroup gives up
+ * after spin_limit polls (err[0] = 1; cnt and flag must then be cleared
+ * before the next launch) */
 #define JXAMD_SPIN_LIMIT (1 << 24)
-int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm);
-int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, int spin_limit,
-                     void *stream);
+int jxamd_mgs_coop_plan(size_t n, int coop_launch, int *grid, int *tm);
+int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *V, size_t ld, int m, double *w,
+                     double *hval, double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost,
+                     int spin_limit, void *stream);
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream);
 int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream);

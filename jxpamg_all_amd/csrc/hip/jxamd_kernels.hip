@@ -2151,7 +2151,7 @@ int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *str
 // Cooperative MGS plan: the largest resident grid of MGS_BS-thread
 // workgroups for the smallest register tile TM in {4, 8, 16} that covers n;
 // 0 if none does or the device has no cooperative launch.
-int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm)
+int jxamd_mgs_coop_plan(size_t n, int coop_launch, int *grid, int *tm)
 {
     *grid = 0;
     *tm = 0;
@@ -2159,7 +2159,7 @@ int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm)
     if (hipGetDevice(&dev) != hipSuccess) return JXAMD_ERROR_DEVICE;
     if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess) return JXAMD_ERROR_DEVICE;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return JXAMD_ERROR_DEVICE;
-    if (!coop || cus < 1) return 0;
+    if ((coop_launch && !coop) || cus < 1) return 0;
     const void *fn[3] = {(const void *)k_mgs_coop<4>, (const void *)k_mgs_coop<8>, (const void *)k_mgs_coop<16>};
     const int tms[3] = {4, 8, 16};
     for (int q = 0; q < 3; q++) {
@@ -2179,9 +2179,9 @@ int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm)
     return 0;
 }
 
-int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, int spin_limit,
-                     void *stream)
+int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *V, size_t ld, int m, double *w,
+                     double *hval, double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost,
+                     int spin_limit, void *stream)
 {
     if (grid < 1 || m < 0 || (size_t)grid * MGS_BS * (size_t)tm < n || spin_limit < 0) return JXAMD_ERROR_ARG;
     void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err, &hhost, &spin_limit};
@@ -2192,7 +2192,12 @@ int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int
     case 16: fn = (const void *)k_mgs_coop<16>; break;
     default: return JXAMD_ERROR_ARG;
     }
-    return chk(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(MGS_BS), args, 0, (hipStream_t)stream));
+    if (coop_launch) return chk(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(MGS_BS), args, 0, (hipStream_t)stream));
+    // an ordinary launch of a grid no larger than the resident capacity (at
+    // most one workgroup per CU at this size): the stream's previous kernel
+    // has finished when it starts, so every workgroup is placed at once;
+    // were one ever held back, the bounded waits end with err[0] set
+    return chk(hipLaunchKernel(fn, dim3(grid), dim3(MGS_BS), args, 0, (hipStream_t)stream));
 }
 
 int jxamd_dot_grid(size_t n)

@@ -201,6 +201,7 @@ void jxamd_params_default(jxamd_params *p)
     p->stage2_weight = 0.7;
     p->stage2_omega = 0.9;
     p->convfac_threshold = 1.0e4;
+    p->arnoldi_kernel = 2;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1717,7 +1718,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (p->poly_degree < 0 || p->poly_degree > 8) return JXAMD_ERROR_ARG;
     if (p->stage2_chunk < 0 || p->stage2_chunk > 8) return JXAMD_ERROR_ARG;
     if (p->stage2_precision < 0 || p->stage2_precision > 2) return JXAMD_ERROR_ARG;
-    if (p->arnoldi_kernel < 0 || p->arnoldi_kernel > 1) return JXAMD_ERROR_ARG;
+    if (p->arnoldi_kernel < 0 || p->arnoldi_kernel > 2) return JXAMD_ERROR_ARG;
     if (p->solution_update < 0 || p->solution_update > 1) return JXAMD_ERROR_ARG;
     if (!(p->convfac_threshold > 0.0)) return JXAMD_ERROR_ARG;
     if (p->coarse_rows < 0 || p->coarse_rows > 8192) return JXAMD_ERROR_ARG;
@@ -1796,7 +1797,7 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_dev_malloc((void **)&s->Xd, sizeof(double) * N);
     if (!e) e = jxamd_dev_malloc((void **)&s->dsc, sizeof(double) * (2 * (size_t)p->k_dim + 16));
     if (!e) e = jxamd_dev_malloc((void **)&s->part, sizeof(double) * 4096); /* >= jxamd_dot_grid() blocks */
-    if (!e && p->arnoldi_kernel == 0) e = jxamd_mgs_coop_plan(N, &s->mgs_grid, &s->mgs_tm);
+    if (!e && p->arnoldi_kernel != 1) e = jxamd_mgs_coop_plan(N, p->arnoldi_kernel == 0, &s->mgs_grid, &s->mgs_tm);
     if (!e && s->mgs_grid > 0) {
         e = jxamd_dev_malloc((void **)&s->mpart, sizeof(double) * (size_t)(p->k_dim + 1) * s->mgs_grid);
         if (!e) e = zero_ints(&s->mflag, 1, s->stream);
@@ -2617,7 +2618,8 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
         /* the whole MGS step in one launch: v_0..v_{i-1} read once,
          * w = v_i read and written once */
         int pq = prof_begin(s);
-        const int e = jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, N, VEC(0), N, i, VEC(i), h, s->mpart, s->count + 1,
+        const int e = jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, s->prm.arnoldi_kernel == 0, N, VEC(0), N, i, VEC(i), h,
+                                       s->mpart, s->count + 1,
                                        s->mflag, s->mgs_base, s->err,
                                        s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->spin_limit, s->stream);
         if (!e) {
@@ -2625,8 +2627,9 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
             s->mgs_base += i + 1;
             return jxamd_event_record(s->evh[slot], s->stream); /* the kernel wrote the column to hpin[slot] */
         }
-        /* the cooperative launch was refused (the grid could not be made
-         * resident): nothing ran; use the launch-per-projection MGS from now on */
+        /* the launch was refused (for a cooperative launch: the grid could
+         * not be made resident): nothing ran; use the launch-per-projection
+         * MGS from now on */
         (void)jxamd_last_error(); /* the opened timing slot is reused by the next prof_begin */
         s->mgs_grid = 0;
     }

@@ -41,6 +41,10 @@ CASES = {
     "c2": (dict(jx.CONFIGS["C2"]), 1e-6, 100, 3, 1, 0, 30),
     # lvl3 with OMP_NUM_THREADS=4: the threaded relax-6 (par_relax_6.c) golden vectors
     "lvl3_t4": (dict(jx.CONFIGS["C1"], nx=20, ny=20, nz=10, seed=3), 1e-6, 100, 3, 1, 1, 30),
+    # tiny with OMP_NUM_THREADS=16 and 64: relax-6 chunks of 16 and of 4 rows
+    # on the fine level (256 rows), the chunk size the 60x220x85 benchmark runs
+    "tiny_t16": (dict(jx.CONFIGS["C1"], nx=8, ny=8, nz=4, seed=7), 1e-6, 100, 3, 1, 1, 30),
+    "tiny_t64": (dict(jx.CONFIGS["C1"], nx=8, ny=8, nz=4, seed=7), 1e-6, 100, 3, 1, 1, 30),
     # True-IMPES decoupling (type 5) before the solve, as the reference's hybrid drivers use it
     "timpes": (dict(jx.CONFIGS["C1"], nx=10, ny=10, nz=6, seed=8), 1e-6, 100, 3, 5, 2, 30),
     # the full 60x220x85 system decoupled by ABF and by True-IMPES (SHA-256 of
@@ -73,7 +77,7 @@ RANKS = (1, 2, 3, 4, 5, 6, 8, 12, 24)
 MPI_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_bhgs_mpi")
 MPIRUN = "/opt/conda/bin/mpirun"
 # OpenMP threads of the reference run (1 unless listed)
-THREADS = {"lvl3_t4": 4}
+THREADS = {"lvl3_t4": 4, "tiny_t16": 16, "tiny_t64": 64}
 # operator vectors not kept for these cases (the hierarchy and the solve are)
 SOLVE_ONLY = {"restart", "layered"}
 OPERATOR_KEYS = ("hsgs_", "coarse_", "vcycle_", "bhgs_", "cpr_", "CF")

@@ -110,11 +110,11 @@ int jxamd_k_axpy(size_t n, const double *pa, double a, int neg, const double *x,
 int jxamd_k_scale(size_t n, const double *pnorm2, double a, double *y, void *stream)
 { (void)n; (void)pnorm2; (void)a; (void)y; (void)stream; return 0; }
 int jxamd_dot_grid(size_t n) { (void)n; return 1; }
-int jxamd_mgs_coop_plan(size_t n, int *grid, int *tm) { *grid = n > 0 ? 2 : 0; *tm = 4; return 0; }
-int jxamd_k_mgs_coop(int grid, int tm, size_t n, const double *V, size_t ld, int m, double *w, double *hval,
-                     double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost, int spin_limit,
-                     void *stream)
-{ (void)grid; (void)tm; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)spin_limit; (void)stream; return 0; }
+int jxamd_mgs_coop_plan(size_t n, int coop_launch, int *grid, int *tm) { (void)coop_launch; *grid = n > 0 ? 2 : 0; *tm = 4; return 0; }
+int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *V, size_t ld, int m, double *w,
+                     double *hval, double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost,
+                     int spin_limit, void *stream)
+{ (void)grid; (void)tm; (void)coop_launch; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)spin_limit; (void)stream; return 0; }
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream)
 { (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }

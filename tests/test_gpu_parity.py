@@ -100,9 +100,18 @@ def _threads_per_level(H, relax_threads=0, relax_chunk=0):
     return out
 
 
-def test_threaded_smoother_vs_reference():
-    """relax_threads = 4 against the OMP_NUM_THREADS=4 reference run."""
-    d, Ad, bd, H, meta, s = _solver("lvl3_t4", relax_threads=4)
+@pytest.mark.parametrize("name,T", [("lvl3_t4", 4), ("tiny_t16", 16), ("tiny_t64", 64)])
+def test_threaded_smoother_vs_reference(name, T):
+    """relax_threads = T against the OMP_NUM_THREADS=T reference run; on
+    tiny_t64 also relax_chunk = 4 (the benchmark's chunk), which gives the
+    fine level the same 64 chunks."""
+    if name == "tiny_t64":
+        d, Ad, bd, H, meta, s = _solver(name, relax_chunk=4)
+        u = dev(d["hsgs_u_in"])
+        s.hsgs(0, dev(d["hsgs_f"]).data_ptr(), u.data_ptr())
+        assert relerr(host(u), d["hsgs_u_out"]) < OP_TOL
+        s.close()
+    d, Ad, bd, H, meta, s = _solver(name, relax_threads=T)
     u = dev(d["hsgs_u_in"])
     s.hsgs(0, dev(d["hsgs_f"]).data_ptr(), u.data_ptr())
     assert relerr(host(u), d["hsgs_u_out"]) < OP_TOL
@@ -172,18 +181,26 @@ def test_arnoldi_kernels_agree(name):
     """The one-launch MGS step (cooperative, new vector in registers) and the
     launch-per-projection MGS give the same Arnoldi process: same iteration
     count, residual histories and solutions to rounding (the reductions group
-    partial sums differently)."""
+    partial sums differently); the one-launch step by an ordinary launch
+    gives the cooperative launch's results bit for bit."""
     d, Ad, bd, H, meta, s0 = _solver(name, arnoldi_kernel=0)
     x0, st0, h0 = s0.solve(bd)
     s0.close()
     d, Ad, bd, H, meta, s1 = _solver(name, arnoldi_kernel=1)
     x1, st1, h1 = s1.solve(bd)
     s1.close()
-    print(f"[gpu] {name}: arnoldi kernels iterations {st0.iterations} / {st1.iterations}", flush=True)
+    d, Ad, bd, H, meta, s2 = _solver(name, arnoldi_kernel=2)
+    x2, st2, h2 = s2.solve(bd)
+    assert s2.error_flag() == 0
+    s2.close()
+    print(f"[gpu] {name}: arnoldi kernels iterations {st0.iterations} / {st1.iterations} / {st2.iterations}",
+          flush=True)
     assert st0.iterations == st1.iterations and st0.converged == st1.converged == 1
     n = st0.iterations + 1
     assert relerr(h0[:n], h1[:n]) < 1e-9
     assert relerr(x0, x1) < 1e-9
+    # the same kernel and grid launched without the cooperative launch: bit for bit
+    assert st2.iterations == st0.iterations and np.array_equal(h2[:n], h0[:n]) and np.array_equal(x2, x0)
 
 
 @pytest.mark.parametrize("name", ["lvl3", "restart", "b4"])

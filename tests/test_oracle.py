@@ -93,20 +93,22 @@ def test_relax6_is_forward_then_backward_triangular_solve():
     assert relerr(got, u2) < 1e-13
 
 
-def test_threaded_relax6_bit_exact_vs_reference():
-    """OMP_NUM_THREADS=4 reference run: relax 6 splits every level into four
-    contiguous chunks (par_relax_6.c:132-223)."""
-    d, A, rhs, Ad, bd, H, meta = system("lvl3_t4")
-    assert int(meta["threads"]) == 4
+@pytest.mark.parametrize("name,T", [("lvl3_t4", 4), ("tiny_t16", 16), ("tiny_t64", 64)])
+def test_threaded_relax6_bit_exact_vs_reference(name, T):
+    """OMP_NUM_THREADS=T reference runs: relax 6 splits every level into T
+    contiguous chunks (par_relax_6.c:132-223); T = 64 on the 256-row fine
+    level is the 4-row chunk of the 60x220x85 benchmark."""
+    d, A, rhs, Ad, bd, H, meta = system(name)
+    assert int(meta["threads"]) == T
     o = Oracle(Ad, H)
-    assert np.array_equal(o.relax6_threads(0, d["hsgs_f"], d["hsgs_u_in"], 4), d["hsgs_u_out"])
+    assert np.array_equal(o.relax6_threads(0, d["hsgs_f"], d["hsgs_u_in"], T), d["hsgs_u_out"])
     # the one-thread sweep gives a different answer on the same inputs
     assert not np.array_equal(o.relax6(0, d["hsgs_f"], d["hsgs_u_in"]), d["hsgs_u_out"])
-    o.set_relax_threads([4] * H.num_levels)
+    o.set_relax_threads([T] * H.num_levels)
     assert np.array_equal(o.vcycle(d["vcycle_f"]), d["vcycle_u"])
     assert np.array_equal(o.cpr(d["cpr_r"]), d["cpr_x"])
     x, st, hist, rc = o.gmres(bd, o.params(tol=meta["rtol"]))
-    # the threaded reference also sums its dot products in four parts
+    # the threaded reference also sums its dot products in T parts
     assert st.iterations == int(d["iterations"][0])
     assert relerr(hist, d["res_history"]) < 1e-12
 
