@@ -148,5 +148,13 @@ int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream);
 int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream);
+/* the same with m <= JXAMD_COEFS_MAX coefficients from host memory, passed
+ * by value in the launch (no copy); add: w += sum_j y_j V_j */
+#define JXAMD_COEFS_MAX 32
+typedef struct {
+    double c[JXAMD_COEFS_MAX];
+} jxamd_coefs;
+int jxamd_k_lincomb_host(size_t n, int m, const double *V, size_t ld, const double *y, int add, double *w,
+                         void *stream);
 
 #endif

@@ -1589,6 +1589,21 @@ __global__ void k_lincomb(size_t n, int m, const double *__restrict__ V, size_t 
     }
 }
 
+// the same with the coefficients passed by value (no copy to the device):
+// w = sum_j y[j] V[j], or with ADD x += sum_j y[j] V[j] (x + the sum, as an
+// axpy of the sum with factor 1 would give)
+template <bool ADD>
+__global__ void k_lincomb_args(size_t n, int m, const double *__restrict__ V, size_t ld, jxamd_coefs y,
+                               double *__restrict__ w)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double acc = y.c[m - 1] * V[(size_t)(m - 1) * ld + i];
+        for (int j = m - 2; j >= 0; j--) acc += y.c[j] * V[(size_t)j * ld + i];
+        w[i] = ADD ? w[i] + acc : acc;
+    }
+}
+
 inline int nblocks_stream(size_t n, int bs) {
     size_t b = (n + bs - 1) / bs;
     if (b > 4096) b = 4096;
@@ -2218,6 +2233,18 @@ int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y
 {
     if (n == 0 || m == 0) return 0;
     k_lincomb<<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, m, V, ld, y, w);
+    return chk(hipGetLastError());
+}
+
+int jxamd_k_lincomb_host(size_t n, int m, const double *V, size_t ld, const double *y, int add, double *w,
+                         void *stream)
+{
+    if (n == 0 || m == 0) return 0;
+    if (m < 0 || m > JXAMD_COEFS_MAX || !y) return JXAMD_ERROR_ARG;
+    jxamd_coefs c;
+    for (int j = 0; j < JXAMD_COEFS_MAX; j++) c.c[j] = j < m ? y[j] : 0.0;
+    if (add) k_lincomb_args<true><<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, m, V, ld, c, w);
+    else k_lincomb_args<false><<<nblocks_stream(n, 256), 256, 0, (hipStream_t)stream>>>(n, m, V, ld, c, w);
     return chk(hipGetLastError());
 }
 

@@ -2582,6 +2582,20 @@ static int host_norm(jxamd_solver *s, const double *v, double *nrm)
     return 0;
 }
 
+/* ||u|| and ||v|| with one copy and one wait */
+static int host_norm2(jxamd_solver *s, const double *u, const double *v, double *nu, double *nv)
+{
+    double *slot = s->dsc + 2 * s->prm.k_dim + 8;
+    CK(dev_dot(s, u, (double *)u, NULL, NULL, slot));
+    CK(dev_dot(s, v, (double *)v, NULL, NULL, slot + 1));
+    double h[2];
+    CK(jxamd_d2h(h, slot, sizeof(h), s->stream));
+    CK(jxamd_stream_sync(s->stream));
+    *nu = sqrt(h[0]);
+    *nv = sqrt(h[1]);
+    return 0;
+}
+
 /* CPR application, timed with events when the solve is timed (print_level
  * > 0 or profiling: each timed application is followed by a stream sync) */
 static int precond_timed(jxamd_solver *s, const double *r, double *x, int timed)
@@ -2687,9 +2701,8 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     TRY(jxamd_event_record(s->ev0, s->stream));
     /* P0 = b - A x */
     TRY(bsr_apply(s, x, bvec, VEC(0), 1));
-    TRY(host_norm(s, bvec, &b_norm));
+    TRY(host_norm2(s, bvec, VEC(0), &b_norm, &r_norm)); /* one wait for both */
     if (b_norm != 0.0 && (b_norm / b_norm) != (b_norm / b_norm)) { err = JXAMD_ERROR_GENERIC; goto done; }
-    TRY(host_norm(s, VEC(0), &r_norm));
     r_norm_0 = r_norm;
     if (r_norm != 0.0 && (r_norm / r_norm) != (r_norm / r_norm)) { err = JXAMD_ERROR_GENERIC; goto done; }
     normdown = r_norm;
@@ -2772,16 +2785,24 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             t += rs[k];
             rs[k] = t / hh[k][k];
         }
-        TRY(jxamd_h2d(ydev, rs, sizeof(double) * i, s->stream));
+        /* y goes to the device inside the launch when it is short enough */
+        const int yargs = i <= JXAMD_COEFS_MAX;
+        if (!yargs) TRY(jxamd_h2d(ydev, rs, sizeof(double) * i, s->stream));
         if (s->Z) {
-            /* x += Z y */
+            /* x += Z y (one pass: x + the sum, as the axpy of the sum gives) */
             int pq = prof_begin(s);
-            TRY(jxamd_k_lincomb(N, i, s->Z, N, ydev, s->Wv, s->stream));
-            prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 1));
-            TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Wv, x, s->stream));
+            if (yargs) {
+                TRY(jxamd_k_lincomb_host(N, i, s->Z, N, rs, 1, x, s->stream));
+                prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 2));
+            } else {
+                TRY(jxamd_k_lincomb(N, i, s->Z, N, ydev, s->Wv, s->stream));
+                prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 1));
+                TRY(jxamd_k_axpy(N, NULL, 1.0, 0, s->Wv, x, s->stream));
+            }
         } else {
             int pq = prof_begin(s);
-            TRY(jxamd_k_lincomb(N, i, V, N, ydev, s->Wv, s->stream));
+            if (yargs) TRY(jxamd_k_lincomb_host(N, i, V, N, rs, 0, s->Wv, s->stream));
+            else TRY(jxamd_k_lincomb(N, i, V, N, ydev, s->Wv, s->stream));
             prof_end(s, pq, JXAMD_PROF_VEC, 8.0 * N * (i + 1));
             TRY(precond_timed(s, s->Wv, s->Rv, timed));
             nprec++;

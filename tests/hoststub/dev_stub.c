@@ -118,6 +118,9 @@ int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, void *stream)
 { (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }
+int jxamd_k_lincomb_host(size_t n, int m, const double *V, size_t ld, const double *y, int add, double *w,
+                         void *stream)
+{ (void)n; (void)V; (void)ld; (void)add; (void)w; (void)stream; if (m < 0 || m > JXAMD_COEFS_MAX || (m && !y)) return JXAMD_ERROR_ARG; return 0; }
 int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream)
 { (void)n; (void)m; (void)V; (void)ld; (void)y; (void)w; (void)stream; return 0; }
 
