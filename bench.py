@@ -429,7 +429,7 @@ def main():
     prof = s.profile_read()
     levels = s.profile_read_levels()
     s.profile(False)
-    single = {"bhgs_sweep": ("k_bhgs_part<3, 4, double, double, 4, double>" if s2chunk else "k_bhgs_multi<3>"),
+    single = {"bhgs_sweep": ("k_bhgs_part<3, 4, double, double, 6, double>" if s2chunk else "k_bhgs_multi<3>"),
               "bsr_spmv": "k_bsr_spmv<3, 8, true>", "mgs": "k_mgs_coop<16>", "coarse_solve": "k_dense_gemv<double>",
               "bilu_solve": "k_btri"}
     cands = [(name, prof[c]) for c, name in single.items() if prof[c][0] > 0]

@@ -1843,8 +1843,12 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
     const int maxlen = jxamd_gs_chunk_lds_rows(nb, nparts);
     if (maxlen < 1 || maxlen > 8 || (!xo && !xp) || (vt == JXAMD_T_F16 && (!rsc || !pptr))) return JXAMD_ERROR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    // four slots per batch (FP64 blocks: 3.90 -> 3.76 ms of stage 2 per
-    // 60x220x85 solve against one slot per step)
+    // slots per batch, measured on the 60x220x85 system: FP64 blocks 6
+    // (129 -> 124 us per sweep against 4; 8 needs 142 VGPRs and drops to
+    // three waves per SIMD: 140 us; one slot per step was slower still),
+    // FP32 4, FP16 2 (66 -> 52 us against 4: 75 instead of 109 VGPRs, six
+    // waves per SIMD instead of four)
+    constexpr int PU64 = 6, PU32 = 4, PU16 = 2;
 #define LAUNCH_PART(BS, LM)                                                                                            \
     do {                                                                                                               \
         if (xp_f32) LAUNCH_PART_X(BS, LM, float);                                                                     \
@@ -1857,17 +1861,17 @@ int jxamd_k_bhgs_part(int b, int vt, int nb, int nparts, const int *sptr, const 
         const size_t lb = (size_t)cpb * ((LM * (LM - 1) / 2) * BS * BS + 1);                                          \
         const size_t lds = sizeof(double) * 2 * 256 * BS + (vt == JXAMD_T_F64 ? sizeof(double) : sizeof(float)) * lb; \
         if (vt == JXAMD_T_F64)                                                                                        \
-            k_bhgs_part<BS, LM, double, double, 4, XT><<<grid, 256, lds, st>>>(                                        \
+            k_bhgs_part<BS, LM, double, double, PU64, XT><<<grid, 256, lds, st>>>(                                     \
                 nb, nparts, cpb, sptr, pptr, col, (const double *)val, rsc, (const double *)dinv, rhs, xo,             \
                 (const XT *)xp,                                                                                        \
                 pidx, xn, w, omega);                                                                                   \
         else if (vt == JXAMD_T_F32)                                                                                    \
-            k_bhgs_part<BS, LM, float, float, 4, XT><<<grid, 256, lds, st>>>(                                          \
+            k_bhgs_part<BS, LM, float, float, PU32, XT><<<grid, 256, lds, st>>>(                                       \
                 nb, nparts, cpb, sptr, pptr, col, (const float *)val, rsc, (const float *)dinv, rhs, xo,               \
                 (const XT *)xp,                                                                                        \
                 pidx, xn, w, omega);                                                                                   \
         else                                                                                                           \
-            k_bhgs_part<BS, LM, __half, float, 4, XT><<<grid, 256, lds, st>>>(                                         \
+            k_bhgs_part<BS, LM, __half, float, PU16, XT><<<grid, 256, lds, st>>>(                                      \
                 nb, nparts, cpb, sptr, pptr, col, (const __half *)val, rsc, (const float *)dinv, rhs, xo,              \
                 (const XT *)xp,                                                                                        \
                 pidx, xn, w, omega);                                                                                   \
