@@ -1695,16 +1695,19 @@ int jxamd_k_bsr_spmv(int b, int nb, int unit, const int *sptr, const int *col, c
     if (grid == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     switch (b) {
-    // eight slots per batch: a 7-point row in one batch (3x3 blocks on the
-    // 60x220x85 system: 117.8 us with one slot per step, 114.8 us with eight)
-#define SPMV_B(BS)                                                                                                     \
+    // slots per batch, measured back to back: 3x3 blocks on the 60x220x85
+    // system 8 (a 7-point row in one batch; 117.8 us with one slot per step,
+    // 103.3 us with eight, 103.6 with six, 104.5 with four); 4x4 blocks on
+    // the 100^3 system 6 (167.1 us with eight: 130 VGPRs, three waves per
+    // SIMD; 153.8 us with six, 154.7 with four)
+#define SPMV_B(BS, SU)                                                                                                 \
     do {                                                                                                               \
-        if (unit) k_bsr_spmv<BS, 8, true><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta);   \
-        else k_bsr_spmv<BS, 8, false><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta);        \
+        if (unit) k_bsr_spmv<BS, SU, true><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta);  \
+        else k_bsr_spmv<BS, SU, false><<<grid, 256, 0, st>>>(nb, sptr, col, val, x, rhs, y, mode, alpha, beta);       \
     } while (0)
-    case 2: SPMV_B(2); break;
-    case 3: SPMV_B(3); break;
-    case 4: SPMV_B(4); break;
+    case 2: SPMV_B(2, 8); break;
+    case 3: SPMV_B(3, 8); break;
+    case 4: SPMV_B(4, 6); break;
     default: return JXAMD_ERROR_ARG;
 #undef SPMV_B
     }
