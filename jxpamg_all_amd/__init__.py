@@ -74,11 +74,22 @@ _lib = None
 
 
 def lib():
-    """The loaded libjxamd.so (raises if it has not been built)."""
+    """The loaded libjxamd.so (raises if it has not been built).
+
+    torch, where installed, is imported first: its wheel carries its own HIP
+    runtime, and libjxamd.so must bind to that one (the dynamic linker then
+    finds libamdhip64 already loaded).  Loaded before torch, the library would
+    bring up the system HIP runtime beside torch's, two runtimes with their
+    own queues in one process: device memory from torch and the library's
+    synchronisations would no longer refer to the same device state."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (no CPU fallback)")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = ctypes.CDLL(LIB_PATH)
         _declare(_lib)
     return _lib

@@ -7,11 +7,11 @@
 // Every decoupling type is a per-block-row transformation: a b x b matrix M_i
 // (or a few coefficients) derived from block row i is applied to every block
 // of the row and to the row's right-hand side.  k_decouple_apply runs one
-// workgroup over 256 consecutive block rows: one thread per row forms M_i
-// (in LDS) and updates the rhs, then the workgroup streams the rows' blocks
-// through LDS in tiles of 128-256 blocks (coalesced loads and stores of the
-// whole value range, one thread per block for the arithmetic), so the matrix
-// is read and written once.  True-IMPES needs the column sums of A first
+// wavefront over 64 consecutive block rows: one lane per row forms M_i and
+// updates the rhs, then the wavefront streams the rows' blocks through its
+// own LDS tile of 64 blocks (coalesced loads and stores of the whole value
+// range, one lane per block for the arithmetic), so the matrix is read and
+// written once.  True-IMPES needs the column sums of A first
 // (k_timpes_coef, one thread per block column, gathering the column's blocks
 // through the transpose pattern of a structurally symmetric matrix) and
 // stores its coefficients per row in a work buffer.
@@ -21,6 +21,8 @@
 // output aliases their input); no multiply-add is contracted, so results are
 // the reference's own roundings.
 #include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
 
 extern "C" {
 #include "jxamd_dev.h"
@@ -28,11 +30,18 @@ extern "C" {
 
 #pragma clang fp contract(off)
 
-#define DC_ROWS 256   // block rows per workgroup (one thread each)
-// blocks per LDS tile: 256 (<= 18 KB of values for 3x3), 128 for 4x4 (16 KB)
-#define DC_TILE(B) ((B) == 4 ? 128 : 256)
+#define WAVE_DC 64    // lanes per wavefront: block rows per wavefront and blocks per LDS tile
+#define DC_ROWS 256   // block rows per workgroup (four wavefronts)
 
 namespace {
+
+// HIP status -> error flag; with JXAMD_DEBUG set, failures are named on stderr
+int dc_chk(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    if (getenv("JXAMD_DEBUG")) fprintf(stderr, "jxamd decouple: %s: %s\n", what, hipGetErrorName(e));
+    return JXAMD_ERROR_DEVICE;
+}
 
 // closed-form inverses, jxf_bsr_decoup.c:53-183 (the 4x4 one is the
 // reference's: diagonal cofactors right, off-diagonal ones transposed with
@@ -304,34 +313,54 @@ template <int B, int T> __device__ void apply_rhs(const double *M, double *v)
     }
 }
 
+// LDS exchange between the lanes of one wavefront: the fences keep the
+// compiler from moving LDS accesses across the point; a wavefront's LDS
+// operations complete in order
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per 64 consecutive block rows, the wavefronts of a workgroup
+// independent (no workgroup barrier).  Lane l forms M of row r0 + l and
+// updates the row's rhs.  The wavefront then streams its rows' blocks
+// [ia[r0], ia[r0 + 64]) through its own LDS tile of 64 blocks: each lane
+// loads B*B values of the tile (coalesced; the next tile's loads are issued
+// before the current tile is transformed), transforms one block in LDS with
+// M of the block's row (read from the lane that formed it), and the tile is
+// stored back coalesced.  The matrix is read and written once.
 template <int B, int T>
-__global__ void __launch_bounds__(DC_ROWS) k_decouple_apply(int nb, const int *__restrict__ ia, double *a,
-                                                            double *rhs, int thermal, const double *__restrict__ coef,
-                                                            int pidx, double *__restrict__ p_a)
+__global__ void __launch_bounds__(256) k_decouple_apply(int nb, const int *__restrict__ ia, double *a, double *rhs,
+                                                        int thermal, const double *__restrict__ coef, int pidx,
+                                                        double *__restrict__ p_a)
 {
     constexpr int BB = B * B;
-    __shared__ double sM[DC_ROWS * BB];
-    constexpr int TILE = DC_TILE(B);
-    __shared__ double sv[TILE * BB];
-    __shared__ int sia[DC_ROWS + 1];
-    const int r0 = blockIdx.x * DC_ROWS;
-    const int nr = min(DC_ROWS, nb - r0);
-    const int t = threadIdx.x;
-    for (int q = t; q <= nr; q += DC_ROWS) sia[q] = ia[r0 + q];
-    __syncthreads();
-    if (t < nr) {
-        const int i = r0 + t;
-        double D[BB], M[BB];
+    constexpr int WPB = 4; // wavefronts per workgroup
+    __shared__ double sv[WPB][WAVE_DC * BB];
+    __shared__ int sia[WPB][WAVE_DC + 1];
+    const int w = threadIdx.x / WAVE_DC, lane = threadIdx.x % WAVE_DC;
+    const int r0 = (blockIdx.x * WPB + w) * WAVE_DC;
+    if (r0 >= nb) return;
+    const int nr = min(WAVE_DC, nb - r0);
+    sia[w][lane] = ia[r0 + min(lane, nr)];
+    if (lane == 0) sia[w][WAVE_DC] = ia[r0 + nr];
+    wave_sync();
+    double M[BB];
+#pragma unroll
+    for (int e = 0; e < BB; e++) M[e] = 0.0;
+    if (lane < nr) {
+        const int i = r0 + lane;
+        double D[BB];
         if (T == JXAMD_DECOUP_TIMPES || T == JXAMD_DECOUP_TIMPES2) {
             row_matrix<B, T>(D, thermal, coef + (size_t)i * B, M);
         } else {
-            const double *d = a + (size_t)sia[t] * BB;
+            const double *d = a + (size_t)sia[w][lane] * BB;
 #pragma unroll
             for (int e = 0; e < BB; e++) D[e] = d[e];
             row_matrix<B, T>(D, thermal, nullptr, M);
         }
-#pragma unroll
-        for (int e = 0; e < BB; e++) sM[t * BB + e] = M[e];
         if (rhs) {
             double v[B];
 #pragma unroll
@@ -341,33 +370,54 @@ __global__ void __launch_bounds__(DC_ROWS) k_decouple_apply(int nb, const int *_
             for (int q = 0; q < B; q++) rhs[(size_t)i * B + q] = v[q];
         }
     }
-    __syncthreads();
-    const int kb = sia[0], ke = sia[nr];
-    for (int k0 = kb; k0 < ke; k0 += TILE) {
-        const int nk = min(TILE, ke - k0);
-        double *src = a + (size_t)k0 * BB;
-        for (int e = t; e < nk * BB; e += DC_ROWS) sv[e] = src[e];
-        __syncthreads();
-        if (t < nk) {
-            const int k = k0 + t;
-            // the row holding block k: the last local row whose start is <= k
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (sia[mid] <= k) lo = mid;
-                else hi = mid - 1;
+    const int kb = sia[w][0], ke = sia[w][nr];
+    const size_t vend = (size_t)ke * BB;
+    double buf[BB];
+#pragma unroll
+    for (int e = 0; e < BB; e++) {
+        const size_t x = (size_t)kb * BB + (size_t)e * WAVE_DC + lane;
+        buf[e] = x < vend ? a[x] : 0.0;
+    }
+    for (int k0 = kb; k0 < ke; k0 += WAVE_DC) {
+        const int nk = min(WAVE_DC, ke - k0);
+#pragma unroll
+        for (int e = 0; e < BB; e++) sv[w][e * WAVE_DC + lane] = buf[e];
+        wave_sync();
+        if (k0 + WAVE_DC < ke) {
+#pragma unroll
+            for (int e = 0; e < BB; e++) {
+                const size_t x = (size_t)(k0 + WAVE_DC) * BB + (size_t)e * WAVE_DC + lane;
+                buf[e] = x < vend ? a[x] : 0.0;
             }
+        }
+        // the row holding block k: the last of the wave's rows whose start is <= k
+        const int k = k0 + lane;
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sia[w][mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        double Mr[BB];
+#pragma unroll
+        for (int e = 0; e < BB; e++) Mr[e] = __shfl(M[e], lo, WAVE_DC);
+        if (lane < nk) {
             double blk[BB];
 #pragma unroll
-            for (int e = 0; e < BB; e++) blk[e] = sv[t * BB + e];
-            apply_block<B, T>(sM + lo * BB, k == sia[lo], blk);
+            for (int e = 0; e < BB; e++) blk[e] = sv[w][lane * BB + e];
+            apply_block<B, T>(Mr, k == sia[w][lo], blk);
 #pragma unroll
-            for (int e = 0; e < BB; e++) sv[t * BB + e] = blk[e];
+            for (int e = 0; e < BB; e++) sv[w][lane * BB + e] = blk[e];
             if (p_a) p_a[k] = blk[pidx * B + pidx];
         }
-        __syncthreads();
-        for (int e = t; e < nk * BB; e += DC_ROWS) src[e] = sv[e];
-        __syncthreads();
+        wave_sync();
+        const size_t tend = (size_t)(k0 + nk) * BB;
+#pragma unroll
+        for (int e = 0; e < BB; e++) {
+            const size_t x = (size_t)k0 * BB + (size_t)e * WAVE_DC + lane;
+            if (x < tend) a[x] = sv[w][e * WAVE_DC + lane];
+        }
+        wave_sync();
     }
 }
 
@@ -459,7 +509,7 @@ int launch_apply(int type, int nb, const int *ia, double *a, double *rhs, int th
     case JXAMD_DECOUP_TIMPES: k_decouple_apply<B, JXAMD_DECOUP_TIMPES><<<grid, DC_ROWS, 0, st>>>(nb, ia, a, rhs, thermal, coef, pidx, p_a); break;
     default: k_decouple_apply<B, JXAMD_DECOUP_TIMPES2><<<grid, DC_ROWS, 0, st>>>(nb, ia, a, rhs, thermal, coef, pidx, p_a); break;
     }
-    return hipGetLastError() == hipSuccess ? 0 : JXAMD_ERROR_DEVICE;
+    return dc_chk(hipGetLastError(), "launch");
 }
 
 template <int B>
@@ -470,7 +520,7 @@ int launch_coef(int second, int nb, const int *ia, const int *ja, const double *
     if (grid > 8192) grid = 8192;
     if (second) k_timpes_coef<B, 1><<<grid, 256, 0, st>>>(nb, ia, ja, a, coef, err);
     else k_timpes_coef<B, 0><<<grid, 256, 0, st>>>(nb, ia, ja, a, coef, err);
-    return hipGetLastError() == hipSuccess ? 0 : JXAMD_ERROR_DEVICE;
+    return dc_chk(hipGetLastError(), "launch");
 }
 
 } // namespace
@@ -525,7 +575,8 @@ int jxamd_bsr_decouple_device(int nb, int b, const int *d_ia, const int *d_ja, d
         default: e = launch_apply<4>(type, nb, d_ia, d_a, d_rhs, thermal, coef, pidx, d_p_a, st); break;
         }
     }
-    if (hipStreamSynchronize(st) != hipSuccess && !e) e = JXAMD_ERROR_DEVICE;
+    const int es = dc_chk(hipStreamSynchronize(st), "synchronise");
+    if (es && !e) e = es;
     if (coef && hipFree(coef) != hipSuccess && !e) e = JXAMD_ERROR_DEVICE;
     return e;
 }
@@ -546,7 +597,7 @@ int jxamd_bsr_extract_pressure_device(int nb, int b, int nnzb, const int *d_ia, 
     long long g = ((long long)nnzb + 255) / 256;
     if (g > 16384) g = 16384;
     k_pressure_values<<<(int)g, 256, 0, st>>>(nnzb, b * b, pidx * b + pidx, d_a, d_p_a);
-    return hipGetLastError() == hipSuccess ? 0 : JXAMD_ERROR_DEVICE;
+    return dc_chk(hipGetLastError(), "launch");
 }
 
 } // extern "C"
