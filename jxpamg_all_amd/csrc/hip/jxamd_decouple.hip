@@ -32,6 +32,7 @@ extern "C" {
 
 #define WAVE_DC 64    // lanes per wavefront: block rows per wavefront and blocks per LDS tile
 #define DC_ROWS 256   // block rows per workgroup (four wavefronts)
+#define TC_MAXD 16    // rows of at most this many blocks: True-IMPES column sums from registers
 
 namespace {
 
@@ -436,9 +437,51 @@ __global__ void __launch_bounds__(256) k_timpes_coef(int nb, const int *__restri
 #pragma unroll
         for (int e = 0; e < BB; e++) ac[e] = 0.0;
         const int cb = ia[c], ce = ia[c + 1];
-        int prev = -1;
         bool ok = true;
-        for (int n = 0; n < ce - cb && ok; n++) {
+        if (ce - cb <= TC_MAXD) {
+            // short rows: the row's distinct columns sorted in registers, then
+            // every transpose search advanced one entry per step for all
+            // columns at once (independent loads), then the sum in ascending
+            // row order
+            int cols[TC_MAXD], rb[TC_MAXD], re[TC_MAXD], pos[TC_MAXD];
+            const int deg = ce - cb;
+#pragma unroll
+            for (int n = 0; n < TC_MAXD; n++) cols[n] = n < deg ? ja[cb + n] : 0x7fffffff;
+            // insertion sort, repeated columns dropped
+#pragma unroll
+            for (int n = 1; n < TC_MAXD; n++) {
+                const int v = cols[n];
+                int q = n - 1;
+                while (q >= 0 && cols[q] > v) {
+                    cols[q + 1] = cols[q];
+                    q--;
+                }
+                cols[q + 1] = v;
+            }
+            int nd = 0;
+#pragma unroll
+            for (int n = 0; n < TC_MAXD; n++)
+                if (cols[n] != 0x7fffffff && (nd == 0 || cols[n] != cols[nd - 1])) cols[nd++] = cols[n];
+            int maxlen = 0;
+            for (int n = 0; n < nd; n++) {
+                if (cols[n] < 0 || cols[n] >= nb) ok = false;
+                rb[n] = ok ? ia[cols[n]] : 0;
+                re[n] = ok ? ia[cols[n] + 1] : 0;
+                pos[n] = -1;
+                if (re[n] - rb[n] > maxlen) maxlen = re[n] - rb[n];
+            }
+            for (int t = 0; t < maxlen && ok; t++)
+                for (int n = 0; n < nd; n++)
+                    if (pos[n] < 0 && rb[n] + t < re[n] && ja[rb[n] + t] == c) pos[n] = rb[n] + t;
+            for (int n = 0; n < nd && ok; n++) {
+                if (pos[n] < 0) { ok = false; break; }
+                const double *blk = a + (size_t)pos[n] * BB;
+#pragma unroll
+                for (int e = 0; e < BB; e++) ac[e] += blk[e];
+            }
+        }
+        int prev = -1;
+        for (int n = 0; ce - cb > TC_MAXD && n < ce - cb && ok; n++) {
             // the next row in ascending order among row c's columns
             int i = 0x7fffffff;
             for (int k = cb; k < ce; k++) {
