@@ -194,9 +194,20 @@ __device__ __forceinline__ X sell_dot(int k0, int k1, int w0, int lane, int row,
                 j[u] = row + (int)(short)(wv[u / 2] & 0xffff);
                 j[u + 1] = row + (wv[u / 2] >> 16);
             }
+            if constexpr (constexpr_is_half<V>::value) {
+                // FP16 values paired like the offsets: whole words
+                const __half2 *v2 = reinterpret_cast<const __half2 *>(val);
 #pragma unroll
-            for (int u = 0; u < SELL_U; u++)
-                if (k + u < k1) v[u] = val[(size_t)(k + u) * WAVE + lane];
+                for (int u = 0; u < SELL_U; u += 2) {
+                    const __half2 h = v2[(size_t)(wb + u / 2) * WAVE + lane];
+                    v[u] = __low2half(h);
+                    v[u + 1] = __high2half(h);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < SELL_U; u++)
+                    if (k + u < k1) v[u] = val[(size_t)(k + u) * WAVE + lane];
+            }
         } else {
 #pragma unroll
             for (int u = 0; u < SELL_U; u++) {
@@ -242,9 +253,20 @@ __device__ __forceinline__ void chunk_row_gather(int k0, int k1, int w0, size_t 
                 j[u] = i + (int)(short)(wv[u / 2] & 0xffff);
                 j[u + 1] = i + (wv[u / 2] >> 16);
             }
+            if constexpr (constexpr_is_half<V>::value) {
+                // FP16 values paired like the offsets: whole words
+                const __half2 *v2 = reinterpret_cast<const __half2 *>(val);
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                if (k + u < k1) v[u] = val[(size_t)(k + u) * WAVE + lo];
+                for (int u = 0; u < U; u += 2) {
+                    const __half2 h = v2[(size_t)(wb + u / 2) * WAVE + lo];
+                    v[u] = __low2half(h);
+                    v[u + 1] = __high2half(h);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (k + u < k1) v[u] = val[(size_t)(k + u) * WAVE + lo];
+            }
         } else {
 #pragma unroll
             for (int u = 0; u < U; u++) {

@@ -520,9 +520,32 @@ static int pack16_slices(const int *col, const double *val, const int *sp, int n
     return 1;
 }
 
-/* replace d->sptr / d->col by [sptr | word offsets] and the packed words */
-static int upload_pk16(dsell *d, const int *sp, int nsl, const int32_t *o, const int *wp, size_t rows, void *st)
+/* replace d->sptr / d->col by [sptr | word offsets] and the packed words;
+ * h16 (FP16 layouts: the values' bits, slot-major as uploaded): the values
+ * are stored again as pairs in the words' order (slot 2t of a slice in the
+ * low half of word t, slot 2t + 1 in the high half, zero past the slice's
+ * last slot), so the kernels load them whole with the column words */
+static int upload_pk16(dsell *d, const int *sp, int nsl, const int32_t *o, const int *wp, size_t rows, void *st,
+                       const uint16_t *h16)
 {
+    if (h16) {
+        uint16_t *pv = (uint16_t *)calloc(rows * SELLW * 2, sizeof(uint16_t));
+        if (!pv) return JXAMD_ERROR_MEMORY;
+        for (int q = 0; q < nsl; q++)
+            for (int k = sp[q]; k < sp[q + 1]; k++) {
+                const int t = k - sp[q];
+                for (int ln = 0; ln < SELLW; ln++)
+                    pv[(((size_t)wp[q] + t / 2) * SELLW + ln) * 2 + (t & 1)] = h16[(size_t)k * SELLW + ln];
+            }
+        void *dv = NULL;
+        int rc = jxamd_dev_malloc(&dv, sizeof(uint32_t) * rows * SELLW);
+        if (!rc) rc = jxamd_h2d(dv, pv, sizeof(uint32_t) * rows * SELLW, st);
+        if (!rc) rc = jxamd_stream_sync(st);
+        free(pv);
+        if (rc) { jxamd_dev_free(dv); return rc; }
+        jxamd_dev_free(d->val);
+        d->val = dv;
+    }
     int *sp2 = (int *)malloc(sizeof(int) * 2 * ((size_t)nsl + 1));
     if (!sp2) return JXAMD_ERROR_MEMORY;
     memcpy(sp2, sp, sizeof(int) * ((size_t)nsl + 1));
@@ -599,6 +622,7 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsc
                         int pack)
 {
     memset(d, 0, sizeof(*d));
+    uint16_t *h16 = NULL; /* FP16 values' bits, kept for the packed layout */
     d->n = h->n;
     d->ncols = ncols;
     d->nsl = h->nsl;
@@ -648,8 +672,9 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsc
         }
         int rc = jxamd_h2d(d->val, tmp, type_size(vt) * slots, st);
         if (!rc) rc = jxamd_stream_sync(st);
-        free(tmp);
         free(sc);
+        if (rc || vt != JXAMD_T_F16) free(tmp);
+        else h16 = (uint16_t *)tmp;
         CK(rc);
     }
     if (pack && !h->perm && ncols == h->n) {
@@ -657,12 +682,14 @@ static int dsell_upload(dsell *d, const hsell *h, int ncols, void *st, const hsc
         int *wp = NULL;
         size_t rows = 0;
         if (pack16_slices(h->col, h->val, h->sptr, h->nsl, SELLW, 1, h->n, &o, &wp, &rows)) {
-            const int rc = upload_pk16(d, h->sptr, h->nsl, o, wp, rows, st);
+            const int rc = upload_pk16(d, h->sptr, h->nsl, o, wp, rows, st, h16);
             free(o);
             free(wp);
+            if (rc) free(h16);
             CK(rc);
         }
     }
+    free(h16);
     if (h->perm) {
         CK(jxamd_dev_malloc((void **)&d->perm, sizeof(int) * (h->n ? h->n : 1)));
         CK(jxamd_h2d(d->perm, h->perm, sizeof(int) * h->n, st));
@@ -793,7 +820,7 @@ static int dsellg_upload(dsell *d, int n, int ncols, const int *ia, const int *j
     int *wp = NULL;
     size_t rows = 0;
     if (!rc && n == ncols && pack16_slices(cl, vl, sp, nsl, R, G, n, &o, &wp, &rows))
-        rc = upload_pk16(d, sp, nsl, o, wp, rows, st);
+        rc = upload_pk16(d, sp, nsl, o, wp, rows, st, vt == JXAMD_T_F16 ? (const uint16_t *)tmp : NULL);
     free(o); free(wp);
     free(sp); free(cl); free(vl); free(sc); free(tmp);
     CK(rc);
