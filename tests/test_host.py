@@ -250,6 +250,26 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_library_shares_torchs_hip_runtime_whatever_the_import_order():
+    """libjxamd.so loaded in a fresh process before torch is imported ends up
+    with one HIP runtime in the process (torch's): device memory from torch
+    and the library's launches and synchronisations then refer to one device
+    state (a second runtime beside torch's answered hipErrorNoDevice)."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import jxpamg_all_amd as jx\n"
+            "jx.lib()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "print(sorted(set(l.split()[-1] for l in maps if 'libamdhip64' in l)))\n") % ROOT
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    import ast
+    runtimes = ast.literal_eval(r.stdout.strip().splitlines()[-1])
+    assert len(runtimes) == 1, runtimes
+
+
 def test_solver_rejects_bad_input_without_gpu_calls():
     """Shape errors are caught on the host before any device work."""
     import ctypes
