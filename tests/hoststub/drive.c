@@ -49,6 +49,9 @@ int main(int argc, char **argv)
                 if (ci == 5 && s2 == 1) p.poly_degree = 4;
                 if (ci == 1 || ci == 3) p.stage2_chunk = ci == 1 ? 1 : 5;
                 p.solution_update = (prec + s2) % 2; /* both end-of-cycle updates */
+                p.stage2_precision = (ci + s2) % 3;  /* FP64 / FP32 / FP16 stage-2 blocks or BILU factors */
+                p.arnoldi_kernel = (ci + prec) % 3;  /* cooperative, per projection, ordinary one-launch */
+                if (ci == 2) p.coarse_rows = 64;     /* V-cycle cut at the first level of <= 64 rows */
                 jxamd_solver *s = NULL;
                 int rc = jxamd_solver_create(&s, &A, &H, &p);
                 jxamd_stats st;
