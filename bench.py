@@ -493,6 +493,23 @@ def main():
             "back_to_back_mean_launch_us": round(ms * 1e3, 2),
             "back_to_back_GBs": round(spmv_read / (ms * 1e-3) / 1e9, 1),
             "traffic": read_traffic("k_bsr_spmv<3, 8, true>")}
+    # what plain streaming attains on this GPU, for scale: a read-only
+    # reduction and a device copy over 2 GiB (torch's kernels)
+    big = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+    big2 = torch.empty_like(big)
+    for key, fn, nbytes in (("device_read_GBs", lambda: big.sum(), 8 << 28),
+                            ("device_copy_GBs", lambda: big2.copy_(big), 16 << 28)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        spmv[key] = round(nbytes / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9, 1)
+    del big, big2
+    spmv["back_to_back_over_device_read"] = round(spmv["back_to_back_GBs"] / spmv["device_read_GBs"], 3)
     log(f"BSR SpMV {spmv}")
     assert s.error_flag() == 0
     if extra is not None:
