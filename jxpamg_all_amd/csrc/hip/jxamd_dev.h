@@ -137,16 +137,17 @@ int jxamd_dot_grid(size_t n);
  * launch, cooperative or ordinary (a grid within the resident capacity);
  * hval[0..m-1] = projections, hval[m] = squared norm before scaling;
  * part >= (m + 1) * grid doubles; hhost (device address of mapped host
- * memory, or NULL) also receives hval[0..m]; a waiting workgroup gives up
+ * memory, or NULL) also receives hval[0..m] and, if hpos >= 0 (> m), then
+ * hseq at hhost[hpos] (host polling instead of a completion event); a waiting workgroup gives up
  * after spin_limit polls (err[0] = 1; cnt and flag must then be cleared
  * before the next launch) */
 #define JXAMD_SPIN_LIMIT (1 << 24)
 int jxamd_mgs_coop_plan(size_t n, int coop_launch, int *grid, int *tm);
 int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *V, size_t ld, int m, double *w,
                      double *hval, double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost,
-                     int spin_limit, void *stream);
+                     int spin_limit, int hpos, double hseq, void *stream);
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
-                    unsigned int *count, double *out, void *stream);
+                    unsigned int *count, double *out, double *hout, void *stream);
 int jxamd_k_lincomb(size_t n, int m, const double *V, size_t ld, const double *y, double *w, void *stream);
 /* the same with m <= JXAMD_COEFS_MAX coefficients from host memory, passed
  * by value in the launch (no copy); add: w += sum_j y_j V_j */

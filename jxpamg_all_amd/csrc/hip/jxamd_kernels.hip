@@ -1391,10 +1391,12 @@ __global__ void k_scale(size_t n, const double *__restrict__ pnorm2, double a, d
 //   if (v_prev): y -= (*h_prev) * v_prev   (h_prev from device memory)
 //   out = <u, y>  (u == y gives the squared norm)
 // Block partials land in part[]; the last block to finish sums them in a
-// fixed order and writes *out, then resets the counter.
+// fixed order and writes *out (and, if hout, the same value to hout, a
+// device address of mapped host memory, by a system-scope store the host
+// may poll), then resets the counter.
 __global__ void __launch_bounds__(256) k_dot_mgs(size_t n, const double *u, double *y,
                                                  const double *__restrict__ v_prev, const double *__restrict__ h_prev,
-                                                 double *part, unsigned int *count, double *out)
+                                                 double *part, unsigned int *count, double *out, double *hout)
 {
     __shared__ double sm[256 / WAVE];
     __shared__ bool last;
@@ -1469,6 +1471,7 @@ __global__ void __launch_bounds__(256) k_dot_mgs(size_t n, const double *u, doub
         if (threadIdx.x == 0) {
             *out = a2;
             __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (hout) __hip_atomic_store(hout, a2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }
@@ -1508,7 +1511,7 @@ template <int TM>
 __global__ void __launch_bounds__(MGS_BS) k_mgs_coop(size_t n, const double *__restrict__ V, size_t ld, int m,
                                                      double *__restrict__ w, double *hval, double *part,
                                                      unsigned int *cnt, int *flag, int base, int *err,
-                                                     double *hhost, int spin_limit)
+                                                     double *hhost, int spin_limit, int hpos, double hseq)
 {
     __shared__ double red[MGS_BS / WAVE];
     __shared__ double hb;
@@ -1564,11 +1567,19 @@ __global__ void __launch_bounds__(MGS_BS) k_mgs_coop(size_t n, const double *__r
                 hb = sum;
             }
             if (k == m && hhost) {
-                // the whole column to pinned host memory (system-scope stores,
-                // visible to the host once the launch has completed)
+                // the whole column to pinned host memory (system-scope
+                // stores), then, if hpos >= 0, the sequence number hseq at
+                // hhost[hpos] once every lane's column stores are ordered
+                // before it: the host may poll that word instead of waiting
+                // for the launch to complete
                 for (int q = threadIdx.x; q <= m; q += blockDim.x)
                     __hip_atomic_store(hhost + q, q == m ? sum : ld_coherent(hval + q), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
+                if (hpos >= 0) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                    __syncthreads();
+                    if (threadIdx.x == 0) __hip_atomic_store(hhost + hpos, hseq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
             }
         } else if (threadIdx.x == 0) {
             int spins = 0;
@@ -2225,10 +2236,11 @@ int jxamd_mgs_coop_plan(size_t n, int coop_launch, int *grid, int *tm)
 
 int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *V, size_t ld, int m, double *w,
                      double *hval, double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost,
-                     int spin_limit, void *stream)
+                     int spin_limit, int hpos, double hseq, void *stream)
 {
     if (grid < 1 || m < 0 || (size_t)grid * MGS_BS * (size_t)tm < n || spin_limit < 0) return JXAMD_ERROR_ARG;
-    void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err, &hhost, &spin_limit};
+    if (hpos >= 0 && (!hhost || hpos <= m)) return JXAMD_ERROR_ARG;
+    void *args[] = {&n, &V, &ld, &m, &w, &hval, &part, &cnt, &flag, &base, &err, &hhost, &spin_limit, &hpos, &hseq};
     const void *fn;
     switch (tm) {
     case 4: fn = (const void *)k_mgs_coop<4>; break;
@@ -2252,9 +2264,9 @@ int jxamd_dot_grid(size_t n)
 }
 
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
-                    unsigned int *count, double *out, void *stream)
+                    unsigned int *count, double *out, double *hout, void *stream)
 {
-    k_dot_mgs<<<jxamd_dot_grid(n), 256, 0, (hipStream_t)stream>>>(n, u, y, v_prev, h_prev, part, count, out);
+    k_dot_mgs<<<jxamd_dot_grid(n), 256, 0, (hipStream_t)stream>>>(n, u, y, v_prev, h_prev, part, count, out, hout);
     return chk(hipGetLastError());
 }
 

@@ -17,12 +17,15 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <stdint.h>
 #include <string.h>
+#include <time.h>
 
 #include "jxamd.h"
 #include "../hip/jxamd_dev.h"
 
 #define SELLW 64
+#define HSM_WORDS 8 /* pinned words after the two Arnoldi column slots */
 
 #define CK(x)                                                                                                          \
     do {                                                                                                               \
@@ -93,6 +96,9 @@ struct jxamd_solver_s {
     void *evh[2];          /* Arnoldi columns copied to hpin[slot] (completion events) */
     double *hpin;          /* pinned host memory: 2 x (k_dim + 2) doubles */
     double *hpin_dev;      /* its device address (the cooperative MGS step writes the column there) */
+    double *hsm, *hsm_dev; /* pinned words after the columns (host / device address): squared norms, error flag */
+    double hseq;           /* last sequence number handed to a one-launch MGS step */
+    double hwant[2];       /* per slot: the sequence number its column ends with (0: wait on evh[slot]) */
     int nb, b;
     size_t N;
     dbsr A;
@@ -1790,8 +1796,13 @@ int jxamd_solver_create(jxamd_solver **out, const jxamd_bsr *A, const jxamd_hier
     if (!e) e = jxamd_event_create(&s->ev3);
     if (!e) e = jxamd_event_create_sync(&s->evh[0]);
     if (!e) e = jxamd_event_create_sync(&s->evh[1]);
-    if (!e) e = jxamd_host_alloc((void **)&s->hpin, sizeof(double) * 2 * ((size_t)p->k_dim + 2));
+    if (!e) e = jxamd_host_alloc((void **)&s->hpin, sizeof(double) * (2 * ((size_t)p->k_dim + 2) + HSM_WORDS));
     if (!e) e = jxamd_host_dev_ptr((void **)&s->hpin_dev, s->hpin);
+    if (!e) {
+        memset(s->hpin, 0, sizeof(double) * (2 * ((size_t)p->k_dim + 2) + HSM_WORDS));
+        s->hsm = s->hpin + 2 * ((size_t)p->k_dim + 2);
+        s->hsm_dev = s->hpin_dev + 2 * ((size_t)p->k_dim + 2);
+    }
     if (!e && getenv("JXAMD_DEBUG"))
         fprintf(stderr, "[jxamd] solver %p: pinned column buffer host %p device %p\n", (void *)s, (void *)s->hpin,
                 (void *)s->hpin_dev);
@@ -2538,10 +2549,11 @@ int jxamd_op_bhgs(jxamd_solver *s, double *d_x, const double *d_b, double w, dou
 /* ---------------------------------------------------------------------- */
 /* GMRES (pgmres.c:950-1713)                                               */
 /* ---------------------------------------------------------------------- */
-static int dev_dot(jxamd_solver *s, const double *u, double *y, const double *vprev, const double *hprev, double *out)
+static int dev_dot(jxamd_solver *s, const double *u, double *y, const double *vprev, const double *hprev, double *out,
+                   double *hout)
 {
     int pq = prof_begin(s);
-    int e = jxamd_k_dot_mgs(s->N, u, y, vprev, hprev, s->part, s->count, out, s->stream);
+    int e = jxamd_k_dot_mgs(s->N, u, y, vprev, hprev, s->part, s->count, out, hout, s->stream);
     /* reads u and y (and v_prev, writing y) */
     prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * s->N * (vprev ? (u == y ? 3.0 : 4.0) : (u == y ? 1.0 : 2.0)));
     return e;
@@ -2598,28 +2610,81 @@ int jxamd_solver_profile_read_level(jxamd_solver *s, int level, long *launches, 
     return 0;
 }
 
-static int host_norm(jxamd_solver *s, const double *v, double *nrm)
+static double mono_s(void)
 {
-    double *slot = s->dsc + 2 * s->prm.k_dim + 8;
-    CK(dev_dot(s, v, (double *)v, NULL, NULL, slot));
-    double h;
-    CK(jxamd_d2h(&h, slot, sizeof(double), s->stream));
-    CK(jxamd_stream_sync(s->stream));
-    *nrm = sqrt(h);
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Squared norms reach the host through pinned words (hsm): the dot
+ * kernel's last workgroup writes its result there as well as to the device
+ * slot, so a reading takes no copy, and the host learns of it as soon as
+ * it is written, not when the stream drains.  A word holds an armed
+ * pattern, a signalling NaN that no arithmetic result has, until its
+ * kernel has written it. */
+static const uint64_t HSM_ARMED = 0x7ff4a5a5a5a5a5a5ull;
+
+static void hsm_arm(double *w) { memcpy(w, &HSM_ARMED, sizeof(double)); }
+
+static int hsm_written(const double *w)
+{
+    const double d = *(volatile const double *)w;
+    uint64_t u;
+    memcpy(&u, &d, sizeof(u));
+    return u != HSM_ARMED;
+}
+
+/* wait until hsm[0..k-1] are written: polling, then after 0.2 s a wait
+ * for the stream, after which they must be there */
+static int hsm_wait(jxamd_solver *s, int k)
+{
+    double t0 = -1.0;
+    for (int q = 0; q < k; q++) {
+        for (unsigned long spin = 1; !hsm_written(s->hsm + q); spin++) {
+            if ((spin & 4095) == 0) {
+                if (t0 < 0.0) t0 = mono_s();
+                else if (mono_s() - t0 > 0.2) {
+                    CK(jxamd_stream_sync(s->stream));
+                    for (int r = q; r < k; r++)
+                        if (!hsm_written(s->hsm + r)) return JXAMD_ERROR_GENERIC;
+                    q = k;
+                    break;
+                }
+            }
+        }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
     return 0;
 }
 
-/* ||u|| and ||v|| with one copy and one wait */
-static int host_norm2(jxamd_solver *s, const double *u, const double *v, double *nu, double *nv)
+static int host_norm(jxamd_solver *s, const double *v, double *nrm)
 {
     double *slot = s->dsc + 2 * s->prm.k_dim + 8;
-    CK(dev_dot(s, u, (double *)u, NULL, NULL, slot));
-    CK(dev_dot(s, v, (double *)v, NULL, NULL, slot + 1));
-    double h[2];
-    CK(jxamd_d2h(h, slot, sizeof(h), s->stream));
-    CK(jxamd_stream_sync(s->stream));
-    *nu = sqrt(h[0]);
-    *nv = sqrt(h[1]);
+    hsm_arm(s->hsm);
+    CK(dev_dot(s, v, (double *)v, NULL, NULL, slot, s->hsm_dev));
+    CK(hsm_wait(s, 1));
+    *nrm = sqrt(s->hsm[0]);
+    return 0;
+}
+
+/* ||u|| and ||v|| with one wait, in two halves: the kernels, then the
+ * reading.  The squared norm of v also stays in the device word
+ * dsc[2 k_dim + 9]. */
+static int host_norm2_enqueue(jxamd_solver *s, const double *u, const double *v)
+{
+    double *slot = s->dsc + 2 * s->prm.k_dim + 8;
+    hsm_arm(s->hsm);
+    hsm_arm(s->hsm + 1);
+    CK(dev_dot(s, u, (double *)u, NULL, NULL, slot, s->hsm_dev));
+    return dev_dot(s, v, (double *)v, NULL, NULL, slot + 1, s->hsm_dev + 1);
+}
+
+static int host_norm2_read(jxamd_solver *s, double *nu, double *nv)
+{
+    CK(hsm_wait(s, 2));
+    *nu = sqrt(s->hsm[0]);
+    *nv = sqrt(s->hsm[1]);
     return 0;
 }
 
@@ -2644,8 +2709,10 @@ static int accumulate_precond_time(jxamd_solver *s)
 
 /* Enqueue Arnoldi step i: v_i = A M^{-1} v_{i-1}, MGS against v_0..v_{i-1}
  * and normalisation (column in h on the device: projections, then the
- * squared norm), then the column's copy to hpin[slot] and its completion
- * event.  Nothing here waits for the device. */
+ * squared norm), then the column's copy to hpin[slot].  The one-launch MGS
+ * kernel writes the column there itself and then a fresh sequence number in
+ * the slot's last word (hwant[slot]); the other paths record the slot's
+ * completion event (hwant[slot] = 0).  Nothing here waits for the device. */
 static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
 {
     const size_t N = s->N;
@@ -2659,14 +2726,18 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
         /* the whole MGS step in one launch: v_0..v_{i-1} read once,
          * w = v_i read and written once */
         int pq = prof_begin(s);
+        const double seq = s->hseq + 1.0;
         const int e = jxamd_k_mgs_coop(s->mgs_grid, s->mgs_tm, s->prm.arnoldi_kernel == 0, N, VEC(0), N, i, VEC(i), h,
                                        s->mpart, s->count + 1,
                                        s->mflag, s->mgs_base, s->err,
-                                       s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->spin_limit, s->stream);
+                                       s->hpin_dev + (size_t)slot * (s->prm.k_dim + 2), s->spin_limit,
+                                       s->prm.k_dim + 1, seq, s->stream);
         if (!e) {
             prof_end(s, pq, JXAMD_PROF_MGS, 8.0 * N * (i + 2));
             s->mgs_base += i + 1;
-            return jxamd_event_record(s->evh[slot], s->stream); /* the kernel wrote the column to hpin[slot] */
+            s->hseq = seq;
+            s->hwant[slot] = seq; /* the kernel writes the column, then seq, to hpin[slot] */
+            return 0;
         }
         /* the launch was refused (for a cooperative launch: the grid could
          * not be made resident): nothing ran; use the launch-per-projection
@@ -2676,16 +2747,45 @@ static int arnoldi_enqueue(jxamd_solver *s, int i, int slot, int timed)
     }
     if (s->mgs_grid == 0) {
         /* fused MGS: kernel j subtracts h[j-1] v_{j-1} and forms h[j] = <v_j, w> */
-        CK(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0));
-        for (int j = 1; j < i; j++) CK(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j));
-        CK(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i));
+        CK(dev_dot(s, VEC(0), VEC(i), NULL, NULL, h + 0, NULL));
+        for (int j = 1; j < i; j++) CK(dev_dot(s, VEC(j), VEC(i), VEC(j - 1), h + j - 1, h + j, NULL));
+        CK(dev_dot(s, VEC(i), VEC(i), VEC(i - 1), h + i - 1, h + i, NULL));
         int pq = prof_begin(s);
         CK(jxamd_k_scale(N, h + i, 0.0, VEC(i), s->stream));
         prof_end(s, pq, JXAMD_PROF_VEC, 16.0 * N);
     }
 #undef VEC
     CK(jxamd_d2h(s->hpin + (size_t)slot * (s->prm.k_dim + 2), h, sizeof(double) * (i + 1), s->stream));
+    s->hwant[slot] = 0.0;
     return jxamd_event_record(s->evh[slot], s->stream);
+}
+
+/* Wait until hpin[slot] holds the column arnoldi_enqueue last put there.
+ * A one-launch MGS step announces its column by the sequence number in the
+ * slot's last word: the host polls that word (no completion event, so the
+ * stream carries no marker between the step and the next one, and the host
+ * learns of the column as the kernel writes it).  After 0.2 s of polling the
+ * host waits for the stream instead and then requires the number to be
+ * there: a step whose bounded waits gave up may never write it, and the
+ * solve then ends with JXAMD_ERROR_GENERIC (its caller finds the device
+ * error flag set and resets the synchronisation state). */
+static int column_wait(jxamd_solver *s, int slot)
+{
+    const double want = s->hwant[slot];
+    if (want == 0.0) return jxamd_event_sync(s->evh[slot]);
+    volatile const double *f = s->hpin + (size_t)slot * (s->prm.k_dim + 2) + s->prm.k_dim + 1;
+    if (*f != want) {
+        const double t0 = mono_s();
+        for (unsigned long spin = 1; *f != want; spin++) {
+            if ((spin & 4095) == 0 && mono_s() - t0 > 0.2) {
+                CK(jxamd_stream_sync(s->stream));
+                if (*f != want) return JXAMD_ERROR_GENERIC;
+                break;
+            }
+        }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    return 0;
 }
 
 int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, double *res_history, jxamd_stats *st)
@@ -2704,6 +2804,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     double *hcol = (double *)calloc(k_dim + 2, sizeof(double));
     double **hh = (double **)calloc(k_dim + 1, sizeof(double *));
     int err = 0, converged = 0, nprec = 0, nrestart = 0, nfalse = 0, diverged = 0;
+    int eflag = -1; /* the device error flag, once read */
     if (!rs || !c || !sn || !hcol || !hh) err = JXAMD_ERROR_MEMORY;
     for (int q = 0; q <= k_dim && !err; q++)
         if (!(hh[q] = (double *)calloc(k_dim, sizeof(double)))) err = JXAMD_ERROR_MEMORY;
@@ -2728,7 +2829,19 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     TRY(jxamd_event_record(s->ev0, s->stream));
     /* P0 = b - A x */
     TRY(bsr_apply(s, x, bvec, VEC(0), 1));
-    TRY(host_norm2(s, bvec, VEC(0), &b_norm, &r_norm)); /* one wait for both */
+    TRY(host_norm2_enqueue(s, bvec, VEC(0)));
+    /* Speculation: the first cycle's v_0 = r_0 / ||r_0|| (the norm read on
+     * the device: 1 / sqrt of the same squared norm, as the host forms it)
+     * and its step 1 go in before the norms reach the host, so the device
+     * does not idle through that round trip.  A solve that stops before the
+     * cycle (zero or converged initial residual) leaves the step unused. */
+    int spec = 0;
+    if (!timed && k_dim >= 1 && max_iter >= 1) {
+        TRY(jxamd_k_scale(N, s->dsc + 2 * k_dim + 9, 0.0, VEC(0), s->stream));
+        TRY(arnoldi_enqueue(s, 1, 1, timed));
+        spec = 1;
+    }
+    TRY(host_norm2_read(s, &b_norm, &r_norm)); /* one wait for both */
     if (b_norm != 0.0 && (b_norm / b_norm) != (b_norm / b_norm)) { err = JXAMD_ERROR_GENERIC; goto done; }
     r_norm_0 = r_norm;
     if (r_norm != 0.0 && (r_norm / r_norm) != (r_norm / r_norm)) { err = JXAMD_ERROR_GENERIC; goto done; }
@@ -2747,10 +2860,18 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
             if (r_norm <= epsilon) { converged = 1; true_norm = r_norm; break; }
             nfalse++;
             if (s->prm.is_check_restarted == 0) break;
+            if (spec) {
+                /* the speculative cycle start is not used: v_0 back to the
+                 * unscaled b - A x (the same kernel, the same values) */
+                TRY(bsr_apply(s, x, bvec, VEC(0), 1));
+                spec = 0;
+            }
         }
-        TRY(jxamd_k_scale(N, NULL, 1.0 / r_norm, VEC(0), s->stream));
+        if (!spec) TRY(jxamd_k_scale(N, NULL, 1.0 / r_norm, VEC(0), s->stream));
         i = 0;
-        int ahead = 0; /* step i + 1 already enqueued */
+        int ahead = spec; /* step i + 1 already enqueued */
+        nprec += spec;
+        spec = 0;
         while (i < k_dim && iter < max_iter) {
             i++;
             iter++;
@@ -2775,7 +2896,7 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
                 }
             }
             if (timed) TRY(accumulate_precond_time(s)); /* also synchronises the stream */
-            else TRY(jxamd_event_sync(s->evh[slot]));
+            else TRY(column_wait(s, slot));
             memcpy(hcol, s->hpin + (size_t)slot * (k_dim + 2), sizeof(double) * (i + 1));
             for (j = 0; j < i; j++) hh[j][i - 1] = hcol[j];
             hh[i][i - 1] = sqrt(hcol[i]);
@@ -2866,7 +2987,9 @@ int jxamd_solver_solve_device(jxamd_solver *s, const double *bvec, double *x, do
     }
 done:
     if (!err) err = jxamd_event_record(s->ev1, s->stream);
+    if (!err) err = jxamd_d2h(s->hsm + 4, s->err, sizeof(int), s->stream); /* to pinned memory: no wait of its own */
     if (!err) err = jxamd_stream_sync(s->stream);
+    if (!err) memcpy(&eflag, (const void *)(s->hsm + 4), sizeof(int));
     if (st) {
         memset(st, 0, sizeof(*st));
         st->iterations = iter;
@@ -2893,9 +3016,12 @@ done:
             st->true_rel_res = b_norm > 0.0 ? tr / b_norm : tr;
         }
     }
-    if (!err) {
-        int ef = 0;
-        if (!jxamd_solver_error_flag(s, &ef) && ef) {
+    if (!err || err == JXAMD_ERROR_GENERIC) {
+        /* (a give-up can also keep a one-launch MGS step from announcing its
+         * column: column_wait then ends the solve with JXAMD_ERROR_GENERIC) */
+        int ef = eflag;
+        if (ef < 0 && jxamd_solver_error_flag(s, &ef)) ef = 0;
+        if (ef) {
             /* a bounded wait gave up during this solve: its result is not
              * trusted, and the synchronisation state is reset for the next */
             err = JXAMD_ERROR_GENERIC;

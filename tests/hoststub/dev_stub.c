@@ -113,11 +113,20 @@ int jxamd_dot_grid(size_t n) { (void)n; return 1; }
 int jxamd_mgs_coop_plan(size_t n, int coop_launch, int *grid, int *tm) { (void)coop_launch; *grid = n > 0 ? 2 : 0; *tm = 4; return 0; }
 int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *V, size_t ld, int m, double *w,
                      double *hval, double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost,
-                     int spin_limit, void *stream)
-{ (void)grid; (void)tm; (void)coop_launch; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)spin_limit; (void)stream; return 0; }
+                     int spin_limit, int hpos, double hseq, void *stream)
+{ if (hhost && hpos >= 0) hhost[hpos] = hseq; /* the device stand-in writes the column's sequence number at once */
+  (void)grid; (void)tm; (void)coop_launch; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)spin_limit; (void)stream; return 0; }
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
-                    unsigned int *count, double *out, void *stream)
-{ (void)n; (void)u; (void)y; (void)v_prev; (void)h_prev; (void)part; (void)count; (void)out; (void)stream; return 0; }
+                    unsigned int *count, double *out, double *hout, void *stream)
+{ /* the device stand-in computes the product on the host (device memory is host memory here) */
+  double a = 0.0;
+  for (size_t i = 0; i < n; i++) {
+      if (v_prev) y[i] -= *h_prev * v_prev[i];
+      a += u[i] * y[i];
+  }
+  *out = a;
+  if (hout) *hout = a;
+  (void)part; (void)count; (void)stream; return 0; }
 int jxamd_k_lincomb_host(size_t n, int m, const double *V, size_t ld, const double *y, int add, double *w,
                          void *stream)
 { (void)n; (void)V; (void)ld; (void)add; (void)w; (void)stream; if (m < 0 || m > JXAMD_COEFS_MAX || (m && !y)) return JXAMD_ERROR_ARG; return 0; }

@@ -204,6 +204,47 @@ def test_arnoldi_kernels_agree(name):
 
 
 @pytest.mark.parametrize("name", ["lvl3", "restart", "b4"])
+def test_speculative_cycle_start_bit_exact(name):
+    """An unprofiled solve scales v_0 on the device and enqueues its first
+    Arnoldi step before the norms reach the host, and looks a step ahead; a
+    profiled solve does neither (host-side 1 / ||r_0||, one step at a time).
+    Same iterations, residual histories and x, bit for bit."""
+    d, Ad, bd, H, meta, s = _solver(name)
+    x0, st0, h0 = s.solve(bd)
+    s.profile(True)
+    x1, st1, h1 = s.solve(bd)
+    s.profile(False)
+    x2, st2, h2 = s.solve(bd)
+    assert s.error_flag() == 0
+    s.close()
+    n = st0.iterations + 1
+    print(f"[gpu] {name}: iterations {st0.iterations} / {st1.iterations} / {st2.iterations}", flush=True)
+    assert st0.converged == st1.converged == 1
+    assert st0.iterations == st1.iterations == st2.iterations
+    assert np.array_equal(h0[:n], h1[:n]) and np.array_equal(x0, x1)
+    assert np.array_equal(h0[:n], h2[:n]) and np.array_equal(x0, x2)
+
+
+def test_solve_starts_that_stop_at_once():
+    """The speculative first step is left unused when the solve stops before
+    its first cycle: a zero right-hand side (zero residual: x stays zero, no
+    iteration) and an initial guess that already meets the tolerance (the
+    converged x of a first solve: no iteration, x unchanged); a fresh solve
+    afterwards gives the first solve's x bit for bit."""
+    d, Ad, bd, H, meta, s = _solver("lvl3")
+    x0, st0, _ = s.solve(bd)
+    assert st0.converged == 1 and st0.iterations > 0
+    xz, stz, _ = s.solve(np.zeros(Ad.n))
+    assert stz.iterations == 0 and not np.any(xz)
+    xc, stc, _ = s.solve(bd, x0=x0)
+    assert stc.iterations == 0 and stc.converged == 1 and np.array_equal(xc, x0)
+    x3, st3, _ = s.solve(bd)
+    assert s.error_flag() == 0
+    s.close()
+    assert st3.iterations == st0.iterations and np.array_equal(x3, x0)
+
+
+@pytest.mark.parametrize("name", ["lvl3", "restart", "b4"])
 def test_solution_updates_agree(name):
     """x += Z y with the Arnoldi steps' z_j = M^{-1} v_j kept, against the
     reference's x += M^{-1}(V y): for the FP64 preconditioner (linear) the
