@@ -416,12 +416,14 @@ def _bilu_solve_f16(Ad, lu, dinv, r):
 
 
 @pytest.mark.parametrize("name,prec,s2p", [("c1", 0, 0), ("b4", 0, 0), ("b2", 0, 0), ("lvl3", 0, 0), ("lvl3", 1, 0),
-                                           ("c1", 0, 1), ("lvl3", 0, 2), ("b4", 0, 2), ("b2", 0, 2)])
+                                           ("c1", 0, 1), ("c1", 1, 1), ("lvl3", 0, 2), ("b4", 0, 2), ("b2", 0, 2)])
 def test_bilu0_stage2_vs_oracle(name, prec, s2p):
     """Stage-2 solver type 5: x += (LU)^{-1}(r - A x) with exact BILU(0)
     factors stored in FP64, in FP32 (prec = 1 or stage2_precision s2p = 1),
     or in FP16 with a power-of-two scale per block row and FP32 inverted
-    diagonals (s2p = 2).  The FP16 solve is checked against its restatement
+    diagonals (s2p = 2).  ("c1", 1, 1) is the 16x16x8 mixed configuration:
+    AMG hierarchy and BILU(0) factors in FP32, FP64 GMRES and true residual,
+    reported against the FP64 oracle's iterations.  The FP16 solve is checked against its restatement
     on the rounded factors (1e-10: only the summation order differs) and,
     within 1e-2, against the FP64 factors; its solve reaches rtol within
     three iterations of the oracle's FP64 BILU(0) solve."""
