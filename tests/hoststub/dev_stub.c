@@ -31,9 +31,17 @@ int jxamd_host_free(void *p) { free(p); return 0; }
 int jxamd_host_dev_ptr(void **d, void *h) { *d = h; return 0; }
 int jxamd_last_error(void) { return 0; }
 
+/* failure-path switches set by the driver: bit 0 drops the norms the dot
+ * stand-in writes to pinned memory, bit 1 the MGS stand-in's column sequence
+ * word (a pinned word that never arrives), bit 2 makes the residual form of
+ * the BSR product return its right-hand side (a nonzero residual, so a solve
+ * reaches its Arnoldi loop) */
+int jxamd_stub_faults = 0;
+
 int jxamd_k_bsr_spmv(int b, int nb, int unit, const int *sptr, const int *col, const double *val, const double *x,
                      const double *rhs, double *y, int mode, double alpha, double beta, void *stream)
-{ (void)b; (void)nb; (void)unit; (void)sptr; (void)col; (void)val; (void)x; (void)rhs; (void)y; (void)mode; (void)alpha; (void)beta; (void)stream; return 0; }
+{ if ((jxamd_stub_faults & 4) && mode == 1) for (size_t i = 0; i < (size_t)nb * b; i++) y[i] = rhs[i];
+  (void)unit; (void)sptr; (void)col; (void)val; (void)x; (void)alpha; (void)beta; (void)stream; return 0; }
 int jxamd_k_csr_spmv(int vt, int xt, int n, const int *sptr, const int *col, const void *val, const float *rsc,
                      const void *x, int xs, const void *rhs, int rs, void *y, int mode, void *stream)
 { (void)vt; (void)xt; (void)n; (void)sptr; (void)col; (void)val; (void)rsc; (void)x; (void)xs; (void)rhs; (void)rs; (void)y; (void)mode; (void)stream; return 0; }
@@ -114,7 +122,7 @@ int jxamd_mgs_coop_plan(size_t n, int coop_launch, int *grid, int *tm) { (void)c
 int jxamd_k_mgs_coop(int grid, int tm, int coop_launch, size_t n, const double *V, size_t ld, int m, double *w,
                      double *hval, double *part, unsigned int *cnt, int *flag, int base, int *err, double *hhost,
                      int spin_limit, int hpos, double hseq, void *stream)
-{ if (hhost && hpos >= 0) hhost[hpos] = hseq; /* the device stand-in writes the column's sequence number at once */
+{ if (hhost && hpos >= 0 && !(jxamd_stub_faults & 2)) hhost[hpos] = hseq; /* the column's sequence number at once */
   (void)grid; (void)tm; (void)coop_launch; (void)n; (void)V; (void)ld; (void)m; (void)w; (void)hval; (void)part; (void)cnt; (void)flag; (void)base; (void)err; (void)hhost; (void)spin_limit; (void)stream; return 0; }
 int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, const double *h_prev, double *part,
                     unsigned int *count, double *out, double *hout, void *stream)
@@ -125,7 +133,7 @@ int jxamd_k_dot_mgs(size_t n, const double *u, double *y, const double *v_prev, 
       a += u[i] * y[i];
   }
   *out = a;
-  if (hout) *hout = a;
+  if (hout && !(jxamd_stub_faults & 1)) *hout = a;
   (void)part; (void)count; (void)stream; return 0; }
 int jxamd_k_lincomb_host(size_t n, int m, const double *V, size_t ld, const double *y, int add, double *w,
                          void *stream)

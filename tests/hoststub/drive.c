@@ -75,6 +75,39 @@ int main(int argc, char **argv)
                     fprintf(stderr, "chunk %d prec %d stage2 %d: rc %d\n", chunks[ci], prec, s2, rc);
                 }
             }
+    /* pinned words that never arrive (the norms, then an Arnoldi column):
+     * each solve ends with JXAMD_ERROR_GENERIC after its bounded wait, and
+     * the next solve on the same solver succeeds */
+    {
+        extern int jxamd_stub_faults;
+        const int faults[2] = {1, 2 | 4};
+        for (int f = 0; f < 2; f++) {
+            jxamd_params p;
+            jxamd_params_default(&p);
+            p.max_iter = 3;
+            jxamd_solver *s = NULL;
+            jxamd_stats st;
+            int rc = jxamd_solver_create(&s, &A, &H, &p);
+            if (!rc) {
+                jxamd_stub_faults = faults[f];
+                rc = jxamd_solver_solve(s, rhs, x, NULL, &st);
+                jxamd_stub_faults = 0;
+                runs++;
+                if (rc != JXAMD_ERROR_GENERIC) {
+                    fails++;
+                    fprintf(stderr, "missing pinned word (faults %d): rc %d, expected %d\n", faults[f], rc,
+                            JXAMD_ERROR_GENERIC);
+                }
+                rc = jxamd_solver_solve(s, rhs, x, NULL, &st);
+            }
+            runs++;
+            if (rc) {
+                fails++;
+                fprintf(stderr, "solve after a missing pinned word (faults %d): rc %d\n", faults[f], rc);
+            }
+            if (s) jxamd_solver_destroy(s);
+        }
+    }
     jxamd_hierarchy_free(&H);
     free(ia); free(ja); free(a); free(rhs); free(x); free(pia); free(pja); free(pa);
     printf("levels %d runs %d fails %d\n", H.num_levels, runs, fails);
